@@ -1,0 +1,171 @@
+// Attribute interpolation (forward, screen-space derivatives, backward).
+// Pixel formula (diffrp/rendering/interpolator.py:43-45):
+//   out = (a0 - a2) * u + ((a1 - a2) * v + a2)
+#include "drp_common.h"
+#include "drp_kernels.h"
+
+namespace drp {
+
+struct InterpPix {
+    bool hit;
+    int i0, i1, i2;
+    float u, v;
+};
+
+__device__ __forceinline__ InterpPix interp_pix(const float4 *__restrict__ rast, const int *__restrict__ tri,
+                                                long long pix, int F, int V) {
+    InterpPix p;
+    p.hit = false;
+    float4 r = rast[pix];
+    if (!(r.w > 0.f)) return p;
+    int t = float_to_triidx(r.w) - 1;
+    if (t < 0 || t >= F) return p;
+    const int *t3 = tri + 3 * (long long)t;
+    p.i0 = t3[0]; p.i1 = t3[1]; p.i2 = t3[2];
+    if ((unsigned)p.i0 >= (unsigned)V || (unsigned)p.i1 >= (unsigned)V || (unsigned)p.i2 >= (unsigned)V) return p;
+    p.u = r.x;
+    p.v = r.y;
+    p.hit = true;
+    return p;
+}
+
+// one thread per pixel; C channels per thread (C is small: 1..64 typical)
+template <int CV>  // CV > 0: compile-time channel count, 0: runtime
+__global__ void __launch_bounds__(256) k_interp_fwd(const float *__restrict__ attr, int V, int C, int attr_batched,
+                                                    const float4 *__restrict__ rast, const int *__restrict__ tri,
+                                                    int F, long long total_pix, long long pix_per_img,
+                                                    float *__restrict__ out, const float4 *__restrict__ rast_db,
+                                                    const int *__restrict__ diff_idx, int n_diff,
+                                                    float *__restrict__ out_da) {
+    const int NC = CV > 0 ? CV : C;
+    long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= total_pix) return;
+    InterpPix p = interp_pix(rast, tri, gid, F, V);
+    float *o = out + gid * NC;
+    if (!p.hit) {
+        for (int ch = 0; ch < NC; ++ch) o[ch] = 0.f;
+        if (out_da)
+            for (int j = 0; j < 2 * n_diff; ++j) out_da[gid * 2 * n_diff + j] = 0.f;
+        return;
+    }
+    const float *A = attr + (attr_batched ? (gid / pix_per_img) * (long long)V * NC : 0);
+    const float *a0 = A + (long long)p.i0 * NC, *a1 = A + (long long)p.i1 * NC, *a2 = A + (long long)p.i2 * NC;
+    if (CV > 0 && (CV % 4) == 0) {
+        for (int ch = 0; ch < NC; ch += 4) {
+            float4 x0 = *reinterpret_cast<const float4 *>(a0 + ch);
+            float4 x1 = *reinterpret_cast<const float4 *>(a1 + ch);
+            float4 x2 = *reinterpret_cast<const float4 *>(a2 + ch);
+            float4 y;
+            y.x = fma_(x0.x - x2.x, p.u, fma_(x1.x - x2.x, p.v, x2.x));
+            y.y = fma_(x0.y - x2.y, p.u, fma_(x1.y - x2.y, p.v, x2.y));
+            y.z = fma_(x0.z - x2.z, p.u, fma_(x1.z - x2.z, p.v, x2.z));
+            y.w = fma_(x0.w - x2.w, p.u, fma_(x1.w - x2.w, p.v, x2.w));
+            *reinterpret_cast<float4 *>(o + ch) = y;
+        }
+    } else {
+        for (int ch = 0; ch < NC; ++ch) {
+            float x2 = a2[ch];
+            o[ch] = fma_(a0[ch] - x2, p.u, fma_(a1[ch] - x2, p.v, x2));
+        }
+    }
+    if (out_da) {
+        float4 db = rast_db[gid];
+        float *od = out_da + gid * 2 * n_diff;
+        for (int j = 0; j < n_diff; ++j) {
+            int ch = diff_idx[j];
+            if ((unsigned)ch >= (unsigned)NC) { od[2 * j] = od[2 * j + 1] = 0.f; continue; }
+            float d0 = a0[ch] - a2[ch], d1 = a1[ch] - a2[ch];
+            od[2 * j + 0] = fma_(d0, db.x, d1 * db.z);
+            od[2 * j + 1] = fma_(d0, db.y, d1 * db.w);
+        }
+    }
+}
+
+// backward: grad_attr scatter (atomics), grad_rast (u, v) written per pixel
+template <int CV>
+__global__ void __launch_bounds__(256) k_interp_bwd(const float *__restrict__ attr, int V, int C, int attr_batched,
+                                                    const float4 *__restrict__ rast, const int *__restrict__ tri,
+                                                    int F, long long total_pix, long long pix_per_img,
+                                                    const float *__restrict__ grad_out, float *__restrict__ grad_attr,
+                                                    float4 *__restrict__ grad_rast, const float4 *__restrict__ rast_db,
+                                                    const int *__restrict__ diff_idx, int n_diff,
+                                                    const float *__restrict__ grad_da, float4 *__restrict__ grad_db) {
+    const int NC = CV > 0 ? CV : C;
+    long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= total_pix) return;
+    InterpPix p = interp_pix(rast, tri, gid, F, V);
+    if (!p.hit) {
+        grad_rast[gid] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (grad_db) grad_db[gid] = make_float4(0.f, 0.f, 0.f, 0.f);
+        return;
+    }
+    long long aoff = attr_batched ? (gid / pix_per_img) * (long long)V * NC : 0;
+    const float *A = attr + aoff;
+    float *GA = grad_attr + aoff;
+    const float *a0 = A + (long long)p.i0 * NC, *a1 = A + (long long)p.i1 * NC, *a2 = A + (long long)p.i2 * NC;
+    float *g0 = GA + (long long)p.i0 * NC, *g1 = GA + (long long)p.i1 * NC, *g2 = GA + (long long)p.i2 * NC;
+    const float *g = grad_out + gid * NC;
+    float w2 = (1.f - p.u) - p.v;
+    float su = 0.f, sv = 0.f;
+    for (int ch = 0; ch < NC; ++ch) {
+        float gc = g[ch];
+        if (gc != 0.f) {
+            atomicAdd(g0 + ch, p.u * gc);
+            atomicAdd(g1 + ch, p.v * gc);
+            atomicAdd(g2 + ch, w2 * gc);
+        }
+        float x2 = a2[ch];
+        su += gc * (a0[ch] - x2);
+        sv += gc * (a1[ch] - x2);
+    }
+    grad_rast[gid] = make_float4(su, sv, 0.f, 0.f);
+    if (grad_da && n_diff > 0) {
+        float4 db = rast_db[gid];
+        const float *gd = grad_da + gid * 2 * n_diff;
+        float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+        for (int j = 0; j < n_diff; ++j) {
+            int ch = diff_idx[j];
+            if ((unsigned)ch >= (unsigned)NC) continue;
+            float gx = gd[2 * j], gy = gd[2 * j + 1];
+            float d0 = a0[ch] - a2[ch], d1 = a1[ch] - a2[ch];
+            float k0 = gx * db.x + gy * db.y, k1 = gx * db.z + gy * db.w;
+            atomicAdd(g0 + ch, k0);
+            atomicAdd(g1 + ch, k1);
+            atomicAdd(g2 + ch, -(k0 + k1));
+            s0 += gx * d0; s1 += gy * d0; s2 += gx * d1; s3 += gy * d1;
+        }
+        if (grad_db) grad_db[gid] = make_float4(s0, s1, s2, s3);
+    } else if (grad_db) {
+        grad_db[gid] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+}
+
+#define DRP_INTERP_DISPATCH(KERNEL, ...)                                                          \
+    switch (L.cv) {                                                                               \
+        case 1: hipLaunchKernelGGL(KERNEL<1>, grid, block, 0, st, __VA_ARGS__); break;            \
+        case 2: hipLaunchKernelGGL(KERNEL<2>, grid, block, 0, st, __VA_ARGS__); break;            \
+        case 3: hipLaunchKernelGGL(KERNEL<3>, grid, block, 0, st, __VA_ARGS__); break;            \
+        case 4: hipLaunchKernelGGL(KERNEL<4>, grid, block, 0, st, __VA_ARGS__); break;            \
+        case 8: hipLaunchKernelGGL(KERNEL<8>, grid, block, 0, st, __VA_ARGS__); break;            \
+        case 16: hipLaunchKernelGGL(KERNEL<16>, grid, block, 0, st, __VA_ARGS__); break;          \
+        default: hipLaunchKernelGGL(KERNEL<0>, grid, block, 0, st, __VA_ARGS__); break;           \
+    }
+
+int launch_interpolate(const InterpLaunch &L, hipStream_t st) {
+    if (L.total_pix == 0) return 0;
+    dim3 grid((unsigned)((L.total_pix + 255) / 256)), block(256);
+    DRP_INTERP_DISPATCH(k_interp_fwd, L.attr, L.V, L.C, L.attr_batched, (const float4 *)L.rast, L.tri, L.F,
+                        L.total_pix, L.pix_per_img, L.out, (const float4 *)L.rast_db, L.diff_idx, L.n_diff, L.out_da)
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int launch_interpolate_bwd(const InterpLaunch &L, hipStream_t st) {
+    if (L.total_pix == 0) return 0;
+    dim3 grid((unsigned)((L.total_pix + 255) / 256)), block(256);
+    DRP_INTERP_DISPATCH(k_interp_bwd, L.attr, L.V, L.C, L.attr_batched, (const float4 *)L.rast, L.tri, L.F,
+                        L.total_pix, L.pix_per_img, L.grad_out, L.grad_attr, (float4 *)L.grad_rast,
+                        (const float4 *)L.rast_db, L.diff_idx, L.n_diff, L.grad_da, (float4 *)L.grad_db)
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+}  // namespace drp

@@ -1,0 +1,82 @@
+// Launch descriptors shared by the kernel files and the C-ABI layer.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "drp_common.h"
+
+namespace drp {
+
+struct RasterLaunch {
+    Batch b;
+    const float *pos;
+    const int *tri;
+    int tile_w, tile_h, bin_cap;
+    int *bin_count;   // [n_tiles], immediately followed by list_count [n_img]
+    int *list_count;
+    int *bins;        // [n_tiles * bin_cap]
+    int *lists;       // spill lists
+    int n_tiles;
+    int max_tris;     // largest triangle range of any image
+    const float *prev;
+    float *rast;
+    float *rast_db;
+};
+
+struct RasterBwdLaunch {
+    Batch b;
+    const float *pos;
+    const int *tri;
+    const float *rast, *grad_rast, *grad_db;
+    float *grad_pos;
+    long long total_pix;
+    int pix_per_img;
+};
+
+struct InterpLaunch {
+    const float *attr;
+    int V, C, attr_batched;
+    int cv;           // compiled channel variant (0 = generic loop)
+    const float *rast;
+    const int *tri;
+    int F;
+    long long total_pix, pix_per_img;
+    const float *rast_db;
+    const int *diff_idx;
+    int n_diff;
+    float *out, *out_da;
+    const float *grad_out, *grad_da;
+    float *grad_attr, *grad_rast, *grad_db;
+};
+
+struct TopoLaunch {
+    const int *tri;
+    int F;
+    unsigned long long *keys;
+    int *cnt;
+    int2 *slots;
+    unsigned mask;
+    int *opp;
+};
+
+struct AALaunch {
+    const float *color, *rast, *pos;
+    const int *tri, *opp;
+    int n_img, H, W, C, V, F, pos_batched;
+    int cv;           // compiled channel variant (0 = generic loop)
+    float2 *proj;
+    signed char *orient;
+    float2 *weights;
+    float *out;
+    const float *grad_out;
+    float *grad_color, *grad_pos;
+    float boost;
+};
+
+int launch_rasterize(const RasterLaunch &L, hipStream_t st);
+int launch_rasterize_bwd(const RasterBwdLaunch &L, hipStream_t st);
+int launch_interpolate(const InterpLaunch &L, hipStream_t st);
+int launch_interpolate_bwd(const InterpLaunch &L, hipStream_t st);
+int launch_topology(const TopoLaunch &L, hipStream_t st);
+int launch_antialias(const AALaunch &L, hipStream_t st);
+int launch_antialias_bwd(const AALaunch &L, hipStream_t st);
+
+}  // namespace drp

@@ -1,0 +1,471 @@
+"""
+nvdiffrast.torch-compatible operators on MI355X (gfx950).
+
+This module is the drop-in for the surface diffrp calls
+(``diffrp/rendering/surface_deferred.py:4,209,225,260-265,775``,
+``diffrp/rendering/interpolator.py:12,83``):
+
+    RasterizeCudaContext()                               surface_deferred.py:225
+    RasterizeGLContext                                   surface_deferred.py:209 (type only)
+    DepthPeeler(ctx, pos, tri, resolution, ranges)       surface_deferred.py:260
+      .rasterize_next_layer() -> (rast, rast_db)         surface_deferred.py:262
+    interpolate(attr, rast, tri) -> (out, out_da)        interpolator.py:83
+    antialias(color, rast, pos, tri) -> out              surface_deferred.py:775
+
+plus the rest of nvdiffrast's documented surface the configurations need:
+``rasterize`` (range and instanced modes), ``interpolate(..., rast_db,
+diff_attrs)``, ``antialias(..., topology_hash, pos_gradient_boost)`` and
+``antialias_construct_topology_hash``.  Every operator runs hand-written HIP
+kernels through the C-ABI of ``libdrp_hip.so``; gradients flow through
+``torch.autograd``.  Inputs must be ROCm device tensors: there is no CPU path.
+``install_as_nvdiffrast()`` registers this module as ``nvdiffrast.torch`` so
+diffrp itself can import it unchanged.
+"""
+import sys
+import types
+import weakref
+from typing import Optional, Sequence, Tuple, Union
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import ptr, check, stream_of, require_device
+
+__all__ = [
+    'RasterizeCudaContext', 'RasterizeGLContext', 'RasterizeHipContext', 'DepthPeeler', 'rasterize',
+    'interpolate', 'antialias', 'antialias_construct_topology_hash', 'TopologyHash', 'install_as_nvdiffrast',
+    'get_log_level', 'set_log_level',
+]
+
+_TILES = {(8, 8), (16, 8), (16, 16), (32, 8)}
+
+
+# ----------------------------------------------------------------------------
+# context
+# ----------------------------------------------------------------------------
+class RasterizeCudaContext:
+    """Rasteriser context (``dr.RasterizeCudaContext``, built at ``surface_deferred.py:225``).
+
+    Holds the device and the binning configuration (tile shape, bin capacity).
+    Scratch memory is taken from PyTorch's caching allocator per call, so one
+    context may be shared by threads as diffrp's per-thread cache does.
+    ``tile`` / ``bin_capacity`` = None lets the heuristic (or an autotuned
+    table installed with ``set_tuning``) choose per call shape.
+    """
+
+    def __init__(self, device: Optional[Union[int, str, torch.device]] = None, tile=None, bin_capacity=None):
+        if device is None:
+            device = torch.cuda.current_device()
+        self.device = torch.device('cuda', device) if isinstance(device, int) else torch.device(device)
+        if tile is not None and tuple(tile) not in _TILES:
+            raise ValueError(f'tile must be one of {sorted(_TILES)}')
+        self.tile = None if tile is None else tuple(tile)
+        self.bin_capacity = bin_capacity
+        self._tuned = {}
+        _lib.lib()
+
+    def set_tuning(self, key, tile, bin_capacity):
+        """Install a tuned (tile, bin capacity) for shapes keyed by ``tuning_key``."""
+        self._tuned[key] = (tuple(tile), int(bin_capacity))
+
+    @staticmethod
+    def tuning_key(n_img, H, W, F):
+        return (int(n_img), int(H), int(W), int(F))
+
+    def config(self, n_img, H, W, F) -> Tuple[Tuple[int, int], int]:
+        key = self.tuning_key(n_img, H, W, F)
+        if key in self._tuned and self.tile is None and self.bin_capacity is None:
+            return self._tuned[key]
+        tile = self.tile or default_tile(H, W, F)
+        cap = self.bin_capacity or default_bin_capacity(tile, H, W, F)
+        return tile, cap
+
+
+RasterizeHipContext = RasterizeCudaContext
+
+
+class RasterizeGLContext:
+    """OpenGL contexts do not exist on this platform; the name is kept for type annotations
+    (``surface_deferred.py:209``)."""
+
+    def __init__(self, *args, **kwargs):
+        raise RuntimeError('RasterizeGLContext is not available on MI355X; use RasterizeCudaContext')
+
+
+def default_tile(H, W, F):
+    return (16, 16) if H * W >= 256 * 256 else (8, 8)
+
+
+def default_bin_capacity(tile, H, W, F):
+    # expected triangles per tile for a closed mesh seen from outside (both
+    # windings rasterised), with generous headroom; overflow only costs speed
+    tiles = max(1, ((W + tile[0] - 1) // tile[0]) * ((H + tile[1] - 1) // tile[1]))
+    est = 2.5 * F / tiles + 16
+    cap = 32
+    while cap < est and cap < 4096:
+        cap *= 2
+    return cap
+
+
+_log_level = 1
+
+
+def get_log_level():
+    return _log_level
+
+
+def set_log_level(level):
+    global _log_level
+    _log_level = level
+
+
+# ----------------------------------------------------------------------------
+# argument handling
+# ----------------------------------------------------------------------------
+def _as_f32(t, name):
+    if t.dtype != torch.float32:
+        raise TypeError(f'{name} must be float32 (got {t.dtype})')
+    t = t.contiguous()
+    if t.data_ptr() % 16:
+        t = t.clone()
+    return t
+
+
+def _as_tri(tri):
+    if tri.dtype != torch.int32:
+        raise TypeError(f'tri must be int32 (got {tri.dtype})')
+    if tri.dim() != 2 or tri.shape[1] != 3:
+        raise ValueError(f'tri must have shape [F, 3] (got {tuple(tri.shape)})')
+    return tri.contiguous()
+
+
+class _Ranges:
+    """Range-mode table uploaded to the device: (first, count, spill-list offset) per image."""
+
+    def __init__(self, ranges, F, device):
+        r = torch.as_tensor(ranges)
+        if r.is_cuda:
+            raise ValueError('ranges must be a CPU tensor (as in nvdiffrast)')
+        r = r.to(torch.int64).reshape(-1, 2).numpy()
+        start = np.clip(r[:, 0], 0, F)
+        count = np.clip(r[:, 0] + r[:, 1], 0, F) - start
+        count = np.maximum(count, 0)
+        off = np.concatenate([[0], np.cumsum(count)[:-1]]) if len(count) else np.zeros(0, np.int64)
+        tab = np.stack([r[:, 0], r[:, 1], off], 1).astype(np.int32)
+        self.n = len(tab)
+        self.list_total = int(count.sum())
+        self.max_range = int(count.max()) if len(count) else 0
+        self.dev = torch.from_numpy(tab).to(device)
+
+
+# ----------------------------------------------------------------------------
+# rasterize
+# ----------------------------------------------------------------------------
+def _rasterize_launch(ctx, pos, tri, H, W, rng, prev, want_db):
+    batched = pos.dim() == 3
+    n = pos.shape[0] if batched else (rng.n if rng is not None else 1)
+    V = pos.shape[-2]
+    F = tri.shape[0]
+    dev = pos.device
+    rast = torch.empty((n, H, W, 4), dtype=torch.float32, device=dev)
+    rast_db = torch.empty((n, H, W, 4), dtype=torch.float32, device=dev) if want_db else None
+    if n == 0:
+        return rast, rast_db
+    (tw, th), cap = ctx.config(n, H, W, F)
+    list_total = rng.list_total if rng is not None else n * F
+    max_range = rng.max_range if rng is not None else F
+    L = _lib.lib()
+    ws_bytes = L.drp_rasterize_workspace_size(n, H, W, list_total, tw, th, cap)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+    rc = L.drp_rasterize_fwd(stream_of(pos), ptr(pos), n, V, int(batched), ptr(tri), F,
+                             ptr(rng.dev) if rng is not None else None, H, W, ptr(prev), ptr(rast), ptr(rast_db),
+                             ptr(ws), ws_bytes, tw, th, cap, list_total, max_range)
+    check(rc, 'rasterize')
+    return rast, rast_db
+
+
+class _RasterizeFunc(torch.autograd.Function):
+    @staticmethod
+    def forward(fctx, raster_ctx, pos, tri, H, W, rng, prev, want_db):
+        rast, rast_db = _rasterize_launch(raster_ctx, pos, tri, H, W, rng, prev, want_db)
+        fctx.save_for_backward(pos, tri, rast)
+        fctx.rng = rng
+        fctx.HW = (H, W)
+        fctx.want_db = want_db
+        if rast_db is None:
+            rast_db = torch.empty((rast.shape[0], H, W, 0), dtype=torch.float32, device=pos.device)
+        fctx.mark_non_differentiable(*(() if want_db else (rast_db,)))
+        return rast, rast_db
+
+    @staticmethod
+    def backward(fctx, dy, ddb):
+        pos, tri, rast = fctx.saved_tensors
+        H, W = fctx.HW
+        rng = fctx.rng
+        gpos = torch.zeros_like(pos)
+        dy = _as_f32(dy, 'grad_rast')
+        gdb = _as_f32(ddb, 'grad_db') if (fctx.want_db and ddb is not None and ddb.numel()) else None
+        n = rast.shape[0]
+        rc = _lib.lib().drp_rasterize_bwd(stream_of(pos), ptr(pos), n, pos.shape[-2], int(pos.dim() == 3), ptr(tri),
+                                          tri.shape[0], ptr(rng.dev) if rng is not None else None, H, W, ptr(rast),
+                                          ptr(dy), ptr(gdb), ptr(gpos))
+        check(rc, 'rasterize backward')
+        return None, gpos, None, None, None, None, None, None
+
+
+def _rasterize_impl(glctx, pos, tri, resolution, ranges, grad_db, prev):
+    if not isinstance(glctx, RasterizeCudaContext):
+        raise TypeError('glctx must be a RasterizeCudaContext')
+    require_device(pos, tri, names=('pos', 'tri'))
+    H, W = int(resolution[0]), int(resolution[1])
+    pos = _as_f32(pos, 'pos')
+    tri = _as_tri(tri)
+    if pos.dim() not in (2, 3) or pos.shape[-1] != 4:
+        raise ValueError(f'pos must have shape [V, 4] or [B, V, 4] (got {tuple(pos.shape)})')
+    rng = None
+    if pos.dim() == 2:
+        if ranges is not None:
+            rng = _Ranges(ranges, tri.shape[0], pos.device)
+    elif ranges is not None:
+        raise ValueError('ranges are only valid with 2-D pos (range mode)')
+    return _RasterizeFunc.apply(glctx, pos, tri, H, W, rng, prev, bool(grad_db))
+
+
+def rasterize(glctx, pos, tri, resolution, ranges=None, grad_db=True):
+    """Rasterize triangles (``nvdiffrast.torch.rasterize``).
+
+    pos [V,4] (range mode, ``ranges`` CPU int32 [n,2]) or [B,V,4] (instanced);
+    tri [F,3] int32.  Returns ``(rast, rast_db)``: rast [n,H,W,4] =
+    (u, v, z/w, triangle id + 1), bottom row first; rast_db the pixel-space
+    barycentric derivatives when ``grad_db`` (else an empty [n,H,W,0] tensor).
+    """
+    return _rasterize_impl(glctx, pos, tri, resolution, ranges, grad_db, None)
+
+
+class DepthPeeler:
+    """Depth peeling (``nvdiffrast.torch.DepthPeeler``, used at ``surface_deferred.py:260-265``).
+
+    Each ``rasterize_next_layer()`` returns the nearest fragments strictly
+    behind the previous layer's (by the (z/w, triangle) order).
+    """
+
+    def __init__(self, glctx, pos, tri, resolution, ranges=None, grad_db=True):
+        self.glctx, self.pos, self.tri = glctx, pos, tri
+        self.resolution, self.ranges, self.grad_db = resolution, ranges, grad_db
+        self.prev = None
+        self.active = False
+
+    def __enter__(self):
+        if self.active:
+            raise RuntimeError('DepthPeeler is already active')
+        self.active = True
+        self.prev = None
+        return self
+
+    def __exit__(self, *args):
+        self.active = False
+        self.prev = None
+        return False
+
+    def rasterize_next_layer(self):
+        if not self.active:
+            raise RuntimeError('DepthPeeler must be used as a context manager')
+        prev = None if self.prev is None else self.prev.detach()
+        rast, rast_db = _rasterize_impl(self.glctx, self.pos, self.tri, self.resolution, self.ranges, self.grad_db,
+                                        prev)
+        self.prev = rast
+        return rast, rast_db
+
+
+# ----------------------------------------------------------------------------
+# interpolate
+# ----------------------------------------------------------------------------
+def _attr_layout(attr, rast):
+    if attr.dim() == 2:
+        return 0, attr.shape[0], attr.shape[1]
+    if attr.dim() == 3:
+        if attr.shape[0] != rast.shape[0]:
+            raise ValueError('batched attr must have one attribute buffer per image')
+        return 1, attr.shape[1], attr.shape[2]
+    raise ValueError(f'attr must have shape [V, C] or [n, V, C] (got {tuple(attr.shape)})')
+
+
+class _InterpolateFunc(torch.autograd.Function):
+    @staticmethod
+    def forward(fctx, attr, rast, tri, rast_db, diff_idx):
+        batched, V, C = _attr_layout(attr, rast)
+        n, H, W, _ = rast.shape
+        out = torch.empty((n, H, W, C), dtype=torch.float32, device=attr.device)
+        nd = 0 if diff_idx is None else diff_idx.numel()
+        out_da = torch.empty((n, H, W, 2 * nd), dtype=torch.float32, device=attr.device)
+        rc = _lib.lib().drp_interpolate_fwd(stream_of(attr), ptr(attr), n, V, batched, C, ptr(rast), H, W, ptr(tri),
+                                            tri.shape[0], ptr(rast_db) if nd else None,
+                                            ptr(diff_idx) if nd else None, nd, ptr(out), ptr(out_da) if nd else None)
+        check(rc, 'interpolate')
+        fctx.save_for_backward(attr, rast, tri, rast_db if nd else None, diff_idx if nd else None)
+        fctx.nd = nd
+        if not nd:
+            fctx.mark_non_differentiable(out_da)
+        return out, out_da
+
+    @staticmethod
+    def backward(fctx, dy, dda):
+        attr, rast, tri, rast_db, diff_idx = fctx.saved_tensors
+        batched, V, C = _attr_layout(attr, rast)
+        n, H, W, _ = rast.shape
+        nd = fctx.nd
+        dy = _as_f32(dy, 'grad_out')
+        gattr = torch.zeros_like(attr)
+        grast = torch.empty_like(rast)
+        gdb = torch.empty_like(rast) if nd else None
+        dda = _as_f32(dda, 'grad_da') if (nd and dda is not None) else None
+        if nd and dda is None:
+            dda = torch.zeros((n, H, W, 2 * nd), dtype=torch.float32, device=attr.device)
+        rc = _lib.lib().drp_interpolate_bwd(stream_of(attr), ptr(attr), n, V, batched, C, ptr(rast), H, W, ptr(tri),
+                                            tri.shape[0], ptr(rast_db), ptr(diff_idx), nd, ptr(dy), ptr(dda),
+                                            ptr(gattr), ptr(grast), ptr(gdb))
+        check(rc, 'interpolate backward')
+        return gattr, grast, None, gdb, None
+
+
+def interpolate(attr, rast, tri, rast_db=None, diff_attrs=None):
+    """Interpolate vertex attributes (``nvdiffrast.torch.interpolate``, ``interpolator.py:83``).
+
+    attr [V,C] or [n,V,C]; rast [n,H,W,4]; tri [F,3] int32.  ``diff_attrs``
+    ('all' or a list of channel indices) with ``rast_db`` adds out_da
+    [n,H,W,2*len] = (dA/dX, dA/dY) per listed channel.  Returns ``(out, out_da)``.
+    """
+    require_device(attr, rast, tri, rast_db, names=('attr', 'rast', 'tri', 'rast_db'))
+    attr = _as_f32(attr, 'attr')
+    rast = _as_f32(rast, 'rast')
+    tri = _as_tri(tri)
+    if rast.dim() != 4 or rast.shape[-1] != 4:
+        raise ValueError(f'rast must have shape [n, H, W, 4] (got {tuple(rast.shape)})')
+    diff_idx = None
+    if diff_attrs is not None:
+        C = attr.shape[-1]
+        idx = list(range(C)) if diff_attrs == 'all' else [int(i) for i in diff_attrs]
+        if any(i < 0 or i >= C for i in idx):
+            raise ValueError('diff_attrs index out of range')
+        if rast_db is None or rast_db.numel() == 0:
+            raise ValueError('diff_attrs requires rast_db (rasterize with grad_db=True)')
+        if len(idx):
+            diff_idx = torch.tensor(idx, dtype=torch.int32, device=attr.device)
+            rast_db = _as_f32(rast_db, 'rast_db')
+    return _InterpolateFunc.apply(attr, rast, tri, rast_db if diff_idx is not None else None, diff_idx)
+
+
+# ----------------------------------------------------------------------------
+# antialias
+# ----------------------------------------------------------------------------
+class TopologyHash:
+    """Edge -> opposite-triangle table of a triangle list (``antialias_construct_topology_hash``)."""
+
+    def __init__(self, opp: torch.Tensor, F: int):
+        self.opp = opp
+        self.F = F
+
+
+def _build_topology(tri):
+    F = tri.shape[0]
+    opp = torch.empty((F, 3), dtype=torch.int32, device=tri.device)
+    L = _lib.lib()
+    ws_bytes = L.drp_topology_workspace_size(F)
+    ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=tri.device)
+    check(L.drp_topology_build(stream_of(tri), ptr(tri), F, ptr(opp), ptr(ws), ws_bytes), 'topology hash')
+    return TopologyHash(opp, F)
+
+
+def antialias_construct_topology_hash(tri):
+    require_device(tri, names=('tri',))
+    return _build_topology(_as_tri(tri))
+
+
+# per-tensor reuse of the topology table when antialias is called without one
+# (diffrp never passes it, surface_deferred.py:775); keyed by tensor identity
+# and its in-place version counter
+_topo_cache = {}
+
+
+def _cached_topology(tri: torch.Tensor):
+    key = id(tri)
+    ent = _topo_cache.get(key)
+    if ent is not None and ent[0]() is tri and ent[1] == tri._version:
+        return ent[2]
+    th = _build_topology(tri)
+    _topo_cache[key] = (weakref.ref(tri, lambda _r, k=key: _topo_cache.pop(k, None)), tri._version, th)
+    return th
+
+
+class _AntialiasFunc(torch.autograd.Function):
+    @staticmethod
+    def forward(fctx, color, rast, pos, tri, opp, boost):
+        n, H, W, C = color.shape
+        batched = int(pos.dim() == 3)
+        V, F = pos.shape[-2], tri.shape[0]
+        out = torch.empty_like(color)
+        L = _lib.lib()
+        ws_bytes = L.drp_antialias_workspace_size(n, H, W, V, F, batched)
+        ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=color.device)
+        rc = L.drp_antialias_fwd(stream_of(color), ptr(color), n, H, W, C, ptr(rast), ptr(pos), V, batched, ptr(tri),
+                                 F, ptr(opp), ptr(out), ptr(ws), ws_bytes)
+        check(rc, 'antialias')
+        fctx.save_for_backward(color, rast, pos, tri, opp)
+        fctx.boost = boost
+        return out
+
+    @staticmethod
+    def backward(fctx, dy):
+        color, rast, pos, tri, opp = fctx.saved_tensors
+        n, H, W, C = color.shape
+        batched = int(pos.dim() == 3)
+        V, F = pos.shape[-2], tri.shape[0]
+        dy = _as_f32(dy, 'grad_out')
+        gcol = torch.empty_like(color)
+        gpos = torch.zeros_like(pos) if fctx.needs_input_grad[2] else None
+        L = _lib.lib()
+        ws_bytes = L.drp_antialias_workspace_size(n, H, W, V, F, batched)
+        ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=color.device)
+        rc = L.drp_antialias_bwd(stream_of(color), ptr(color), n, H, W, C, ptr(rast), ptr(pos), V, batched, ptr(tri),
+                                 F, ptr(opp), ptr(dy), ptr(gcol), ptr(gpos), float(fctx.boost), ptr(ws), ws_bytes)
+        check(rc, 'antialias backward')
+        return gcol, None, gpos, None, None, None
+
+
+def antialias(color, rast, pos, tri, topology_hash=None, pos_gradient_boost=1.0):
+    """Silhouette antialiasing with visibility gradients (``nvdiffrast.torch.antialias``,
+    ``surface_deferred.py:775``).
+
+    color [n,H,W,C]; rast [n,H,W,4] (a single layer); pos [V,4] or [n,V,4];
+    tri [F,3].  Differentiable w.r.t. ``color`` and ``pos``.
+    """
+    require_device(color, rast, pos, tri, names=('color', 'rast', 'pos', 'tri'))
+    color = _as_f32(color, 'color')
+    rast = _as_f32(rast, 'rast')
+    pos = _as_f32(pos, 'pos')
+    tri = _as_tri(tri)
+    if color.dim() != 4 or rast.shape[:3] != color.shape[:3]:
+        raise ValueError('color must be [n, H, W, C] matching rast [n, H, W, 4]')
+    if pos.dim() == 3 and pos.shape[0] != color.shape[0]:
+        raise ValueError('batched pos must have one buffer per image')
+    if topology_hash is None:
+        topology_hash = _cached_topology(tri)
+    elif not isinstance(topology_hash, TopologyHash) or topology_hash.F != tri.shape[0]:
+        raise ValueError('topology_hash does not match tri')
+    return _AntialiasFunc.apply(color, rast, pos, tri, topology_hash.opp, float(pos_gradient_boost))
+
+
+# ----------------------------------------------------------------------------
+def install_as_nvdiffrast():
+    """Register this module as ``nvdiffrast`` / ``nvdiffrast.torch`` so that
+    ``import nvdiffrast.torch as dr`` (diffrp's import) resolves here."""
+    mod = sys.modules[__name__]
+    pkg = sys.modules.get('nvdiffrast')
+    if pkg is None:
+        pkg = types.ModuleType('nvdiffrast')
+        pkg.__path__ = []
+        sys.modules['nvdiffrast'] = pkg
+    pkg.torch = mod
+    sys.modules['nvdiffrast.torch'] = mod
+    return mod

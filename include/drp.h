@@ -1,0 +1,110 @@
+/*
+ * drp.h -- C-ABI of the MI355X (gfx950) differentiable rasterisation path.
+ *
+ * Plain pointers and sizes only.  Every pointer except `ranges` host copies is
+ * device memory owned by the caller; work is enqueued on `stream` (a
+ * hipStream_t) with no implicit synchronisation and no allocation, so calls
+ * can be captured into a HIP graph.  Scratch memory is a caller-provided
+ * workspace sized by the matching *_workspace_size query.
+ *
+ * Return value: 0 on success, < 0 on error (message: drp_last_error(), which
+ * is thread-local).
+ *
+ * Layouts (float32 unless noted, all contiguous):
+ *   pos   [V,4] clip space, or [n,V,4] when pos_batched (one buffer per image)
+ *   tri   [F,3] int32 vertex indices
+ *   rast  [n,H,W,4] = (u, v, z/w, id), bottom row first; id is the float
+ *         encoding of (triangle index + 1), 0 = empty
+ *   rast_db [n,H,W,4] = (du/dX, du/dY, dv/dX, dv/dY) in pixel units
+ *   attr  [V,C] or [n,V,C] (attr_batched); out [n,H,W,C]
+ *
+ * Each entry point replaces an operator of the nvdiffrast.torch module that
+ * diffrp calls; citations are the diffrp call sites (paths relative to the
+ * diffrp source tree).
+ */
+#ifndef DRP_H
+#define DRP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#define DRP_ABI_VERSION 1
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Message of the last failed call on this thread. */
+const char *drp_last_error(void);
+int drp_abi_version(void);
+
+/* ---- rasterize -----------------------------------------------------------
+ * Replaces dr.DepthPeeler(ctx, pos, tri, (h, w), ranges).rasterize_next_layer()
+ * (diffrp/rendering/surface_deferred.py:259-265) and dr.rasterize; the context
+ * object dr.RasterizeCudaContext (surface_deferred.py:219-226) has no state
+ * here beyond the caller's workspace.
+ *   ranges_dev: nullptr (all triangles for every image; pos 3-D = instanced),
+ *     or device int32 [n,3] = (first triangle, count, spill-list offset) for
+ *     range mode (pos 2-D); spill-list offsets are prefix sums of the counts and
+ *     list_total their sum; max_range the largest count.
+ *   prev: previous depth layer (depth peeling) or nullptr.
+ *   rast_db: nullptr unless derivatives are wanted.
+ *   tile_w x tile_h in {8x8, 16x8, 16x16, 32x8}; bin_cap >= 1 (performance
+ *   only: full bins spill to a per-image list, a full list to a full scan). */
+size_t drp_rasterize_workspace_size(int n_img, int H, int W, long long list_total, int tile_w, int tile_h,
+                                    int bin_cap);
+int drp_rasterize_fwd(void *stream, const float *pos, int n_img, int V, int pos_batched, const int32_t *tri, int F,
+                      const int32_t *ranges_dev, int H, int W, const float *prev, float *rast, float *rast_db,
+                      void *workspace, size_t workspace_bytes, int tile_w, int tile_h, int bin_cap,
+                      long long list_total, int max_range);
+
+/* Backward of rasterize: autograd of `rast` as diffrp consumes it
+ * (u, v through diffrp/rendering/interpolator.py:42 and :92).  Accumulates
+ * into grad_pos (same layout as pos; caller zero-fills).  grad_db may be null. */
+int drp_rasterize_bwd(void *stream, const float *pos, int n_img, int V, int pos_batched, const int32_t *tri, int F,
+                      const int32_t *ranges_dev, int H, int W, const float *rast, const float *grad_rast,
+                      const float *grad_db, float *grad_pos);
+
+/* ---- interpolate ---------------------------------------------------------
+ * Replaces dr.interpolate(attr, rast, tri[, rast_db, diff_attrs])
+ * (diffrp/rendering/interpolator.py:83).  out_da [n,H,W,2*n_diff] holds
+ * (dA/dX, dA/dY) of the channels listed in diff_idx (n_diff = 0: none). */
+int drp_interpolate_fwd(void *stream, const float *attr, int n_img, int V, int attr_batched, int C,
+                        const float *rast, int H, int W, const int32_t *tri, int F, const float *rast_db,
+                        const int32_t *diff_idx, int n_diff, float *out, float *out_da);
+
+/* Backward: grad_attr accumulated (caller zero-fills), grad_rast (and grad_db
+ * when not null) written; channels 2, 3 of grad_rast are zero. */
+int drp_interpolate_bwd(void *stream, const float *attr, int n_img, int V, int attr_batched, int C,
+                        const float *rast, int H, int W, const int32_t *tri, int F, const float *rast_db,
+                        const int32_t *diff_idx, int n_diff, const float *grad_out, const float *grad_da,
+                        float *grad_attr, float *grad_rast, float *grad_db);
+
+/* ---- antialias -----------------------------------------------------------
+ * Topology table for antialias: replaces dr.antialias_construct_topology_hash
+ * (built implicitly by dr.antialias at diffrp/rendering/surface_deferred.py:775,
+ * which passes no hash).  opp [F,3] int32: triangle across edge (v_k, v_k+1)
+ * of each triangle, -1 for boundary / non-manifold edges. */
+size_t drp_topology_workspace_size(int F);
+int drp_topology_build(void *stream, const int32_t *tri, int F, int32_t *opp, void *workspace,
+                       size_t workspace_bytes);
+
+/* Replaces dr.antialias(color, rast, pos, tri) (diffrp/rendering/
+ * surface_deferred.py:775, via nvdr_antialias :758-775).  color/out [n,H,W,C]. */
+size_t drp_antialias_workspace_size(int n_img, int H, int W, int V, int F, int pos_batched);
+int drp_antialias_fwd(void *stream, const float *color, int n_img, int H, int W, int C, const float *rast,
+                      const float *pos, int V, int pos_batched, const int32_t *tri, int F, const int32_t *opp,
+                      float *out, void *workspace, size_t workspace_bytes);
+
+/* Backward of antialias: grad_color written; grad_pos accumulated (caller
+ * zero-fills; may be null to skip), scaled by pos_gradient_boost. */
+int drp_antialias_bwd(void *stream, const float *color, int n_img, int H, int W, int C, const float *rast,
+                      const float *pos, int V, int pos_batched, const int32_t *tri, int F, const int32_t *opp,
+                      const float *grad_out, float *grad_color, float *grad_pos, float pos_gradient_boost,
+                      void *workspace, size_t workspace_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DRP_H */

@@ -1,0 +1,125 @@
+"""
+diffrp's PyTorch glue on the rasterisation path, restated on torch-CPU -- TEST
+INFRASTRUCTURE ONLY (the oracle of the interpolation / shading side).
+
+Each function follows the cited reference lines; they run on CPU tensors
+(float32 like the reference, or float64 for finite differences).
+"""
+from typing import Callable, List, Optional
+
+import numpy as np
+import torch
+import torch.nn.functional as F
+
+
+def float_to_triidx(x: torch.Tensor) -> torch.Tensor:
+    """diffrp/rendering/interpolator.py:24-25."""
+    return torch.where(x <= 16777216, x.int(), x.view(torch.int32) - 0x4a800000)
+
+
+def triidx_to_float(x: torch.Tensor) -> torch.Tensor:
+    """diffrp/rendering/interpolator.py:28-29."""
+    return torch.where(x <= 0x01000000, x.float(), (0x4a800000 + x).view(torch.float32))
+
+
+def polyfill_interpolate(vertex_buffer, vi_data, tri_idx, empty_region: Optional[float] = None):
+    """diffrp/rendering/interpolator.py:32-65: fma(v1 - v3, u, fma(v2 - v3, v, v3))
+    with fma(x, y, b) = torch.addcmul(b, x, y) (diffrp/utils/shader_ops.py:77-81)."""
+    v123 = vertex_buffer[tri_idx]
+    v1, v2, v3 = v123.select(-2, 0), v123.select(-2, 1), v123.select(-2, 2)
+    u, v = vi_data[..., 0:1], vi_data[..., 1:2]
+    result = torch.addcmul(torch.addcmul(v3, v2 - v3, v), v1 - v3, u)
+    if empty_region is not None:
+        result = torch.where(vi_data[..., -1:] > 0, result, empty_region)
+    return result
+
+
+class MaskedSparseInterpolator:
+    """diffrp/rendering/interpolator.py:87-97."""
+
+    def __init__(self, vi_data, tris, mask):
+        self.tris = tris
+        self.indices = mask.nonzero(as_tuple=True)
+        self.vi_data = vi_data[self.indices]
+        self.tri_idx = tris[float_to_triidx(self.vi_data[..., -1]) - 1].long()
+
+    def interpolate(self, vertex_buffer):
+        return polyfill_interpolate(vertex_buffer, self.vi_data, self.tri_idx)
+
+
+def normalized(x):
+    """diffrp/utils/shader_ops.py:353-360."""
+    return F.normalize(x, dim=-1)
+
+
+def transform_point(xyz, matrix):
+    """diffrp/utils/shader_ops.py:123-136."""
+    return torch.matmul(F.pad(xyz, (0, 1), value=1), matrix.T)
+
+
+def transform_point4x3(xyz, matrix):
+    """diffrp/utils/shader_ops.py:139-152."""
+    return torch.addmm(matrix[:-1, -1], xyz, matrix[:-1, :-1].T)
+
+
+def transform_vector3x3(xyz, matrix):
+    """diffrp/utils/shader_ops.py:171-185."""
+    return torch.matmul(xyz, matrix[:-1, :-1].T)
+
+
+def face_soup_flat(verts, tris):
+    """MeshObject.preprocess with normals='flat' (diffrp/scene/objects.py:87-97,
+    diffrp/utils/geometry.py:5-41): vertex 3f+i = corner i of face f."""
+    fl = tris.long()
+    fn = normalized(torch.linalg.cross(verts[fl[:, 1]] - verts[fl[:, 0]], verts[fl[:, 2]] - verts[fl[:, 0]]))
+    sv = verts[fl].reshape(-1, 3)
+    st = torch.arange(fl.numel(), dtype=torch.int32).reshape(fl.shape)
+    sn = torch.stack([fn] * 3, 1).reshape(-1, 3)
+    return sv, st, sn
+
+
+def gbuffer_collect_single(rast, tris, stencils, values_fn: Callable, default: List[float]):
+    """One-object stencil-masked G-buffer of one layer (surface_deferred.py:283-290,
+    :324-332; mixin.py:130-155): default-filled [1,H,W,C], then index_put of the
+    material's [P,C] values at the object's pixels."""
+    vi_idx = float_to_triidx(rast[..., -1])
+    stencil_buf = stencils[vi_idx]
+    interp = MaskedSparseInterpolator(rast, tris, stencil_buf == 1)
+    H, W = rast.shape[1:3]
+    buf = torch.full([1, H, W, len(default)], default[0], dtype=rast.dtype) if default == [default[0]] * len(default) \
+        else torch.tensor(default, dtype=rast.dtype).expand(1, H, W, len(default))
+    if len(interp.indices[0]) == 0:
+        return buf, interp
+    return buf.index_put(interp.indices, values_fn(interp)), interp
+
+
+def alpha_blend(bgc, bga, fgc, fga):
+    """diffrp/utils/composite.py:28-29."""
+    return torch.lerp(bgc, fgc, fga), torch.addcmul(fga, bga, 1 - fga)
+
+
+def compose_layers(colors, alphas, return_alpha=True):
+    """surface_deferred.py:348-384 with alpha_blend; result is top row first."""
+    fb, fa = colors[-1], alphas[-1]
+    for g, a in zip(reversed(colors[:-1]), reversed(alphas[:-1])):
+        fb, fa = alpha_blend(fb, fa, g, a)
+    out = torch.cat([fb, fa], -1) if return_alpha else fb
+    return torch.flipud(out.squeeze(0))
+
+
+def false_color_camera_space_normal(rast, verts, tris, normals, M, V):
+    """The README scene's output (surface_deferred.py:531-543 via :505-516,
+    :422-428, :418-420, mixin.py:115-128, base_material.py:183-210), opaque,
+    one object, on a rasterised layer ``rast`` [1,H,W,4]."""
+    stencils = torch.tensor([0] + [1] * len(tris), dtype=torch.int32)
+    wn_vert = normalized(transform_vector3x3(normals, M))
+    nrm, _ = gbuffer_collect_single(rast, tris, stencils, lambda it: normalized(it.interpolate(wn_vert)), [0., 0., 0.])
+    alpha, _ = gbuffer_collect_single(rast, tris, stencils, lambda it: torch.ones_like(it.vi_data[..., :1]), [0.])
+    cam = normalized(transform_vector3x3(nrm, V))
+    n = compose_layers([cam], [alpha])
+    return torch.cat([n[..., :3] * 0.5 + 0.5, n[..., 3:]], -1)
+
+
+def to_uint8(x: torch.Tensor) -> np.ndarray:
+    """diffrp/utils/exchange.py:7-18: (saturate(x) * 255).byte()  (truncation)."""
+    return (torch.clamp(x.float(), 0.0, 1.0) * 255).byte().numpy()
