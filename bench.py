@@ -1,0 +1,326 @@
+"""
+Benchmark: differentiable rasterisation fwd+bwd on one MI355X.
+
+Workload (one "step"): the noisy 82k-triangle icosphere (subdivision 6,
+40,962 vertices, radius 0.8, N(0, 0.01) vertex noise, seed 0) rendered from 64
+Fibonacci-sphere cameras at distance 3.2, 512x512, 8 per-vertex float32
+attributes U(0,1) (seed 1):
+
+    pos  = clip_space(verts, VP)                  [64, V, 4]
+    rast = rasterize(pos, tri)                    [64, 512, 512, 4]
+    col  = interpolate(attr, rast, tri)           [64, 512, 512, 8]
+    y    = antialias(col, rast, pos, tri)
+    L    = sum((y - target)^2),  target U(0,1) (seed 2)
+    L.backward()   -> d/d verts, d/d attr
+
+"batched": the whole step captured once in a HIP graph and replayed (one launch
+per operator for all 64 views).  "plain": a Python loop over the 64 views, each
+an eager single-view fwd+bwd (diffrp's one-session-per-view usage).
+Inputs are resident in HBM before the timed region.
+
+Prints ONE JSON line.  ``--gpus`` other than 1 exits: this build measures one GPU.
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+import torch
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from diffrp_amd import nvdr  # noqa: E402
+from diffrp_amd.camera import PerspectiveCamera, clip_space  # noqa: E402
+from diffrp_amd.geometry import fibonacci_directions, noisy_icosphere  # noqa: E402
+
+METRIC = 'fwd+bwd Mpixel/s @512² 82k tris, batched-64-view vs plain; % HBM roofline'
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(n_views, H, W, V, F, C, layers=1):
+    """Bytes each operator must move once (fwd+bwd), per view-layer:
+    (128 + 28 C) N + (96 + 8 C) V + 72 F, plus 16 N per extra peel layer."""
+    N = H * W
+    per_view = (128 + 28 * C) * N + (96 + 8 * C) * V + 72 * F + 16 * N * (layers - 1)
+    return n_views * per_view
+
+
+# per-operator algorithmic bytes for one launch over n views (fwd or bwd half)
+def op_bytes(op, n, N, V, F, C):
+    return {
+        'rasterize_fwd': n * (16 * N + 16 * V + 12 * F),
+        'rasterize_bwd': n * (32 * N + 32 * V + 12 * F),
+        'interpolate_fwd': n * (16 * N + 4 * C * N + 12 * F) + 4 * C * V,
+        'interpolate_bwd': n * (4 * C * N + 16 * N + 16 * N + 12 * F) + 4 * C * V,
+        'antialias_fwd': n * (4 * C * N + 16 * N + 4 * C * N + 16 * V + 12 * F),
+        'antialias_bwd': n * (4 * C * N + 4 * C * N + 16 * N + 4 * C * N + 16 * V + 12 * F + 16 * V),
+    }[op]
+
+
+class Scene:
+    def __init__(self, dev, views, res, subdiv, C, sigma=0.01, radius=0.8):
+        verts, tri = noisy_icosphere(subdiv, radius, sigma, seed=0)
+        self.V, self.F = verts.shape[0], tri.shape[0]
+        self.H = self.W = res
+        self.C = C
+        self.n = views
+        g1 = torch.Generator().manual_seed(1)
+        g2 = torch.Generator().manual_seed(2)
+        attr = torch.rand(self.V, C, generator=g1)
+        target = torch.rand(views, res, res, C, generator=g2)
+        VP = []
+        for d in fibonacci_directions(views):
+            cam = PerspectiveCamera.looking_at_origin_from(d * 3.2, h=res, w=res)
+            VP.append(cam.P('cpu') @ cam.V('cpu'))
+        self.verts = verts.to(dev).requires_grad_(True)
+        self.attr = attr.to(dev).requires_grad_(True)
+        self.tri = tri.to(dev)
+        self.VP = torch.stack(VP).to(dev)
+        self.target = target.to(dev)
+        self.ctx = nvdr.RasterizeCudaContext(dev)
+        self.topo = nvdr.antialias_construct_topology_hash(self.tri)
+        self.verts_cpu, self.tri_cpu, self.attr_cpu, self.VP_cpu = verts, tri, attr, torch.stack(VP)
+        self.target_cpu = target
+
+    def step_batched(self):
+        pos = clip_space(self.verts, self.VP)
+        rast, _ = nvdr.rasterize(self.ctx, pos, self.tri, (self.H, self.W), grad_db=False)
+        col, _ = nvdr.interpolate(self.attr, rast, self.tri)
+        y = nvdr.antialias(col, rast, pos, self.tri, topology_hash=self.topo)
+        loss = ((y - self.target) ** 2).sum()
+        loss.backward()
+        return loss
+
+    def step_plain(self):
+        total = 0.0
+        for v in range(self.n):
+            pos = clip_space(self.verts, self.VP[v])
+            rast, _ = nvdr.rasterize(self.ctx, pos, self.tri, (self.H, self.W), grad_db=False)
+            col, _ = nvdr.interpolate(self.attr, rast, self.tri)
+            y = nvdr.antialias(col, rast, pos, self.tri)
+            loss = ((y - self.target[v:v + 1]) ** 2).sum()
+            loss.backward()
+            total = total + loss.detach()
+        return total
+
+    def zero_grad(self):
+        self.verts.grad = None
+        self.attr.grad = None
+
+
+def time_region(fn, steps, warmup):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def time_events(fn, iters):
+    """Average duration of fn's GPU work on the current stream, HIP events."""
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    fn()
+    torch.cuda.synchronize()
+    e0.record(st)
+    for _ in range(iters):
+        fn()
+    e1.record(st)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / iters * 1e-3
+
+
+def per_op_times(S, iters=10):
+    """Time each operator alone (forward and backward launches) over the 64-view batch."""
+    from diffrp_amd import _lib
+    from diffrp_amd._lib import ptr, stream_of
+    L = _lib.lib()
+    with torch.no_grad():
+        pos = clip_space(S.verts, S.VP).contiguous()
+        rast, _ = nvdr.rasterize(S.ctx, pos, S.tri, (S.H, S.W), grad_db=False)
+        col, _ = nvdr.interpolate(S.attr, rast, S.tri)
+    n, H, W, C, V, F = S.n, S.H, S.W, S.C, S.V, S.F
+    boxes = rast._drp_boxes[0]
+    (tw, th), cap = S.ctx.config(n, H, W, F)
+    rws = L.drp_rasterize_workspace_size(n, H, W, n * F, tw, th, cap)
+    rws_t = torch.empty(rws, dtype=torch.uint8, device=pos.device)
+    rout = torch.empty_like(rast)
+    boxes2 = torch.empty_like(boxes)
+    attr = S.attr.detach()
+    out = torch.empty_like(col)
+    gout = torch.randn_like(col)
+    gattr = torch.zeros_like(attr)
+    grast = torch.empty_like(rast)
+    gpos = torch.zeros_like(pos)
+    gcol = torch.empty_like(col)
+    aws = L.drp_antialias_workspace_size(n, H, W, V, F, 1)
+    aws_t = torch.empty(aws, dtype=torch.uint8, device=pos.device)
+    opp = S.topo.opp
+    off, inc = nvdr.incidence(S.tri, V)
+    vpg = nvdr._views_per_group(n, F)
+    iws = L.drp_interpolate_bwd_workspace_size(n, F, C, 0, vpg)
+    iws_t = torch.empty(iws, dtype=torch.uint8, device=pos.device)
+    pws = L.drp_rasterize_bwd_workspace_size(n, F)
+    pws_t = torch.empty(pws, dtype=torch.uint8, device=pos.device)
+    s = stream_of(pos)
+    fns = {
+        'rasterize_fwd': lambda: L.drp_rasterize_fwd(s, ptr(pos), n, V, 1, ptr(S.tri), F, None, H, W, None, ptr(rout),
+                                                     None, ptr(rws_t), rws, tw, th, cap, n * F, F, ptr(boxes2)),
+        'interpolate_fwd': lambda: L.drp_interpolate_fwd(s, ptr(attr), n, V, 0, C, ptr(rast), H, W, ptr(S.tri), F,
+                                                         None, None, 0, ptr(out), None),
+        'antialias_fwd': lambda: L.drp_antialias_fwd(s, ptr(col), n, H, W, C, ptr(rast), ptr(pos), V, 1, ptr(S.tri),
+                                                     F, ptr(opp), ptr(out), ptr(aws_t), aws),
+        'antialias_bwd': lambda: L.drp_antialias_bwd(s, ptr(col), n, H, W, C, ptr(rast), ptr(pos), V, 1, ptr(S.tri),
+                                                     F, ptr(opp), ptr(gout), ptr(gcol), ptr(gpos), 1.0, ptr(aws_t),
+                                                     aws),
+        'interpolate_bwd': lambda: L.drp_interpolate_bwd(s, ptr(attr), n, V, 0, C, ptr(rast), H, W, ptr(S.tri), F,
+                                                         None, None, 0, ptr(gout), None, ptr(gattr), ptr(grast),
+                                                         None, ptr(boxes), vpg, ptr(off), ptr(inc), ptr(iws_t), iws),
+        'rasterize_bwd': lambda: L.drp_rasterize_bwd(s, ptr(pos), n, V, 1, ptr(S.tri), F, None, H, W, ptr(rast),
+                                                     ptr(grast), None, ptr(gpos), ptr(boxes), ptr(off), ptr(inc),
+                                                     ptr(pws_t), pws),
+    }
+    res = {}
+    for k, f in fns.items():
+        t = time_events(f, iters)
+        res[k] = {'ms': t * 1e3, 'bytes': op_bytes(k, n, H * W, V, F, C)}
+        res[k]['gbs'] = res[k]['bytes'] / t / 1e9
+    return res
+
+
+def cpu_baseline(S, views):
+    """The CPU oracle (scalar C restatement + torch-CPU loss) on a bounded sample of views."""
+    import oracle
+    oracle.build()
+    tri = S.tri_cpu.numpy()
+    opp = oracle.topology(tri)
+    attr = S.attr_cpu.numpy()
+    t0 = time.perf_counter()
+    for v in range(views):
+        pos = clip_space(S.verts_cpu, S.VP_cpu[v]).numpy()
+        rast = oracle.rasterize(pos, tri, (S.H, S.W))
+        col = oracle.interpolate(attr, rast, tri)
+        y = oracle.antialias(col, rast, pos, tri, opp)
+        gy = 2.0 * (y - S.target_cpu[v:v + 1].numpy())
+        gcol, gpos_aa = oracle.antialias_bwd(col, rast, pos, tri, gy, opp)
+        gattr, grast = oracle.interpolate_bwd(attr, rast, tri, gcol)
+        gpos = oracle.rasterize_bwd(pos, tri, rast, grast) + gpos_aa
+        _ = gpos.sum() + gattr.sum()
+    dt = time.perf_counter() - t0
+    return {
+        'value': views * S.H * S.W / dt / 1e6,
+        'unit': 'Mpixel/s',
+        'cores': 1,
+        'kind': 'port',
+        'sample': f'{views} of the 64 views at {S.H}x{S.W}, fwd+bwd, scalar C oracle on one core '
+                  f'({dt:.1f} s; host {platform.processor() or platform.machine()}, '
+                  f'{os.cpu_count()} logical cpus)',
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=20)
+    ap.add_argument('--warmup', type=int, default=3)
+    ap.add_argument('--views', type=int, default=64)
+    ap.add_argument('--res', type=int, default=512)
+    ap.add_argument('--subdiv', type=int, default=6)
+    ap.add_argument('--channels', type=int, default=8)
+    ap.add_argument('--plain-steps', type=int, default=3)
+    ap.add_argument('--cpu-views', type=int, default=6)
+    ap.add_argument('--no-plain', action='store_true')
+    ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--no-ops', action='store_true')
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
+        print('bench.py measures one GPU: the rasterisation path has no multi-GPU mode in this build '
+              '(it would shard by view); run with --gpus 1', file=sys.stderr)
+        sys.exit(2)
+    dev = torch.device('cuda:0')
+    S = Scene(dev, args.views, args.res, args.subdiv, args.channels)
+
+    # eager reference step, then capture the whole fwd+bwd in a HIP graph
+    S.zero_grad()
+    S.step_batched()
+    torch.cuda.synchronize()
+    ref_gv, ref_ga = S.verts.grad.clone(), S.attr.grad.clone()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        for _ in range(2):
+            S.zero_grad()
+            S.step_batched()
+    torch.cuda.current_stream().wait_stream(side)
+    S.zero_grad()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        S.step_batched()
+    # replays must reproduce the eager gradients (checked on several replays)
+    for _ in range(3):
+        graph.replay()
+        torch.cuda.synchronize()
+        for a, b, name in ((S.verts.grad, ref_gv, 'verts'), (S.attr.grad, ref_ga, 'attr')):
+            err = (a - b).abs().max().item() / max(b.abs().max().item(), 1e-30)
+            if err > 1e-4:
+                raise RuntimeError(f'graph replay gradient mismatch on {name}: {err:.3e}')
+
+    t_batched = time_region(graph.replay, args.steps, args.warmup)
+    pix = S.n * S.H * S.W
+    value = pix / t_batched / 1e6
+
+    plain = None
+    if not args.no_plain:
+        def plain_step():
+            S.zero_grad()
+            S.step_plain()
+        t_plain = time_region(plain_step, args.plain_steps, 1)
+        plain = {'mpix_s': pix / t_plain / 1e6, 'ms_per_step': t_plain * 1e3, 'steps': args.plain_steps}
+
+    alg = algorithmic_bytes(S.n, S.H, S.W, S.V, S.F, S.C)
+    ops = None if args.no_ops else per_op_times(S)
+    roof = None
+    if ops:
+        dom = max(ops, key=lambda k: ops[k]['ms'])
+        roof = {'bound': 'hbm', 'kernel': dom, 'achieved': round(ops[dom]['gbs'], 1), 'peak': HBM_PEAK_GBS,
+                'unit': 'GB/s', 'frac': round(ops[dom]['gbs'] / HBM_PEAK_GBS, 4), 'traffic': None,
+                'bytes_per_launch': ops[dom]['bytes'], 'ms_per_launch': round(ops[dom]['ms'], 4)}
+    cpu = None if args.no_cpu else cpu_baseline(S, args.cpu_views)
+
+    out = {
+        'metric': METRIC,
+        'value': round(value, 2),
+        'unit': 'Mpixel/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': round(t_batched * 1e3, 4),
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'f32',
+        'data': 'synthetic',
+        'config': {'workload': f'{S.n}-view fwd+bwd, noisy icosphere s={args.subdiv} ({S.V} verts, {S.F} faces), '
+                               f'{S.H}x{S.W}, {S.C} attribute channels, batched HIP-graph replay',
+                   'views': S.n, 'resolution': [S.H, S.W], 'faces': S.F, 'verts': S.V, 'channels': S.C},
+        'mtri_s': round(S.n * S.F / t_batched / 1e6, 2),
+        'step_roofline': {'algorithmic_bytes': alg, 'achieved_gbs': round(alg / t_batched / 1e9, 1),
+                          'frac': round(alg / t_batched / 1e9 / HBM_PEAK_GBS, 4)},
+        'plain': plain,
+        'batched_vs_plain': round(value / plain['mpix_s'], 2) if plain else None,
+        'ops': {k: {'ms': round(v['ms'], 4), 'gbs': round(v['gbs'], 1)} for k, v in ops.items()} if ops else None,
+        'roofline': roof,
+        'cpu_baseline': cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == '__main__':
+    main()

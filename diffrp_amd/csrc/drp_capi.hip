@@ -41,6 +41,7 @@ int channel_variant(int C) { return (C == 1 || C == 2 || C == 3 || C == 4 || C =
 
 struct RasterWs {
     int *bin_count, *list_count, *bins, *lists;
+    int2 *list_box;
     size_t bytes;
 };
 
@@ -48,13 +49,16 @@ RasterWs raster_ws_layout(void *base, long long n_tiles, int n_img, long long li
     RasterWs w;
     size_t off = 0;
     char *b = (char *)base;
+    size_t lt = (size_t)(list_total > 0 ? list_total : 1);
     w.bin_count = (int *)(b + off);
     w.list_count = w.bin_count + n_tiles;
     off = align256(sizeof(int) * (size_t)(n_tiles + n_img));
     w.bins = (int *)(b + off);
     off += align256(sizeof(int) * (size_t)n_tiles * (size_t)bin_cap);
     w.lists = (int *)(b + off);
-    off += align256(sizeof(int) * (size_t)(list_total > 0 ? list_total : 1));
+    off += align256(sizeof(int) * lt);
+    w.list_box = (int2 *)(b + off);
+    off += align256(sizeof(int2) * lt);
     w.bytes = off;
     return w;
 }
@@ -83,7 +87,7 @@ size_t drp_rasterize_workspace_size(int n_img, int H, int W, long long list_tota
 int drp_rasterize_fwd(void *stream, const float *pos, int n_img, int V, int pos_batched, const int32_t *tri, int F,
                       const int32_t *ranges_dev, int H, int W, const float *prev, float *rast, float *rast_db,
                       void *workspace, size_t workspace_bytes, int tile_w, int tile_h, int bin_cap,
-                      long long list_total, int max_range) {
+                      long long list_total, int max_range, int32_t *boxes) {
     if (int e = check_dims(n_img, H, W)) return e;
     if (n_img == 0) return 0;
     if (V < 0 || F < 0) return fail(-1, "negative V (%d) or F (%d)", V, F);
@@ -124,6 +128,9 @@ int drp_rasterize_fwd(void *stream, const float *pos, int n_img, int V, int pos_
     L.list_count = ws.list_count;
     L.bins = ws.bins;
     L.lists = ws.lists;
+    L.list_box = ws.list_box;
+    L.boxes = (int2 *)boxes;
+    if (boxes && !aligned16(boxes) && ((uintptr_t)boxes & 7)) return fail(-1, "boxes must be 8-byte aligned");
     L.n_tiles = (int)n_tiles;
     L.max_tris = max_range;
     L.prev = prev;
@@ -134,15 +141,32 @@ int drp_rasterize_fwd(void *stream, const float *pos, int n_img, int V, int pos_
     return 0;
 }
 
+size_t drp_rasterize_bwd_workspace_size(int n_img, int F) {
+    if (n_img <= 0 || F <= 0) return 0;
+    return align256((size_t)n_img * (size_t)F * 3 * 16);
+}
+
 int drp_rasterize_bwd(void *stream, const float *pos, int n_img, int V, int pos_batched, const int32_t *tri, int F,
                       const int32_t *ranges_dev, int H, int W, const float *rast, const float *grad_rast,
-                      const float *grad_db, float *grad_pos) {
+                      const float *grad_db, float *grad_pos, const int32_t *boxes, const int32_t *inc_off,
+                      const int32_t *inc, void *workspace, size_t workspace_bytes) {
     if (int e = check_dims(n_img, H, W)) return e;
     if (n_img == 0) return 0;
     if (!pos || !tri || !rast || !grad_rast || !grad_pos) return fail(-1, "null pointer argument");
-    if (!aligned16(pos) || !aligned16(rast) || !aligned16(grad_rast) || (grad_db && !aligned16(grad_db)))
+    if (!aligned16(pos) || !aligned16(rast) || !aligned16(grad_rast) || (grad_db && !aligned16(grad_db)) ||
+        !aligned16(grad_pos))
         return fail(-1, "pos/rast/grad buffers must be 16-byte aligned");
+    bool tri_path = boxes && inc_off && inc && !ranges_dev;
+    if (tri_path && (!workspace || workspace_bytes < drp_rasterize_bwd_workspace_size(n_img, F)))
+        return fail(-1, "workspace too small: need %zu bytes, got %zu", drp_rasterize_bwd_workspace_size(n_img, F),
+                    workspace_bytes);
     RasterBwdLaunch L{};
+    if (tri_path) {
+        L.boxes = (const int2 *)boxes;
+        L.partial = (float *)workspace;
+        L.inc_off = inc_off;
+        L.inc = inc;
+    }
     L.b.n_img = n_img;
     L.b.V = V;
     L.b.F = F;
@@ -196,10 +220,19 @@ int drp_interpolate_fwd(void *stream, const float *attr, int n_img, int V, int a
     return 0;
 }
 
+size_t drp_interpolate_bwd_workspace_size(int n_img, int F, int C, int attr_batched, int views_per_group) {
+    if (n_img <= 0 || F <= 0 || C <= 0) return 0;
+    int vpg = attr_batched ? 1 : (views_per_group < 1 ? 1 : views_per_group);
+    size_t groups = ((size_t)n_img + vpg - 1) / vpg;
+    return align256(groups * (size_t)F * 3 * (size_t)C * 4);
+}
+
 int drp_interpolate_bwd(void *stream, const float *attr, int n_img, int V, int attr_batched, int C,
                         const float *rast, int H, int W, const int32_t *tri, int F, const float *rast_db,
                         const int32_t *diff_idx, int n_diff, const float *grad_out, const float *grad_da,
-                        float *grad_attr, float *grad_rast, float *grad_db) {
+                        float *grad_attr, float *grad_rast, float *grad_db, const int32_t *boxes,
+                        int views_per_group, const int32_t *inc_off, const int32_t *inc, void *workspace,
+                        size_t workspace_bytes) {
     if (int e = check_dims(n_img, H, W)) return e;
     if (n_img == 0) return 0;
     if (C < 1) return fail(-1, "attribute channel count must be >= 1");
@@ -227,8 +260,37 @@ int drp_interpolate_bwd(void *stream, const float *attr, int n_img, int V, int a
     L.grad_attr = grad_attr;
     L.grad_rast = grad_rast;
     L.grad_db = grad_db;
+    L.H = H;
+    L.W = W;
+    L.views_per_group = views_per_group < 1 ? 1 : views_per_group;
+    bool tri_path = boxes && inc_off && inc && n_diff == 0 && L.cv > 0;
+    if (tri_path) {
+        size_t need = drp_interpolate_bwd_workspace_size(n_img, F, C, attr_batched, L.views_per_group);
+        if (!workspace || workspace_bytes < need)
+            return fail(-1, "workspace too small: need %zu bytes, got %zu", need, workspace_bytes);
+        if (C % 4 == 0 && !aligned16(grad_out)) return fail(-1, "grad_out must be 16-byte aligned");
+        L.boxes = (const int2 *)boxes;
+        L.partial = (float *)workspace;
+        L.inc_off = inc_off;
+        L.inc = inc;
+    }
     int rc = launch_interpolate_bwd(L, (hipStream_t)stream);
     if (rc) return fail(rc, "interpolate_bwd launch failed: %s", hipGetErrorString(hipGetLastError()));
+    return 0;
+}
+
+size_t drp_incidence_workspace_size(int V) { return V > 0 ? incidence_ws_bytes(V) : 0; }
+
+int drp_incidence_build(void *stream, const int32_t *tri, int F, int V, int32_t *inc_off, int32_t *inc,
+                        void *workspace, size_t workspace_bytes) {
+    if (F < 0 || V < 0) return fail(-1, "negative F or V");
+    if (F > 0x2aaaaaaa) return fail(-1, "too many triangles");
+    if (V == 0) return 0;
+    if ((F > 0 && !tri) || !inc_off || (F > 0 && !inc)) return fail(-1, "null pointer argument");
+    if (!workspace || workspace_bytes < incidence_ws_bytes(V))
+        return fail(-1, "workspace too small: need %zu bytes, got %zu", incidence_ws_bytes(V), workspace_bytes);
+    int rc = launch_incidence(tri, F, V, inc_off, inc, workspace, (hipStream_t)stream);
+    if (rc) return fail(rc, "incidence launch failed: %s", hipGetErrorString(hipGetLastError()));
     return 0;
 }
 

@@ -159,8 +159,150 @@ int launch_interpolate(const InterpLaunch &L, hipStream_t st) {
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
+// ----------------------------------------------------------------------------
+// triangle-centric backward (no atomics, deterministic)
+//   k_interp_grad_rast: per pixel, grad_rast = (<g, a0-a2>, <g, a1-a2>, 0, 0)
+//   k_interp_bwd_tri:   per (view group, triangle), walk the triangle's pixel box
+//                       in each view and accumulate (u, v, 1-u-v) * g per corner
+//   k_gather_attr:      per vertex, sum the corner partials (CSR, fixed order)
+// ----------------------------------------------------------------------------
+template <int CV>
+__global__ void __launch_bounds__(256) k_interp_grad_rast(const float *__restrict__ attr, int V, int attr_batched,
+                                                          const float4 *__restrict__ rast,
+                                                          const int *__restrict__ tri, int F, long long total_pix,
+                                                          long long pix_per_img, const float *__restrict__ grad_out,
+                                                          float4 *__restrict__ grad_rast) {
+    constexpr int NC = CV;
+    long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= total_pix) return;
+    InterpPix p = interp_pix(rast, tri, gid, F, V);
+    if (!p.hit) {
+        grad_rast[gid] = make_float4(0.f, 0.f, 0.f, 0.f);
+        return;
+    }
+    const float *A = attr + (attr_batched ? (gid / pix_per_img) * (long long)V * NC : 0);
+    const float *a0 = A + (long long)p.i0 * NC, *a1 = A + (long long)p.i1 * NC, *a2 = A + (long long)p.i2 * NC;
+    const float *g = grad_out + gid * NC;
+    float su = 0.f, sv = 0.f;
+#pragma unroll
+    for (int ch = 0; ch < NC; ++ch) {
+        float gc = g[ch], x2 = a2[ch];
+        su += gc * (a0[ch] - x2);
+        sv += gc * (a1[ch] - x2);
+    }
+    grad_rast[gid] = make_float4(su, sv, 0.f, 0.f);
+}
+
+template <int NC>
+__global__ void __launch_bounds__(256) k_interp_bwd_tri(const float4 *__restrict__ rast, int F, int n_img, int H,
+                                                        int W, const float *__restrict__ grad_out,
+                                                        const int2 *__restrict__ boxes, int views_per_group,
+                                                        float *__restrict__ partial) {
+    int grp = blockIdx.y;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= F) return;
+    float acc[3][NC];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int ch = 0; ch < NC; ++ch) acc[k][ch] = 0.f;
+    float want = triidx_to_float(t + 1);
+    int v0 = grp * views_per_group, v1 = min(v0 + views_per_group, n_img);
+    for (int img = v0; img < v1; ++img) {
+        int2 bx = boxes[(long long)img * F + t];
+        int c0 = bx.x & 0xffff, c1 = (unsigned)bx.x >> 16, r0 = bx.y & 0xffff, r1 = (unsigned)bx.y >> 16;
+        long long base = (long long)img * H * W;
+        for (int r = r0; r <= r1; ++r) {
+            const float4 *row = rast + base + (long long)r * W;
+            for (int c = c0; c <= c1; ++c) {
+                float4 rv = row[c];
+                if (rv.w != want) continue;
+                float u = rv.x, v = rv.y, w2 = (1.f - u) - v;
+                const float *g = grad_out + (base + (long long)r * W + c) * NC;
+                float gv[NC];
+                if ((NC % 4) == 0) {
+#pragma unroll
+                    for (int ch = 0; ch < NC; ch += 4) {
+                        float4 q = *reinterpret_cast<const float4 *>(g + ch);
+                        gv[ch] = q.x; gv[ch + 1] = q.y; gv[ch + 2] = q.z; gv[ch + 3] = q.w;
+                    }
+                } else {
+#pragma unroll
+                    for (int ch = 0; ch < NC; ++ch) gv[ch] = g[ch];
+                }
+#pragma unroll
+                for (int ch = 0; ch < NC; ++ch) {
+                    acc[0][ch] += u * gv[ch];
+                    acc[1][ch] += v * gv[ch];
+                    acc[2][ch] += w2 * gv[ch];
+                }
+            }
+        }
+    }
+    float *P = partial + ((long long)grp * F + t) * 3 * NC;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int ch = 0; ch < NC; ++ch) P[k * NC + ch] = acc[k][ch];
+}
+
+// grad_attr[out][v] += sum over groups g in the output's range and incident corners
+template <int NC>
+__global__ void __launch_bounds__(256) k_gather_attr(const float *__restrict__ partial, int n_groups, int F, int V,
+                                                     int per_output_groups, const int *__restrict__ inc_off,
+                                                     const int *__restrict__ inc, float *__restrict__ grad_attr) {
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    int out = blockIdx.y;
+    if (v >= V) return;
+    float s[NC];
+#pragma unroll
+    for (int ch = 0; ch < NC; ++ch) s[ch] = 0.f;
+    int i0 = inc_off[v], i1 = inc_off[v + 1];
+    int g0 = per_output_groups ? out : 0, g1 = per_output_groups ? out + 1 : n_groups;
+    for (int g = g0; g < g1; ++g) {
+        const float *P = partial + (long long)g * F * 3 * NC;
+        for (int i = i0; i < i1; ++i) {
+            const float *q = P + (long long)inc[i] * NC;
+#pragma unroll
+            for (int ch = 0; ch < NC; ++ch) s[ch] += q[ch];
+        }
+    }
+    float *o = grad_attr + ((long long)out * V + v) * NC;
+#pragma unroll
+    for (int ch = 0; ch < NC; ++ch) o[ch] += s[ch];
+}
+
+template <int NC>
+static void launch_interp_bwd_tri_t(const InterpLaunch &L, hipStream_t st) {
+    long long total = L.total_pix;
+    hipLaunchKernelGGL(k_interp_grad_rast<NC>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, L.attr, L.V,
+                       L.attr_batched, (const float4 *)L.rast, L.tri, L.F, total, L.pix_per_img, L.grad_out,
+                       (float4 *)L.grad_rast);
+    int n_img = (int)(total / L.pix_per_img);
+    int vpg = L.attr_batched ? 1 : L.views_per_group;
+    int groups = (n_img + vpg - 1) / vpg;
+    hipLaunchKernelGGL(k_interp_bwd_tri<NC>, dim3((L.F + 255) / 256, groups), dim3(256), 0, st,
+                       (const float4 *)L.rast, L.F, n_img, L.H, L.W, L.grad_out, (const int2 *)L.boxes, vpg,
+                       L.partial);
+    int outs = L.attr_batched ? n_img : 1;
+    hipLaunchKernelGGL(k_gather_attr<NC>, dim3((L.V + 255) / 256, outs), dim3(256), 0, st, L.partial, groups, L.F,
+                       L.V, L.attr_batched, L.inc_off, L.inc, L.grad_attr);
+}
+
 int launch_interpolate_bwd(const InterpLaunch &L, hipStream_t st) {
     if (L.total_pix == 0) return 0;
+    if (L.boxes && L.partial && L.inc_off && L.inc && L.n_diff == 0 && L.cv > 0) {
+        switch (L.cv) {
+            case 1: launch_interp_bwd_tri_t<1>(L, st); break;
+            case 2: launch_interp_bwd_tri_t<2>(L, st); break;
+            case 3: launch_interp_bwd_tri_t<3>(L, st); break;
+            case 4: launch_interp_bwd_tri_t<4>(L, st); break;
+            case 8: launch_interp_bwd_tri_t<8>(L, st); break;
+            case 16: launch_interp_bwd_tri_t<16>(L, st); break;
+            default: return -1;
+        }
+        return hipGetLastError() == hipSuccess ? 0 : -2;
+    }
     dim3 grid((unsigned)((L.total_pix + 255) / 256)), block(256);
     DRP_INTERP_DISPATCH(k_interp_bwd, L.attr, L.V, L.C, L.attr_batched, (const float4 *)L.rast, L.tri, L.F,
                         L.total_pix, L.pix_per_img, L.grad_out, L.grad_attr, (float4 *)L.grad_rast,

@@ -14,6 +14,8 @@ struct RasterLaunch {
     int *list_count;
     int *bins;        // [n_tiles * bin_cap]
     int *lists;       // spill lists
+    int2 *list_box;   // pixel box of each spill-list entry
+    int2 *boxes;      // optional [n_img, F] pixel box per (image, triangle)
     int n_tiles;
     int max_tris;     // largest triangle range of any image
     const float *prev;
@@ -29,6 +31,10 @@ struct RasterBwdLaunch {
     float *grad_pos;
     long long total_pix;
     int pix_per_img;
+    // triangle-centric path (all set) or per-pixel atomics (any null)
+    const int2 *boxes;
+    float *partial;          // [n_img, F, 3] float4
+    const int *inc_off, *inc;
 };
 
 struct InterpLaunch {
@@ -45,6 +51,12 @@ struct InterpLaunch {
     float *out, *out_da;
     const float *grad_out, *grad_da;
     float *grad_attr, *grad_rast, *grad_db;
+    // triangle-centric backward (all set) or per-pixel atomics (any null)
+    int H, W;
+    const int2 *boxes;
+    float *partial;          // [groups, F, 3, C]
+    const int *inc_off, *inc;
+    int views_per_group;
 };
 
 struct TopoLaunch {
@@ -78,5 +90,7 @@ int launch_interpolate_bwd(const InterpLaunch &L, hipStream_t st);
 int launch_topology(const TopoLaunch &L, hipStream_t st);
 int launch_antialias(const AALaunch &L, hipStream_t st);
 int launch_antialias_bwd(const AALaunch &L, hipStream_t st);
+size_t incidence_ws_bytes(int V);
+int launch_incidence(const int *tri, int F, int V, int *inc_off, int *inc, void *ws, hipStream_t st);
 
 }  // namespace drp

@@ -1,72 +1,163 @@
-// Rasterizer kernels for gfx950: triangle setup + screen-tile binning, per-tile
-// coverage/depth in registers with triangle records staged in LDS, and the
-// backward pass (barycentric -> clip-space position chain).
+// Rasterizer kernels for gfx950.
+//
+// Forward: (1) k_setup_bin, one thread per (image, triangle): setup, per-(image,
+// triangle) pixel box, and wave-aggregated insertion into fixed-capacity
+// screen-tile bins (full bins and large triangles spill, with their box, to a
+// per-image list); (2) k_fine, one workgroup per tile: triangle records staged
+// in LDS, then fragment-parallel coverage -- lanes take (record, pixel of its
+// box inside the tile) candidates from a block prefix sum -- with the depth
+// test resolved by 64-bit LDS atomicMin on (orderable(z/w) << 32 | triangle);
+// (u, v) are recomputed for the winner only.
+//
+// Backward: triangle-centric.  One thread per (image, triangle) walks the
+// triangle's pixel box, accumulates the position gradient of every pixel it
+// won (no atomics), writes a per-corner partial, and a CSR vertex gather sums
+// the partials (deterministic).  A per-pixel atomic kernel handles inputs that
+// come without pixel boxes.
 #include "drp_common.h"
 #include "drp_kernels.h"
 
 namespace drp {
 
-// ----------------------------------------------------------------------------
-// setup + binning: one thread per (image, triangle)
-//   small triangles (<= kMaxBinTiles tiles) go to each tile's bin;
-//   large ones, and ones whose bin is full, go to the image's spill list;
-//   if that list overflows too, every tile of the image scans all triangles.
-// ----------------------------------------------------------------------------
 constexpr int kMaxBinTiles = 16;
 
+__device__ __forceinline__ int2 pack_box(int c0, int c1, int r0, int r1) {
+    return make_int2(c0 | (c1 << 16), r0 | (r1 << 16));
+}
+__device__ __forceinline__ int2 empty_box() { return make_int2(0xffff, 0xffff); }  // c0 = 65535 > c1 = 0
+__device__ __forceinline__ void unpack_box(int2 b, int &c0, int &c1, int &r0, int &r1) {
+    c0 = b.x & 0xffff; c1 = (unsigned)b.x >> 16; r0 = b.y & 0xffff; r1 = (unsigned)b.y >> 16;
+}
+
+__device__ __forceinline__ unsigned long long lanemask_lt() {
+    return (1ull << (threadIdx.x & 63)) - 1ull;
+}
+
+// ----------------------------------------------------------------------------
+// setup + binning
+// ----------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_setup_bin(Batch b, const float *__restrict__ pos,
                                                    const int *__restrict__ tri, int tile_w, int tile_h,
                                                    int bin_cap, int *__restrict__ bin_count,
                                                    int *__restrict__ bins, int *__restrict__ list_count,
-                                                   int *__restrict__ lists) {
+                                                   int *__restrict__ lists, int2 *__restrict__ list_box,
+                                                   int2 *__restrict__ boxes) {
     int img = blockIdx.y;
     ImageDesc d = image_desc(b, img);
     int t = d.f0 + blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= d.f1) return;
-    TriVerts v;
-    if (!load_tri(pos, tri, t, d, v)) return;
+    bool valid = t < d.f1;
     TriSetup s;
-    setup_tri(v, d.H, d.W, s);
-    if (s.cls == kCull) return;
-    int tx0 = s.c0 / tile_w, tx1 = s.c1 / tile_w, ty0 = s.r0 / tile_h, ty1 = s.r1 / tile_h;
-    int ntiles = (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
-    bool spill = ntiles > kMaxBinTiles;
-    if (!spill) {
-        for (int ty = ty0; ty <= ty1; ++ty)
-            for (int tx = tx0; tx <= tx1; ++tx) {
-                int tile = d.tile_off + ty * d.tiles_x + tx;
-                int slot = atomicAdd(&bin_count[tile], 1);
-                if (slot < bin_cap) bins[(long long)tile * bin_cap + slot] = t;
+    s.cls = kCull;
+    if (valid) {
+        TriVerts v;
+        if (load_tri(pos, tri, t, d, v)) setup_tri(v, d.H, d.W, s);
+        if (boxes) boxes[(long long)img * b.F + t] = s.cls == kCull ? empty_box() : pack_box(s.c0, s.c1, s.r0, s.r1);
+    }
+    bool live = valid && s.cls != kCull;
+    int tx0 = 0, tx1 = -1, ty0 = 0, ty1 = -1, ntiles = 0;
+    if (live) {
+        tx0 = s.c0 / tile_w; tx1 = s.c1 / tile_w; ty0 = s.r0 / tile_h; ty1 = s.r1 / tile_h;
+        ntiles = (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
+    }
+    bool spill = live && ntiles > kMaxBinTiles;
+    int mine = (live && !spill) ? ntiles : 0;
+    int ncols = tx1 - tx0 + 1;
+    const int lane = threadIdx.x & 63;
+    // round k: every lane inserts its k-th tile; lanes hitting the same tile share one atomic
+    for (int k = 0;; ++k) {
+        bool active = k < mine;
+        if (!__any(active)) break;
+        int my_tile = -1;
+        if (active) {
+            int ty = ty0 + k / ncols, tx = tx0 + k % ncols;
+            my_tile = d.tile_off + ty * d.tiles_x + tx;
+        }
+        unsigned long long pending = __ballot(active);
+        while (pending) {
+            int leader = __ffsll((unsigned long long)pending) - 1;
+            int lt = __shfl(my_tile, leader);
+            bool same = active && my_tile == lt;
+            unsigned long long m = __ballot(same);
+            int base = 0;
+            if (lane == leader) base = atomicAdd(&bin_count[lt], __popcll(m));
+            base = __shfl(base, leader);
+            if (same) {
+                int slot = base + __popcll(m & lanemask_lt());
+                if (slot < bin_cap) bins[(long long)lt * bin_cap + slot] = t;
                 else spill = true;
             }
+            pending &= ~m;
+        }
     }
-    if (spill) {
-        int slot = atomicAdd(&list_count[img], 1);
-        if (slot < d.list_cap) lists[(long long)d.list_off + slot] = t;
+    // spill list (per image): wave-aggregated as well
+    unsigned long long sm = __ballot(spill);
+    if (sm) {
+        int leader = __ffsll(sm) - 1;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&list_count[img], __popcll(sm));
+        base = __shfl(base, leader);
+        if (spill) {
+            int slot = base + __popcll(sm & lanemask_lt());
+            if (slot < d.list_cap) {
+                lists[(long long)d.list_off + slot] = t;
+                list_box[(long long)d.list_off + slot] = pack_box(s.c0, s.c1, s.r0, s.r1);
+            }
+        }
     }
 }
 
 // ----------------------------------------------------------------------------
-// fine raster: one workgroup per tile, one thread per pixel
+// fine raster
 // ----------------------------------------------------------------------------
-struct TileRec {
-    int t, cls, r0, r1;
-    long long E0[3];
-    int A[3], B[3];
+struct FineRec {
+    long long E0[3];     // edge functions (with top-left bias) at the tile's first pixel centre
+    int A[3], B[3];      // per-sub-pixel increments in x and y
     TriVerts v;
+    int t, cls;
+    int bx0, by0, bw, bh;  // candidate box inside the tile (tile-local)
 };
+
+template <int NT>
+__device__ __forceinline__ int block_excl_scan(int v, int *wsum, int &total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[wave] = x;
+    __syncthreads();
+    int pre = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < NT / 64; ++w) {
+        int s = wsum[w];
+        if (w < wave) pre += s;
+        tot += s;
+    }
+    total = tot;
+    return pre + x - v;
+}
+
+__device__ __forceinline__ float inv_ord_f32(uint32_t k) {
+    return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+}
 
 template <int TW, int TH, bool PEEL, bool DB>
 __global__ void __launch_bounds__(TW *TH) k_fine(Batch b, const float *__restrict__ pos,
                                                  const int *__restrict__ tri, int bin_cap,
                                                  const int *__restrict__ bin_count, const int *__restrict__ bins,
-                                                 const int *__restrict__ list_count,
-                                                 const int *__restrict__ lists, int n_tiles, int n_tiles_pad,
+                                                 const int *__restrict__ list_count, const int *__restrict__ lists,
+                                                 const int2 *__restrict__ list_box, int n_tiles, int n_tiles_pad,
                                                  const float4 *__restrict__ prev, float4 *__restrict__ rast,
                                                  float4 *__restrict__ rast_db) {
     constexpr int NT = TW * TH;
-    __shared__ TileRec rec[NT];
-    __shared__ int nrec;
+    static_assert(NT % 64 == 0, "tile must be a whole number of waves");
+    __shared__ FineRec rec[NT];
+    __shared__ int cstart[NT + 1];
+    __shared__ unsigned long long zbuf[NT];
+    __shared__ unsigned long long pkey[PEEL ? NT : 1];
+    __shared__ int wsum[NT / 64];
     int tile = xcd_remap(blockIdx.x, n_tiles_pad);
     if (tile >= n_tiles) return;
     int img = tile_image(b, tile);
@@ -74,27 +165,20 @@ __global__ void __launch_bounds__(TW *TH) k_fine(Batch b, const float *__restric
     int lt = tile - d.tile_off;
     int tx = lt % d.tiles_x, ty = lt / d.tiles_x;
     int tid = threadIdx.x;
-    int c = tx * TW + (tid % TW), r = ty * TH + (tid / TW);
+    int tc0 = tx * TW, tr0 = ty * TH;
+    int c = tc0 + (tid % TW), r = tr0 + (tid / TW);
     bool inimg = c < d.W && r < d.H;
     long long pix = d.pix_off + (long long)r * d.W + c;
-    // wave footprint (rows) for wave-uniform rejection
-    constexpr int ROWS_PER_WAVE = 64 / TW > 0 ? 64 / TW : 1;
-    int wave_r0 = ty * TH + (tid / 64) * ROWS_PER_WAVE;
-    int wave_r1 = wave_r0 + ROWS_PER_WAVE - 1;
-    int tc0 = tx * TW, tr0 = ty * TH;
-
-    uint64_t prev_key = 0;
-    bool alive = inimg;
-    if (PEEL && inimg) {
-        float4 pv = prev[pix];
-        if (pv.w > 0.f) prev_key = frag_key(pv.z, float_to_triidx(pv.w) - 1);
-        else alive = false;
+    zbuf[tid] = ~0ull;
+    if (PEEL) {
+        unsigned long long pk = ~0ull;  // empty previous layer: nothing can follow
+        if (inimg) {
+            float4 pv = prev[pix];
+            if (pv.w > 0.f) pk = frag_key(pv.z, float_to_triidx(pv.w) - 1);
+        }
+        pkey[tid] = pk;
     }
-    uint64_t best = ~0ull;
-    float bu = 0.f, bv = 0.f, bz = 0.f;
-    float4 bdb = make_float4(0.f, 0.f, 0.f, 0.f);
-    float px = pix_ndc(c, d.W), py = pix_ndc(r, d.H);
-    int oxf = (tid % TW) * kSubpix, oyf = (tid / TW) * kSubpix;
+    int tw_img = min(TW, d.W - tc0), th_img = min(TH, d.H - tr0);  // tile clipped to the image
 
     int cnt = bin_count[tile];
     int lcnt = list_count[img];
@@ -104,126 +188,123 @@ __global__ void __launch_bounds__(TW *TH) k_fine(Batch b, const float *__restric
     int total = nbin + nlist;
 
     for (int base = 0; base < total; base += NT) {
-        if (tid == 0) nrec = 0;
-        __syncthreads();
         int idx = base + tid;
+        int count = 0;
         if (idx < total) {
-            int t = idx < nbin ? bins[(long long)tile * bin_cap + idx]
-                               : (full_scan ? d.f0 + (idx - nbin) : lists[(long long)d.list_off + idx - nbin]);
+            int t;
+            bool maybe = true;
+            if (idx < nbin) {
+                t = bins[(long long)tile * bin_cap + idx];
+            } else if (full_scan) {
+                t = d.f0 + (idx - nbin);
+            } else {
+                long long li = (long long)d.list_off + idx - nbin;
+                t = lists[li];
+                int bc0, bc1, br0, br1;
+                unpack_box(list_box[li], bc0, bc1, br0, br1);
+                maybe = bc1 >= tc0 && bc0 < tc0 + TW && br1 >= tr0 && br0 < tr0 + TH;
+            }
             TriVerts v;
-            if (load_tri(pos, tri, t, d, v)) {
+            if (maybe && load_tri(pos, tri, t, d, v)) {
                 TriSetup s;
                 setup_tri(v, d.H, d.W, s);
-                bool hit = s.cls != kCull && s.c1 >= tc0 && s.c0 < tc0 + TW && s.r1 >= tr0 && s.r0 < tr0 + TH;
-                if (hit) {
-                    int k = atomicAdd(&nrec, 1);
-                    TileRec &R = rec[k];
-                    R.t = t;
-                    R.cls = s.cls;
-                    R.r0 = s.r0;
-                    R.r1 = s.r1;
-                    R.v = v;
-                    if (s.cls == kDirect) {
-                        long long Px = (long long)kSubpix * tc0 + kHalfSub, Py = (long long)kSubpix * tr0 + kHalfSub;
+                if (s.cls != kCull) {
+                    int x0 = max(s.c0 - tc0, 0), x1 = min(s.c1 - tc0, tw_img - 1);
+                    int y0 = max(s.r0 - tr0, 0), y1 = min(s.r1 - tr0, th_img - 1);
+                    if (x0 <= x1 && y0 <= y1) {
+                        FineRec &R = rec[tid];
+                        R.t = t;
+                        R.cls = s.cls;
+                        R.v = v;
+                        R.bx0 = x0;
+                        R.by0 = y0;
+                        R.bw = x1 - x0 + 1;
+                        R.bh = y1 - y0 + 1;
+                        count = R.bw * R.bh;
+                        if (s.cls == kDirect) {
+                            long long Px = (long long)kSubpix * tc0 + kHalfSub, Py = (long long)kSubpix * tr0 + kHalfSub;
 #pragma unroll
-                        for (int e = 0; e < 3; ++e) {
-                            int j = e == 2 ? 0 : e + 1;
-                            int dx = s.X[j] - s.X[e], dy = s.Y[j] - s.Y[e];
-                            R.E0[e] = (long long)dx * (Py - s.Y[e]) - (long long)dy * (Px - s.X[e]) - (s.tl[e] ? 0 : 1);
-                            R.A[e] = -dy;
-                            R.B[e] = dx;
+                            for (int e = 0; e < 3; ++e) {
+                                int j = e == 2 ? 0 : e + 1;
+                                int dx = s.X[j] - s.X[e], dy = s.Y[j] - s.Y[e];
+                                R.E0[e] = (long long)dx * (Py - s.Y[e]) - (long long)dy * (Px - s.X[e]) -
+                                          (s.tl[e] ? 0 : 1);
+                                R.A[e] = -dy;
+                                R.B[e] = dx;
+                            }
                         }
                     }
                 }
             }
         }
+        int ntot;
+        int ex = block_excl_scan<NT>(count, wsum, ntot);
+        cstart[tid] = ex;
+        if (tid == NT - 1) cstart[NT] = ex + count;
         __syncthreads();
-        int n = nrec;
-        for (int k = 0; k < n; ++k) {
-            // wave-uniform row rejection
-            int rr0 = __builtin_amdgcn_readfirstlane(rec[k].r0);
-            int rr1 = __builtin_amdgcn_readfirstlane(rec[k].r1);
-            if (rr1 < wave_r0 || rr0 > wave_r1) continue;
-            if (!alive) continue;
-            const TileRec &R = rec[k];
-            bool cov;
-            if (R.cls == kDirect) {
-                long long E0 = R.E0[0] + (long long)R.A[0] * oxf + (long long)R.B[0] * oyf;
-                long long E1 = R.E0[1] + (long long)R.A[1] * oxf + (long long)R.B[1] * oyf;
-                long long E2 = R.E0[2] + (long long)R.A[2] * oxf + (long long)R.B[2] * oyf;
+        for (int i = tid; i < ntot; i += NT) {
+            // record k: the last slot whose start is <= i (empty slots share starts)
+            int lo = 0, hi = NT - 1;
+            while (lo < hi) {
+                int mid = (lo + hi + 1) >> 1;
+                if (cstart[mid] <= i) lo = mid;
+                else hi = mid - 1;
+            }
+            const FineRec &R = rec[lo];
+            int local = i - cstart[lo];
+            int bw = R.bw;
+            int dy = local / bw, dx = local - dy * bw;
+            int lx = R.bx0 + dx, ly = R.by0 + dy;
+            int cls = R.cls;
+            bool cov = true;
+            if (cls == kDirect) {
+                int ox = lx * kSubpix, oy = ly * kSubpix;
+                long long E0 = R.E0[0] + (long long)R.A[0] * ox + (long long)R.B[0] * oy;
+                long long E1 = R.E0[1] + (long long)R.A[1] * ox + (long long)R.B[1] * oy;
+                long long E2 = R.E0[2] + (long long)R.A[2] * ox + (long long)R.B[2] * oy;
                 cov = (E0 | E1 | E2) >= 0;
-            } else {
-                cov = true;  // decided by the homogeneous test below
             }
             if (!cov) continue;
             TriVerts v = R.v;
+            float px = pix_ndc(tc0 + lx, d.W), py = pix_ndc(tr0 + ly, d.H);
             PixEval p;
             eval_pixel(v, px, py, p);
-            if (R.cls == kHomog && !homog_inside(p)) continue;
+            if (cls == kHomog && !homog_inside(p)) continue;
             if (p.s == 0.f) continue;
             float zw = p.zn / p.wn;
             if (!(zw >= -1.f && zw <= 1.f)) continue;
-            uint64_t key = frag_key(zw, R.t);
-            if (PEEL && key <= prev_key) continue;
-            if (key < best) {
-                best = key;
-                bz = zw;
-                bu = p.e[0] / p.s;
-                bv = p.e[1] / p.s;
-                if (DB) bdb = bary_db(v, p, bu, bv, d.H, d.W);
-            }
+            unsigned long long key = frag_key(zw, R.t);
+            int q = ly * TW + lx;
+            if (PEEL && key <= pkey[q]) continue;
+            atomicMin(&zbuf[q], key);
         }
         __syncthreads();
     }
     if (!inimg) return;
-    float4 o;
-    if (best != ~0ull) {
-        int t = (int)(uint32_t)(best & 0xffffffffu);
-        o = make_float4(bu, bv, bz, triidx_to_float(t + 1));
-    } else {
-        o = make_float4(0.f, 0.f, 0.f, 0.f);
-        bdb = make_float4(0.f, 0.f, 0.f, 0.f);
+    unsigned long long key = zbuf[tid];
+    if (key == ~0ull) {
+        rast[pix] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (DB) rast_db[pix] = make_float4(0.f, 0.f, 0.f, 0.f);
+        return;
     }
-    rast[pix] = o;
-    if (DB) rast_db[pix] = bdb;
+    int t = (int)(uint32_t)(key & 0xffffffffu);
+    float zw = inv_ord_f32((uint32_t)(key >> 32));
+    TriVerts v;
+    load_tri(pos, tri, t, d, v);
+    float px = pix_ndc(c, d.W), py = pix_ndc(r, d.H);
+    PixEval p;
+    eval_pixel(v, px, py, p);
+    float u = p.e[0] / p.s, vv = p.e[1] / p.s;
+    rast[pix] = make_float4(u, vv, zw, triidx_to_float(t + 1));
+    if (DB) rast_db[pix] = bary_db(v, p, u, vv, d.H, d.W);
 }
 
 // ----------------------------------------------------------------------------
-// rasterize backward: one thread per pixel, gradients scattered to the three
-// vertices of the pixel's triangle
+// backward math of one pixel: d(u, v, z/w [, db]) -> d(x, y, z, w) of the corners
 // ----------------------------------------------------------------------------
 template <bool DB>
-__global__ void __launch_bounds__(256) k_raster_bwd(Batch b, const float *__restrict__ pos,
-                                                    const int *__restrict__ tri, const float4 *__restrict__ rast,
-                                                    const float4 *__restrict__ grad_rast,
-                                                    const float4 *__restrict__ grad_db, float *__restrict__ grad_pos,
-                                                    long long total_pix, int pix_per_img) {
-    long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= total_pix) return;
-    int img;
-    ImageDesc d;
-    if (b.descs) {
-        // uneven batch: locate the image by its pixel range (images are few)
-        img = 0;
-        while (img + 1 < b.n_img && b.descs[img + 1].pix_off <= gid) ++img;
-        d = b.descs[img];
-    } else {
-        img = (int)(gid / pix_per_img);
-        d = image_desc(b, img);
-    }
-    float4 rv = rast[gid];
-    if (!(rv.w > 0.f)) return;
-    int t = float_to_triidx(rv.w) - 1;
-    if (t < 0 || t >= b.F) return;
-    float4 g = grad_rast[gid];
-    float4 gd = DB ? grad_db[gid] : make_float4(0.f, 0.f, 0.f, 0.f);
-    if (g.x == 0.f && g.y == 0.f && g.z == 0.f && (!DB || (gd.x == 0.f && gd.y == 0.f && gd.z == 0.f && gd.w == 0.f)))
-        return;
-    TriVerts v;
-    if (!load_tri(pos, tri, t, d, v)) return;
-    long long q = gid - d.pix_off;
-    int r = (int)(q / d.W), c = (int)(q % d.W);
-    float px = pix_ndc(c, d.W), py = pix_ndc(r, d.H);
+__device__ __forceinline__ void raster_pixel_grad(const TriVerts &v, float px, float py, float4 g, float4 gd,
+                                                  float4 out[3]) {
     PixEval p;
     eval_pixel(v, px, py, p);
     float u = p.e[0] / p.s, vv = p.e[1] / p.s, zw = p.zn / p.wn;
@@ -238,9 +319,8 @@ __global__ void __launch_bounds__(256) k_raster_bwd(Batch b, const float *__rest
         ey[1] = fma_(v.w[2], p.X[0], -(v.w[0] * p.X[2]));
         ey[2] = fma_(v.w[0], p.X[1], -(v.w[1] * p.X[0]));
         float sx = ex[0] + ex[1] + ex[2], sy = ey[0] + ey[1] + ey[2];
-        float kx = 2.f / (float)d.W, ky = 2.f / (float)d.H;
         float Aux = ex[0] - u * sx, Auy = ey[0] - u * sy, Avx = ex[1] - vv * sx, Avy = ey[1] - vv * sy;
-        float a = gd.x * kx / p.s, bb = gd.y * ky / p.s, cc = gd.z * kx / p.s, dd = gd.w * ky / p.s;
+        float a = gd.x / p.s, bb = gd.y / p.s, cc = gd.z / p.s, dd = gd.w / p.s;  // gd pre-scaled by kx / ky
         float gsx = -(a * u + cc * vv), gsy = -(bb * u + dd * vv);
         float gex[3] = {a + gsx, cc + gsx, gsx}, gey[3] = {bb + gsy, dd + gsy, gsy};
         gu += -(a * sx + bb * sy);
@@ -263,18 +343,121 @@ __global__ void __launch_bounds__(256) k_raster_bwd(Batch b, const float *__rest
     gY[0] += G1 * p.X[2] - G2 * p.X[1];
     gY[1] += G2 * p.X[0] - G0 * p.X[2];
     gY[2] += G0 * p.X[1] - G1 * p.X[0];
-    const int *t3 = tri + 3 * (long long)t;
-    float *G = grad_pos + d.pos_off;
     float gzw = gz / p.wn;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        float *gp = G + 4 * (long long)t3[k];
         float ek = p.e[k];
-        atomicAdd(gp + 0, gX[k]);
-        atomicAdd(gp + 1, gY[k]);
-        if (gz != 0.f) atomicAdd(gp + 2, gzw * ek);
-        atomicAdd(gp + 3, gw[k] - px * gX[k] - py * gY[k] - gzw * zw * ek);
+        out[k] = make_float4(gX[k], gY[k], gzw * ek, gw[k] - px * gX[k] - py * gY[k] - gzw * zw * ek);
     }
+}
+
+__device__ __forceinline__ float4 db_grad_scaled(float4 gd, int H, int W) {
+    float kx = 2.f / (float)W, ky = 2.f / (float)H;
+    return make_float4(gd.x * kx, gd.y * ky, gd.z * kx, gd.w * ky);
+}
+
+// per-pixel fallback: atomics into grad_pos
+template <bool DB>
+__global__ void __launch_bounds__(256) k_raster_bwd(Batch b, const float *__restrict__ pos,
+                                                    const int *__restrict__ tri, const float4 *__restrict__ rast,
+                                                    const float4 *__restrict__ grad_rast,
+                                                    const float4 *__restrict__ grad_db, float *__restrict__ grad_pos,
+                                                    long long total_pix, int pix_per_img) {
+    long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= total_pix) return;
+    int img = (int)(gid / pix_per_img);
+    ImageDesc d = image_desc(b, img);
+    float4 rv = rast[gid];
+    if (!(rv.w > 0.f)) return;
+    int t = float_to_triidx(rv.w) - 1;
+    if (t < 0 || t >= b.F) return;
+    float4 g = grad_rast[gid];
+    float4 gd = DB ? db_grad_scaled(grad_db[gid], d.H, d.W) : make_float4(0.f, 0.f, 0.f, 0.f);
+    if (g.x == 0.f && g.y == 0.f && g.z == 0.f && (!DB || (gd.x == 0.f && gd.y == 0.f && gd.z == 0.f && gd.w == 0.f)))
+        return;
+    TriVerts v;
+    if (!load_tri(pos, tri, t, d, v)) return;
+    long long q = gid - d.pix_off;
+    int r = (int)(q / d.W), c = (int)(q % d.W);
+    float4 o[3];
+    raster_pixel_grad<DB>(v, pix_ndc(c, d.W), pix_ndc(r, d.H), g, gd, o);
+    const int *t3 = tri + 3 * (long long)t;
+    float *G = grad_pos + d.pos_off;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        float *gp = G + 4 * (long long)t3[k];
+        atomicAdd(gp + 0, o[k].x);
+        atomicAdd(gp + 1, o[k].y);
+        if (o[k].z != 0.f) atomicAdd(gp + 2, o[k].z);
+        atomicAdd(gp + 3, o[k].w);
+    }
+}
+
+// triangle-centric: one thread per (image, triangle) -> partial[img][t][corner]
+template <bool DB>
+__global__ void __launch_bounds__(256) k_raster_bwd_tri(Batch b, const float *__restrict__ pos,
+                                                        const int *__restrict__ tri,
+                                                        const float4 *__restrict__ rast,
+                                                        const float4 *__restrict__ grad_rast,
+                                                        const float4 *__restrict__ grad_db,
+                                                        const int2 *__restrict__ boxes,
+                                                        float4 *__restrict__ partial) {
+    int img = blockIdx.y;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= b.F) return;
+    ImageDesc d = image_desc(b, img);
+    float4 acc[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) acc[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+    int c0, c1, r0, r1;
+    unpack_box(boxes[(long long)img * b.F + t], c0, c1, r0, r1);
+    TriVerts v;
+    if (c0 <= c1 && load_tri(pos, tri, t, d, v)) {
+        float want = triidx_to_float(t + 1);
+        for (int r = r0; r <= r1; ++r) {
+            float py = pix_ndc(r, d.H);
+            const float4 *row = rast + d.pix_off + (long long)r * d.W;
+            for (int c = c0; c <= c1; ++c) {
+                if (row[c].w != want) continue;
+                long long pix = d.pix_off + (long long)r * d.W + c;
+                float4 g = grad_rast[pix];
+                float4 gd = DB ? db_grad_scaled(grad_db[pix], d.H, d.W) : make_float4(0.f, 0.f, 0.f, 0.f);
+                float4 o[3];
+                raster_pixel_grad<DB>(v, pix_ndc(c, d.W), py, g, gd, o);
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    acc[k].x += o[k].x; acc[k].y += o[k].y; acc[k].z += o[k].z; acc[k].w += o[k].w;
+                }
+            }
+        }
+    }
+    float4 *P = partial + ((long long)img * b.F + t) * 3;
+    P[0] = acc[0];
+    P[1] = acc[1];
+    P[2] = acc[2];
+}
+
+// vertex gather: grad_pos[img][v] += sum over incident corners of partial[pimg][t][corner]
+// (pos_batched: one output per image; else all images sum into one buffer)
+__global__ void __launch_bounds__(256) k_gather_pos(const float4 *__restrict__ partial, int n_img, int F, int V,
+                                                    int pos_batched, const int *__restrict__ inc_off,
+                                                    const int *__restrict__ inc, float4 *__restrict__ grad_pos) {
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    int oimg = blockIdx.y;
+    if (v >= V) return;
+    int i0 = inc_off[v], i1 = inc_off[v + 1];
+    float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+    int ia = pos_batched ? oimg : 0, ib = pos_batched ? oimg + 1 : n_img;
+    for (int img = ia; img < ib; ++img) {
+        const float4 *P = partial + (long long)img * F * 3;
+        for (int i = i0; i < i1; ++i) {
+            float4 q = P[inc[i]];
+            s.x += q.x; s.y += q.y; s.z += q.z; s.w += q.w;
+        }
+    }
+    float4 *o = grad_pos + (long long)oimg * V + v;
+    float4 cur = *o;
+    *o = make_float4(cur.x + s.x, cur.y + s.y, cur.z + s.z, cur.w + s.w);
 }
 
 // ----------------------------------------------------------------------------
@@ -285,28 +468,22 @@ __global__ void k_zero_i32(int *p, long long n) {
     if (i < n) p[i] = 0;
 }
 
+template <int TW, int TH, bool PEEL, bool DB>
+static void launch_fine_v(const RasterLaunch &L, hipStream_t st) {
+    int pad = (L.n_tiles + 7) & ~7;
+    hipLaunchKernelGGL((k_fine<TW, TH, PEEL, DB>), dim3(pad), dim3(TW * TH), 0, st, L.b, L.pos, L.tri, L.bin_cap,
+                       L.bin_count, L.bins, L.list_count, L.lists, L.list_box, L.n_tiles, pad,
+                       (const float4 *)L.prev, (float4 *)L.rast, (float4 *)L.rast_db);
+}
+
 template <int TW, int TH>
 static void launch_fine_t(const RasterLaunch &L, hipStream_t st) {
-    int pad = (L.n_tiles + 7) & ~7;
-    dim3 grid(pad), block(TW * TH);
     if (L.prev) {
-        if (L.rast_db)
-            hipLaunchKernelGGL((k_fine<TW, TH, true, true>), grid, block, 0, st, L.b, L.pos, L.tri, L.bin_cap,
-                               L.bin_count, L.bins, L.list_count, L.lists, L.n_tiles, pad, (const float4 *)L.prev,
-                               (float4 *)L.rast, (float4 *)L.rast_db);
-        else
-            hipLaunchKernelGGL((k_fine<TW, TH, true, false>), grid, block, 0, st, L.b, L.pos, L.tri, L.bin_cap,
-                               L.bin_count, L.bins, L.list_count, L.lists, L.n_tiles, pad, (const float4 *)L.prev,
-                               (float4 *)L.rast, (float4 *)nullptr);
+        if (L.rast_db) launch_fine_v<TW, TH, true, true>(L, st);
+        else launch_fine_v<TW, TH, true, false>(L, st);
     } else {
-        if (L.rast_db)
-            hipLaunchKernelGGL((k_fine<TW, TH, false, true>), grid, block, 0, st, L.b, L.pos, L.tri, L.bin_cap,
-                               L.bin_count, L.bins, L.list_count, L.lists, L.n_tiles, pad, (const float4 *)nullptr,
-                               (float4 *)L.rast, (float4 *)L.rast_db);
-        else
-            hipLaunchKernelGGL((k_fine<TW, TH, false, false>), grid, block, 0, st, L.b, L.pos, L.tri, L.bin_cap,
-                               L.bin_count, L.bins, L.list_count, L.lists, L.n_tiles, pad, (const float4 *)nullptr,
-                               (float4 *)L.rast, (float4 *)nullptr);
+        if (L.rast_db) launch_fine_v<TW, TH, false, true>(L, st);
+        else launch_fine_v<TW, TH, false, false>(L, st);
     }
 }
 
@@ -317,7 +494,7 @@ int launch_rasterize(const RasterLaunch &L, hipStream_t st) {
     if (L.max_tris > 0) {
         dim3 grid((L.max_tris + 255) / 256, L.b.n_img);
         hipLaunchKernelGGL(k_setup_bin, grid, dim3(256), 0, st, L.b, L.pos, L.tri, L.tile_w, L.tile_h, L.bin_cap,
-                           L.bin_count, L.bins, L.list_count, L.lists);
+                           L.bin_count, L.bins, L.list_count, L.lists, L.list_box, L.boxes);
     }
     if (L.tile_w == 8 && L.tile_h == 8) launch_fine_t<8, 8>(L, st);
     else if (L.tile_w == 16 && L.tile_h == 16) launch_fine_t<16, 16>(L, st);
@@ -329,6 +506,22 @@ int launch_rasterize(const RasterLaunch &L, hipStream_t st) {
 
 int launch_rasterize_bwd(const RasterBwdLaunch &L, hipStream_t st) {
     if (L.total_pix == 0) return 0;
+    if (L.boxes && L.partial && L.inc_off && L.inc) {
+        dim3 grid((L.b.F + 255) / 256, L.b.n_img);
+        if (L.grad_db)
+            hipLaunchKernelGGL(k_raster_bwd_tri<true>, grid, dim3(256), 0, st, L.b, L.pos, L.tri,
+                               (const float4 *)L.rast, (const float4 *)L.grad_rast, (const float4 *)L.grad_db,
+                               (const int2 *)L.boxes, (float4 *)L.partial);
+        else
+            hipLaunchKernelGGL(k_raster_bwd_tri<false>, grid, dim3(256), 0, st, L.b, L.pos, L.tri,
+                               (const float4 *)L.rast, (const float4 *)L.grad_rast, (const float4 *)nullptr,
+                               (const int2 *)L.boxes, (float4 *)L.partial);
+        int n_out = L.b.pos_batched ? L.b.n_img : 1;
+        dim3 gg((L.b.V + 255) / 256, n_out);
+        hipLaunchKernelGGL(k_gather_pos, gg, dim3(256), 0, st, (const float4 *)L.partial, L.b.n_img, L.b.F, L.b.V,
+                           L.b.pos_batched, L.inc_off, L.inc, (float4 *)L.grad_pos);
+        return hipGetLastError() == hipSuccess ? 0 : -2;
+    }
     dim3 grid((unsigned)((L.total_pix + 255) / 256));
     if (L.grad_db)
         hipLaunchKernelGGL(k_raster_bwd<true>, grid, dim3(256), 0, st, L.b, L.pos, L.tri, (const float4 *)L.rast,

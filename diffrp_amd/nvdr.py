@@ -170,8 +170,11 @@ def _rasterize_launch(ctx, pos, tri, H, W, rng, prev, want_db):
     dev = pos.device
     rast = torch.empty((n, H, W, 4), dtype=torch.float32, device=dev)
     rast_db = torch.empty((n, H, W, 4), dtype=torch.float32, device=dev) if want_db else None
+    boxes = None
     if n == 0:
-        return rast, rast_db
+        return rast, rast_db, boxes
+    if rng is None and F > 0 and max(H, W) <= 65535:
+        boxes = torch.empty((n, F, 2), dtype=torch.int32, device=dev)
     (tw, th), cap = ctx.config(n, H, W, F)
     list_total = rng.list_total if rng is not None else n * F
     max_range = rng.max_range if rng is not None else F
@@ -180,38 +183,67 @@ def _rasterize_launch(ctx, pos, tri, H, W, rng, prev, want_db):
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
     rc = L.drp_rasterize_fwd(stream_of(pos), ptr(pos), n, V, int(batched), ptr(tri), F,
                              ptr(rng.dev) if rng is not None else None, H, W, ptr(prev), ptr(rast), ptr(rast_db),
-                             ptr(ws), ws_bytes, tw, th, cap, list_total, max_range)
+                             ptr(ws), ws_bytes, tw, th, cap, list_total, max_range, ptr(boxes))
     check(rc, 'rasterize')
-    return rast, rast_db
+    return rast, rast_db, boxes
+
+
+# vertex -> incident corner tables, reused per triangle tensor (identity + version)
+_inc_cache = {}
+
+
+def incidence(tri: torch.Tensor, V: int):
+    """CSR (inc_off [V+1], inc [3F]) of the corners incident to each vertex, built on the GPU."""
+    key = (id(tri), int(V))
+    ent = _inc_cache.get(key)
+    if ent is not None and ent[0]() is tri and ent[1] == tri._version:
+        return ent[2]
+    F = tri.shape[0]
+    off = torch.empty(V + 1, dtype=torch.int32, device=tri.device)
+    inc = torch.empty(max(3 * F, 1), dtype=torch.int32, device=tri.device)
+    L = _lib.lib()
+    ws_bytes = L.drp_incidence_workspace_size(V)
+    ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=tri.device)
+    check(L.drp_incidence_build(stream_of(tri), ptr(tri), F, V, ptr(off), ptr(inc), ptr(ws), ws_bytes), 'incidence')
+    _inc_cache[key] = (weakref.ref(tri, lambda _r, k=key: _inc_cache.pop(k, None)), tri._version, (off, inc))
+    return off, inc
 
 
 class _RasterizeFunc(torch.autograd.Function):
     @staticmethod
-    def forward(fctx, raster_ctx, pos, tri, H, W, rng, prev, want_db):
-        rast, rast_db = _rasterize_launch(raster_ctx, pos, tri, H, W, rng, prev, want_db)
-        fctx.save_for_backward(pos, tri, rast)
+    def forward(fctx, raster_ctx, pos, tri, H, W, rng, prev, want_db, holder):
+        rast, rast_db, boxes = _rasterize_launch(raster_ctx, pos, tri, H, W, rng, prev, want_db)
+        fctx.save_for_backward(pos, tri, rast, boxes)
         fctx.rng = rng
         fctx.HW = (H, W)
         fctx.want_db = want_db
         if rast_db is None:
             rast_db = torch.empty((rast.shape[0], H, W, 0), dtype=torch.float32, device=pos.device)
         fctx.mark_non_differentiable(*(() if want_db else (rast_db,)))
+        holder.append(boxes)
         return rast, rast_db
 
     @staticmethod
     def backward(fctx, dy, ddb):
-        pos, tri, rast = fctx.saved_tensors
+        pos, tri, rast, boxes = fctx.saved_tensors
         H, W = fctx.HW
         rng = fctx.rng
         gpos = torch.zeros_like(pos)
         dy = _as_f32(dy, 'grad_rast')
         gdb = _as_f32(ddb, 'grad_db') if (fctx.want_db and ddb is not None and ddb.numel()) else None
-        n = rast.shape[0]
-        rc = _lib.lib().drp_rasterize_bwd(stream_of(pos), ptr(pos), n, pos.shape[-2], int(pos.dim() == 3), ptr(tri),
-                                          tri.shape[0], ptr(rng.dev) if rng is not None else None, H, W, ptr(rast),
-                                          ptr(dy), ptr(gdb), ptr(gpos))
+        n, V, F = rast.shape[0], pos.shape[-2], tri.shape[0]
+        L = _lib.lib()
+        off = inc = ws = None
+        ws_bytes = 0
+        if boxes is not None:
+            off, inc = incidence(tri, V)
+            ws_bytes = L.drp_rasterize_bwd_workspace_size(n, F)
+            ws = torch.empty(ws_bytes, dtype=torch.uint8, device=pos.device)
+        rc = L.drp_rasterize_bwd(stream_of(pos), ptr(pos), n, V, int(pos.dim() == 3), ptr(tri), F,
+                                 ptr(rng.dev) if rng is not None else None, H, W, ptr(rast), ptr(dy), ptr(gdb),
+                                 ptr(gpos), ptr(boxes), ptr(off), ptr(inc), ptr(ws), ws_bytes)
         check(rc, 'rasterize backward')
-        return None, gpos, None, None, None, None, None, None
+        return None, gpos, None, None, None, None, None, None, None
 
 
 def _rasterize_impl(glctx, pos, tri, resolution, ranges, grad_db, prev):
@@ -226,10 +258,18 @@ def _rasterize_impl(glctx, pos, tri, resolution, ranges, grad_db, prev):
     rng = None
     if pos.dim() == 2:
         if ranges is not None:
-            rng = _Ranges(ranges, tri.shape[0], pos.device)
+            r = torch.as_tensor(ranges).reshape(-1, 2)
+            # one range over every triangle (diffrp's call, surface_deferred.py:259) is the plain case
+            if not (r.shape[0] == 1 and int(r[0, 0]) == 0 and int(r[0, 1]) >= tri.shape[0]):
+                rng = _Ranges(ranges, tri.shape[0], pos.device)
     elif ranges is not None:
         raise ValueError('ranges are only valid with 2-D pos (range mode)')
-    return _RasterizeFunc.apply(glctx, pos, tri, H, W, rng, prev, bool(grad_db))
+    holder = []
+    rast, rast_db = _RasterizeFunc.apply(glctx, pos, tri, H, W, rng, prev, bool(grad_db), holder)
+    if holder and holder[0] is not None:
+        # pixel boxes of this rasterisation, for the triangle-centric interpolate backward
+        rast._drp_boxes = (holder[0], tri.shape[0], rast._version)
+    return rast, rast_db
 
 
 def rasterize(glctx, pos, tri, resolution, ranges=None, grad_db=True):
@@ -291,9 +331,19 @@ def _attr_layout(attr, rast):
     raise ValueError(f'attr must have shape [V, C] or [n, V, C] (got {tuple(attr.shape)})')
 
 
+_TRI_BWD_C = (1, 2, 3, 4, 8, 16)
+
+
+def _views_per_group(n, F):
+    """Views folded into one partial of the triangle-centric attribute backward:
+    enough (triangle, group) threads to fill the GPU, as few partials as that allows."""
+    groups = min(n, max(1, -(-(1 << 19) // max(F, 1))))
+    return -(-n // groups)
+
+
 class _InterpolateFunc(torch.autograd.Function):
     @staticmethod
-    def forward(fctx, attr, rast, tri, rast_db, diff_idx):
+    def forward(fctx, attr, rast, tri, rast_db, diff_idx, boxes):
         batched, V, C = _attr_layout(attr, rast)
         n, H, W, _ = rast.shape
         out = torch.empty((n, H, W, C), dtype=torch.float32, device=attr.device)
@@ -303,7 +353,7 @@ class _InterpolateFunc(torch.autograd.Function):
                                             tri.shape[0], ptr(rast_db) if nd else None,
                                             ptr(diff_idx) if nd else None, nd, ptr(out), ptr(out_da) if nd else None)
         check(rc, 'interpolate')
-        fctx.save_for_backward(attr, rast, tri, rast_db if nd else None, diff_idx if nd else None)
+        fctx.save_for_backward(attr, rast, tri, rast_db if nd else None, diff_idx if nd else None, boxes)
         fctx.nd = nd
         if not nd:
             fctx.mark_non_differentiable(out_da)
@@ -311,9 +361,10 @@ class _InterpolateFunc(torch.autograd.Function):
 
     @staticmethod
     def backward(fctx, dy, dda):
-        attr, rast, tri, rast_db, diff_idx = fctx.saved_tensors
+        attr, rast, tri, rast_db, diff_idx, boxes = fctx.saved_tensors
         batched, V, C = _attr_layout(attr, rast)
         n, H, W, _ = rast.shape
+        F = tri.shape[0]
         nd = fctx.nd
         dy = _as_f32(dy, 'grad_out')
         gattr = torch.zeros_like(attr)
@@ -322,11 +373,21 @@ class _InterpolateFunc(torch.autograd.Function):
         dda = _as_f32(dda, 'grad_da') if (nd and dda is not None) else None
         if nd and dda is None:
             dda = torch.zeros((n, H, W, 2 * nd), dtype=torch.float32, device=attr.device)
-        rc = _lib.lib().drp_interpolate_bwd(stream_of(attr), ptr(attr), n, V, batched, C, ptr(rast), H, W, ptr(tri),
-                                            tri.shape[0], ptr(rast_db), ptr(diff_idx), nd, ptr(dy), ptr(dda),
-                                            ptr(gattr), ptr(grast), ptr(gdb))
+        L = _lib.lib()
+        off = inc = ws = None
+        ws_bytes = vpg = 0
+        if boxes is not None and nd == 0 and C in _TRI_BWD_C:
+            off, inc = incidence(tri, V)
+            vpg = _views_per_group(n, F)
+            ws_bytes = L.drp_interpolate_bwd_workspace_size(n, F, C, batched, vpg)
+            ws = torch.empty(ws_bytes, dtype=torch.uint8, device=attr.device)
+        else:
+            boxes = None
+        rc = L.drp_interpolate_bwd(stream_of(attr), ptr(attr), n, V, batched, C, ptr(rast), H, W, ptr(tri), F,
+                                   ptr(rast_db), ptr(diff_idx), nd, ptr(dy), ptr(dda), ptr(gattr), ptr(grast),
+                                   ptr(gdb), ptr(boxes), vpg, ptr(off), ptr(inc), ptr(ws), ws_bytes)
         check(rc, 'interpolate backward')
-        return gattr, grast, None, gdb, None
+        return gattr, grast, None, gdb, None, None
 
 
 def interpolate(attr, rast, tri, rast_db=None, diff_attrs=None):
@@ -353,7 +414,13 @@ def interpolate(attr, rast, tri, rast_db=None, diff_attrs=None):
         if len(idx):
             diff_idx = torch.tensor(idx, dtype=torch.int32, device=attr.device)
             rast_db = _as_f32(rast_db, 'rast_db')
-    return _InterpolateFunc.apply(attr, rast, tri, rast_db if diff_idx is not None else None, diff_idx)
+    boxes = None
+    meta = getattr(rast, '_drp_boxes', None)
+    # boxes describe where each triangle's pixels lie; valid while rast is unmodified
+    if meta is not None and meta[1] == tri.shape[0] and meta[2] == rast._version and \
+            meta[0].shape[0] == rast.shape[0]:
+        boxes = meta[0]
+    return _InterpolateFunc.apply(attr, rast, tri, rast_db if diff_idx is not None else None, diff_idx, boxes)
 
 
 # ----------------------------------------------------------------------------

@@ -50,20 +50,38 @@ int drp_abi_version(void);
  *   prev: previous depth layer (depth peeling) or nullptr.
  *   rast_db: nullptr unless derivatives are wanted.
  *   tile_w x tile_h in {8x8, 16x8, 16x16, 32x8}; bin_cap >= 1 (performance
- *   only: full bins spill to a per-image list, a full list to a full scan). */
+ *   only: full bins spill to a per-image list, a full list to a full scan).
+ *   boxes (nullable, int32 [n,F,2], 8-byte aligned): packed pixel box of each
+ *   (image, triangle), (c0 | c1 << 16, r0 | r1 << 16), c0 > c1 when culled; the
+ *   triangle-centric backward passes read it.  Only triangles inside an image's
+ *   range are written. */
 size_t drp_rasterize_workspace_size(int n_img, int H, int W, long long list_total, int tile_w, int tile_h,
                                     int bin_cap);
 int drp_rasterize_fwd(void *stream, const float *pos, int n_img, int V, int pos_batched, const int32_t *tri, int F,
                       const int32_t *ranges_dev, int H, int W, const float *prev, float *rast, float *rast_db,
                       void *workspace, size_t workspace_bytes, int tile_w, int tile_h, int bin_cap,
-                      long long list_total, int max_range);
+                      long long list_total, int max_range, int32_t *boxes);
 
 /* Backward of rasterize: autograd of `rast` as diffrp consumes it
  * (u, v through diffrp/rendering/interpolator.py:42 and :92).  Accumulates
- * into grad_pos (same layout as pos; caller zero-fills).  grad_db may be null. */
+ * into grad_pos (same layout as pos; caller zero-fills).  grad_db may be null.
+ * With boxes (from the forward), the incidence table of tri and a workspace
+ * of drp_rasterize_bwd_workspace_size bytes, and no ranges, the gradient is
+ * gathered per triangle and per vertex without atomics (deterministic);
+ * otherwise per-pixel float atomics are used. */
+size_t drp_rasterize_bwd_workspace_size(int n_img, int F);
 int drp_rasterize_bwd(void *stream, const float *pos, int n_img, int V, int pos_batched, const int32_t *tri, int F,
                       const int32_t *ranges_dev, int H, int W, const float *rast, const float *grad_rast,
-                      const float *grad_db, float *grad_pos);
+                      const float *grad_db, float *grad_pos, const int32_t *boxes, const int32_t *inc_off,
+                      const int32_t *inc, void *workspace, size_t workspace_bytes);
+
+/* Vertex -> incident corner table of tri (CSR): inc_off [V+1], inc [3F] with
+ * corner ids 3t+k grouped by vertex in ascending order.  Built once per mesh;
+ * used by the deterministic backward gathers.  No reference counterpart: it is
+ * the gather index of the backward passes that replace scatter atomics. */
+size_t drp_incidence_workspace_size(int V);
+int drp_incidence_build(void *stream, const int32_t *tri, int F, int V, int32_t *inc_off, int32_t *inc,
+                        void *workspace, size_t workspace_bytes);
 
 /* ---- interpolate ---------------------------------------------------------
  * Replaces dr.interpolate(attr, rast, tri[, rast_db, diff_attrs])
@@ -74,11 +92,18 @@ int drp_interpolate_fwd(void *stream, const float *attr, int n_img, int V, int a
                         const int32_t *diff_idx, int n_diff, float *out, float *out_da);
 
 /* Backward: grad_attr accumulated (caller zero-fills), grad_rast (and grad_db
- * when not null) written; channels 2, 3 of grad_rast are zero. */
+ * when not null) written; channels 2, 3 of grad_rast are zero.  With boxes
+ * (from the rasterize call that produced rast), the incidence table, no
+ * diff_attrs, C in {1,2,3,4,8,16} and a workspace of
+ * drp_interpolate_bwd_workspace_size bytes, grad_attr is gathered per triangle
+ * (views_per_group views per partial) and per vertex without atomics. */
+size_t drp_interpolate_bwd_workspace_size(int n_img, int F, int C, int attr_batched, int views_per_group);
 int drp_interpolate_bwd(void *stream, const float *attr, int n_img, int V, int attr_batched, int C,
                         const float *rast, int H, int W, const int32_t *tri, int F, const float *rast_db,
                         const int32_t *diff_idx, int n_diff, const float *grad_out, const float *grad_da,
-                        float *grad_attr, float *grad_rast, float *grad_db);
+                        float *grad_attr, float *grad_rast, float *grad_db, const int32_t *boxes,
+                        int views_per_group, const int32_t *inc_off, const int32_t *inc, void *workspace,
+                        size_t workspace_bytes);
 
 /* ---- antialias -----------------------------------------------------------
  * Topology table for antialias: replaces dr.antialias_construct_topology_hash
