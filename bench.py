@@ -33,7 +33,7 @@ import torch
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from diffrp_amd import nvdr  # noqa: E402
+from diffrp_amd import autotune, nvdr  # noqa: E402
 from diffrp_amd.camera import PerspectiveCamera, clip_space  # noqa: E402
 from diffrp_amd.geometry import fibonacci_directions, noisy_icosphere  # noqa: E402
 
@@ -239,6 +239,7 @@ def main():
     ap.add_argument('--no-plain', action='store_true')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-ops', action='store_true')
+    ap.add_argument('--no-tune', action='store_true')
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
         print('bench.py measures one GPU: the rasterisation path has no multi-GPU mode in this build '
@@ -246,6 +247,12 @@ def main():
         sys.exit(2)
     dev = torch.device('cuda:0')
     S = Scene(dev, args.views, args.res, args.subdiv, args.channels)
+    tune = None
+    if not args.no_tune:
+        with torch.no_grad():
+            pos_all = clip_space(S.verts, S.VP)
+            tune = autotune.tune_rasterize(S.ctx, pos_all, S.tri, (S.H, S.W))
+            autotune.tune_rasterize(S.ctx, pos_all[0], S.tri, (S.H, S.W))  # the plain loop's single view
 
     # eager reference step, then capture the whole fwd+bwd in a HIP graph
     S.zero_grad()
@@ -318,6 +325,9 @@ def main():
         'ops': {k: {'ms': round(v['ms'], 4), 'gbs': round(v['gbs'], 1)} for k, v in ops.items()} if ops else None,
         'roofline': roof,
         'cpu_baseline': cpu,
+        'autotune': None if tune is None else {
+            'tile': list(tune['best']['tile']), 'bin_capacity': tune['best']['cap'],
+            'candidates_ms': {k: round(v['ms'], 4) for k, v in tune['all'].items()}},
     }
     print(json.dumps(out))
 

@@ -41,6 +41,8 @@ _SIGS = {
     'drp_interpolate_bwd': (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_int, c_int,
                                     c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
                                     c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_size]),
+    'drp_edge_grad_bwd': (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p,
+                                  c_int, c_void_p]),
     'drp_topology_workspace_size': (c_size, [c_int]),
     'drp_topology_build': (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_size]),
     'drp_antialias_workspace_size': (c_size, [c_int, c_int, c_int, c_int, c_int, c_int]),

@@ -254,30 +254,73 @@ __global__ void __launch_bounds__(256) k_aa_pairs(AALaunch L, float2 *__restrict
     weights[g] = w;
 }
 
+// channel chunk handled by one thread: 4 (one float4) when C is a multiple of 4
+template <int CV>
+struct AAChunk {
+    static constexpr int CH = (CV > 0 && CV % 4 == 0) ? 4 : 1;
+};
+
+template <int CH>
+__device__ __forceinline__ void ld(const float *p, float (&x)[CH]) {
+    if (CH == 4) {
+        float4 v = *reinterpret_cast<const float4 *>(p);
+        x[0] = v.x; x[1] = v.y; x[2] = v.z; x[3] = v.w;
+    } else {
+#pragma unroll
+        for (int i = 0; i < CH; ++i) x[i] = p[i];
+    }
+}
+template <int CH>
+__device__ __forceinline__ void st(float *p, const float (&x)[CH]) {
+    if (CH == 4) *reinterpret_cast<float4 *>(p) = make_float4(x[0], x[1], x[2], x[3]);
+    else
+#pragma unroll
+        for (int i = 0; i < CH; ++i) p[i] = x[i];
+}
+
+// one thread per (pixel, channel chunk): consecutive threads touch consecutive bytes
 template <int CV>
 __global__ void __launch_bounds__(256) k_aa_gather(AALaunch L, const float2 *__restrict__ weights) {
+    constexpr int CH = AAChunk<CV>::CH;
     const int C = CV > 0 ? CV : L.C;
-    long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int nch = C / CH;
+    long long gt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     long long npx = (long long)L.H * L.W;
+    long long g = gt / nch;
     if (g >= (long long)L.n_img * npx) return;
+    int ch = (int)(gt - g * nch) * CH;
     int img = (int)(g / npx);
     long long q = g - img * npx;
     int r = (int)(q / L.W), c = (int)(q % L.W);
-    const float *col = L.color;
-    float *out = L.out;
+    const float *col = L.color + ch;
     float2 wp = weights[g];
     float wl = c > 0 ? weights[g - 1].x : 0.f;        // pair (left, p): > 0 -> p receives
     float wd = r > 0 ? weights[g - L.W].y : 0.f;      // pair (below, p): > 0 -> p receives
-    bool rl = wl > 0.f, rr = wp.x < 0.f, rd = wd > 0.f, ru = wp.y < 0.f;
-    for (int ch = 0; ch < C; ++ch) {
-        float cp = col[g * C + ch];
-        float o = cp;
-        if (rl) o = fma_(wl, col[(g - 1) * C + ch] - cp, o);
-        if (rr) o = fma_(-wp.x, col[(g + 1) * C + ch] - cp, o);
-        if (rd) o = fma_(wd, col[(g - L.W) * C + ch] - cp, o);
-        if (ru) o = fma_(-wp.y, col[(g + L.W) * C + ch] - cp, o);
-        out[g * C + ch] = o;
+    float cp[CH], o[CH], nb[CH];
+    ld<CH>(col + g * C, cp);
+#pragma unroll
+    for (int i = 0; i < CH; ++i) o[i] = cp[i];
+    if (wl > 0.f) {
+        ld<CH>(col + (g - 1) * C, nb);
+#pragma unroll
+        for (int i = 0; i < CH; ++i) o[i] = fma_(wl, nb[i] - cp[i], o[i]);
     }
+    if (wp.x < 0.f) {
+        ld<CH>(col + (g + 1) * C, nb);
+#pragma unroll
+        for (int i = 0; i < CH; ++i) o[i] = fma_(-wp.x, nb[i] - cp[i], o[i]);
+    }
+    if (wd > 0.f) {
+        ld<CH>(col + (g - L.W) * C, nb);
+#pragma unroll
+        for (int i = 0; i < CH; ++i) o[i] = fma_(wd, nb[i] - cp[i], o[i]);
+    }
+    if (wp.y < 0.f) {
+        ld<CH>(col + (g + L.W) * C, nb);
+#pragma unroll
+        for (int i = 0; i < CH; ++i) o[i] = fma_(-wp.y, nb[i] - cp[i], o[i]);
+    }
+    st<CH>(L.out + ch + g * C, o);
 }
 
 // backward pairs: recompute each pair once, store its weight, scatter the
@@ -303,8 +346,18 @@ __global__ void __launch_bounds__(256) k_aa_pairs_bwd(AALaunch L, float2 *__rest
         long long ga = g, gb = dir ? g + L.W : g + 1;
         long long gr = pr.recv == 0 ? ga : gb, gs = pr.recv == 0 ? gb : ga;
         float dLdw = 0.f;
-        for (int ch = 0; ch < L.C; ++ch)
-            dLdw += L.grad_out[gr * L.C + ch] * (L.color[gs * L.C + ch] - L.color[gr * L.C + ch]);
+        if ((L.C & 3) == 0) {
+            const float4 *g4 = reinterpret_cast<const float4 *>(L.grad_out + gr * L.C);
+            const float4 *cs = reinterpret_cast<const float4 *>(L.color + gs * L.C);
+            const float4 *cr = reinterpret_cast<const float4 *>(L.color + gr * L.C);
+            for (int k4 = 0; k4 < (L.C >> 2); ++k4) {
+                float4 a = g4[k4], b = cs[k4], e = cr[k4];
+                dLdw += a.x * (b.x - e.x) + a.y * (b.y - e.y) + a.z * (b.z - e.z) + a.w * (b.w - e.w);
+            }
+        } else {
+            for (int ch = 0; ch < L.C; ++ch)
+                dLdw += L.grad_out[gr * L.C + ch] * (L.color[gs * L.C + ch] - L.color[gr * L.C + ch]);
+        }
         if (dLdw == 0.f) continue;
         bool recv_is_from = (pr.recv == 0) == (pr.from_a != 0);
         float dLdt = recv_is_from ? -dLdw : dLdw;
@@ -340,14 +393,18 @@ __global__ void __launch_bounds__(256) k_aa_pairs_bwd(AALaunch L, float2 *__rest
 
 template <int CV>
 __global__ void __launch_bounds__(256) k_aa_gather_bwd(AALaunch L, const float2 *__restrict__ weights) {
+    constexpr int CH = AAChunk<CV>::CH;
     const int C = CV > 0 ? CV : L.C;
-    long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int nch = C / CH;
+    long long gt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     long long npx = (long long)L.H * L.W;
+    long long g = gt / nch;
     if (g >= (long long)L.n_img * npx) return;
+    int ch = (int)(gt - g * nch) * CH;
     int img = (int)(g / npx);
     long long q = g - img * npx;
     int r = (int)(q / L.W), c = (int)(q % L.W);
-    const float *go = L.grad_out;
+    const float *go = L.grad_out + ch;
     float2 wp = weights[g];
     float wl = c > 0 ? weights[g - 1].x : 0.f;
     float wd = r > 0 ? weights[g - L.W].y : 0.f;
@@ -357,15 +414,31 @@ __global__ void __launch_bounds__(256) k_aa_gather_bwd(AALaunch L, const float2 
     if (wp.x < 0.f) self -= wp.x;
     if (wd > 0.f) self += wd;
     if (wp.y < 0.f) self -= wp.y;
-    for (int ch = 0; ch < C; ++ch) {
-        float gp = go[g * C + ch];
-        float acc = gp - self * gp;
-        if (wl < 0.f) acc = fma_(-wl, go[(g - 1) * C + ch], acc);        // left neighbour received from p
-        if (wp.x > 0.f) acc = fma_(wp.x, go[(g + 1) * C + ch], acc);     // right neighbour received from p
-        if (wd < 0.f) acc = fma_(-wd, go[(g - L.W) * C + ch], acc);
-        if (wp.y > 0.f) acc = fma_(wp.y, go[(g + L.W) * C + ch], acc);
-        L.grad_color[g * C + ch] = acc;
+    float gp[CH], acc[CH], nb[CH];
+    ld<CH>(go + g * C, gp);
+#pragma unroll
+    for (int i = 0; i < CH; ++i) acc[i] = gp[i] - self * gp[i];
+    if (wl < 0.f) {  // left neighbour received from p
+        ld<CH>(go + (g - 1) * C, nb);
+#pragma unroll
+        for (int i = 0; i < CH; ++i) acc[i] = fma_(-wl, nb[i], acc[i]);
     }
+    if (wp.x > 0.f) {  // right neighbour received from p
+        ld<CH>(go + (g + 1) * C, nb);
+#pragma unroll
+        for (int i = 0; i < CH; ++i) acc[i] = fma_(wp.x, nb[i], acc[i]);
+    }
+    if (wd < 0.f) {
+        ld<CH>(go + (g - L.W) * C, nb);
+#pragma unroll
+        for (int i = 0; i < CH; ++i) acc[i] = fma_(-wd, nb[i], acc[i]);
+    }
+    if (wp.y > 0.f) {
+        ld<CH>(go + (g + L.W) * C, nb);
+#pragma unroll
+        for (int i = 0; i < CH; ++i) acc[i] = fma_(wp.y, nb[i], acc[i]);
+    }
+    st<CH>(L.grad_color + ch + g * C, acc);
 }
 
 #define DRP_AA_DISPATCH(KERNEL, ...)                                                         \
@@ -387,12 +460,18 @@ static void launch_project(const AALaunch &L, hipStream_t st) {
                                L.F, n_proj, L.orient);
 }
 
+static dim3 gather_grid(const AALaunch &L, long long total) {
+    int nch = (L.cv > 0 && L.cv % 4 == 0) ? L.cv / 4 : (L.cv > 0 ? L.cv : L.C);
+    return dim3((unsigned)((total * nch + 255) / 256));
+}
+
 int launch_antialias(const AALaunch &L, hipStream_t st) {
     long long total = (long long)L.n_img * L.H * L.W;
     if (total == 0) return 0;
     launch_project(L, st);
     dim3 grid((unsigned)((total + 255) / 256)), block(256);
     hipLaunchKernelGGL(k_aa_pairs, grid, block, 0, st, L, L.weights);
+    grid = gather_grid(L, total);
     DRP_AA_DISPATCH(k_aa_gather, L, (const float2 *)L.weights)
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
@@ -403,6 +482,7 @@ int launch_antialias_bwd(const AALaunch &L, hipStream_t st) {
     launch_project(L, st);
     dim3 grid((unsigned)((total + 255) / 256)), block(256);
     hipLaunchKernelGGL(k_aa_pairs_bwd, grid, block, 0, st, L, L.weights);
+    grid = gather_grid(L, total);
     DRP_AA_DISPATCH(k_aa_gather_bwd, L, (const float2 *)L.weights)
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }

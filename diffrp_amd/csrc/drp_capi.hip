@@ -279,6 +279,18 @@ int drp_interpolate_bwd(void *stream, const float *attr, int n_img, int V, int a
     return 0;
 }
 
+int drp_edge_grad_bwd(void *stream, const float *grad_out, const float *rgb, int H, int W, int C, const float *rast,
+                      const float *frag_xy, const float *tris_xy, int F, float *grad_frag_xy) {
+    if (int e = check_dims(1, H, W)) return e;
+    if (!(C == 1 || C == 3 || C == 4)) return fail(-1, "edge gradient supports 1, 3 or 4 channels (got %d)", C);
+    if (!grad_out || !rgb || !rast || !frag_xy || !grad_frag_xy || (F > 0 && !tris_xy))
+        return fail(-1, "null pointer argument");
+    EdgeLaunch L{grad_out, rgb, rast, frag_xy, tris_xy, grad_frag_xy, H, W, C, F};
+    int rc = launch_edge_grad_bwd(L, (hipStream_t)stream);
+    if (rc) return fail(rc, "edge gradient launch failed: %s", hipGetErrorString(hipGetLastError()));
+    return 0;
+}
+
 size_t drp_incidence_workspace_size(int V) { return V > 0 ? incidence_ws_bytes(V) : 0; }
 
 int drp_incidence_build(void *stream, const int32_t *tri, int F, int V, int32_t *inc_off, int32_t *inc,
@@ -346,6 +358,7 @@ static int aa_common(AALaunch &L, void *stream, const float *color, int n_img, i
     if (C < 1) return fail(-1, "color channel count must be >= 1");
     if (!color || !rast || !pos || !tri || !opp) return fail(-1, "null pointer argument");
     if (!aligned16(pos) || !aligned16(rast)) return fail(-1, "pos/rast must be 16-byte aligned");
+    if (C % 4 == 0 && !aligned16(color)) return fail(-1, "color must be 16-byte aligned when C %% 4 == 0");
     size_t need = drp_antialias_workspace_size(n_img, H, W, V, F, pos_batched);
     if (!workspace || workspace_bytes < need)
         return fail(-1, "workspace too small: need %zu bytes, got %zu", need, workspace_bytes);
@@ -380,6 +393,7 @@ int drp_antialias_fwd(void *stream, const float *color, int n_img, int H, int W,
                           workspace_bytes))
         return e;
     if (!out) return fail(-1, "null output");
+    if (C % 4 == 0 && !aligned16(out)) return fail(-1, "out must be 16-byte aligned when C %% 4 == 0");
     L.out = out;
     int rc = launch_antialias(L, (hipStream_t)stream);
     if (rc) return fail(rc, "antialias launch failed: %s", hipGetErrorString(hipGetLastError()));
@@ -396,6 +410,8 @@ int drp_antialias_bwd(void *stream, const float *color, int n_img, int H, int W,
                           workspace_bytes))
         return e;
     if (!grad_out || !grad_color) return fail(-1, "null gradient buffer");
+    if (C % 4 == 0 && (!aligned16(grad_out) || !aligned16(grad_color)))
+        return fail(-1, "grad_out/grad_color must be 16-byte aligned when C %% 4 == 0");
     L.grad_out = grad_out;
     L.grad_color = grad_color;
     L.grad_pos = grad_pos;

@@ -137,24 +137,6 @@ __device__ __forceinline__ long long floor_div16(long long a) {  // floor(a / 16
     return a >> 4;
 }
 
-// clip polygon (x, y, w) against a*x + b*y + w >= 0; returns vertex count
-__device__ __forceinline__ int clip_plane(const float (*in)[3], int n, float (*out)[3], float a, float b) {
-    int m = 0;
-    for (int i = 0; i < n; ++i) {
-        const float *p = in[i];
-        const float *q = in[(i + 1) == n ? 0 : i + 1];
-        float dp = a * p[0] + b * p[1] + p[2];
-        float dq = a * q[0] + b * q[1] + q[2];
-        if (dp >= 0.f) { out[m][0] = p[0]; out[m][1] = p[1]; out[m][2] = p[2]; ++m; }
-        if ((dp >= 0.f) != (dq >= 0.f)) {
-            float t = dp / (dp - dq);
-            for (int k = 0; k < 3; ++k) out[m][k] = p[k] + t * (q[k] - p[k]);
-            ++m;
-        }
-    }
-    return m;
-}
-
 __device__ __forceinline__ void setup_tri(const TriVerts &v, int H, int W, TriSetup &s) {
     s.cls = kCull;
     for (int k = 0; k < 3; ++k)
@@ -196,34 +178,44 @@ __device__ __forceinline__ void setup_tri(const TriVerts &v, int H, int W, TriSe
         s.cls = kDirect;
         return;
     }
-    // homogeneous triangle: clip against the frustum sides only to bound / cull
-    float pa[8][3], pb[8][3];
+    // homogeneous triangle: conservative pixel box of its w > 0 part.
+    // Trivial reject: every vertex beyond one frustum side (pixel rays have |x|,|y| < w).
+    if ((v.x[0] > v.w[0] && v.x[1] > v.w[1] && v.x[2] > v.w[2]) ||
+        (-v.x[0] > v.w[0] && -v.x[1] > v.w[1] && -v.x[2] > v.w[2]) ||
+        (v.y[0] > v.w[0] && v.y[1] > v.w[1] && v.y[2] > v.w[2]) ||
+        (-v.y[0] > v.w[0] && -v.y[1] > v.w[1] && -v.y[2] > v.w[2]))
+        return;
+    if (!(v.w[0] > 0.f) && !(v.w[1] > 0.f) && !(v.w[2] > 0.f)) return;  // nothing in front of the eye
+    // projected vertices with w > 0, plus the directions at infinity of edges
+    // crossing w = 0 (their projections run off to infinity that way)
+    float mnx = INFINITY, mxx = -INFINITY, mny = INFINITY, mxy = -INFINITY;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) { pa[k][0] = v.x[k]; pa[k][1] = v.y[k]; pa[k][2] = v.w[k]; }
-    int n = 3;
-    n = clip_plane(pa, n, pb, -1.f, 0.f); if (!n) return;
-    n = clip_plane(pb, n, pa, 1.f, 0.f);  if (!n) return;
-    n = clip_plane(pa, n, pb, 0.f, -1.f); if (!n) return;
-    n = clip_plane(pb, n, pa, 0.f, 1.f);  if (!n) return;
-    float mnx = 1.f, mxx = -1.f, mny = 1.f, mxy = -1.f;
-    bool degenerate = false;
-    for (int i = 0; i < n; ++i) {
-        if (!(pa[i][2] > 0.f)) { degenerate = true; break; }
-        float nx = pa[i][0] / pa[i][2], ny = pa[i][1] / pa[i][2];
-        mnx = fminf(mnx, nx); mxx = fmaxf(mxx, nx);
-        mny = fminf(mny, ny); mxy = fmaxf(mxy, ny);
+    for (int k = 0; k < 3; ++k) {
+        int j = k == 2 ? 0 : k + 1;
+        if (v.w[k] > 0.f) {
+            float nx = v.x[k] / v.w[k], ny = v.y[k] / v.w[k];
+            mnx = fminf(mnx, nx); mxx = fmaxf(mxx, nx);
+            mny = fminf(mny, ny); mxy = fmaxf(mxy, ny);
+        }
+        if ((v.w[k] > 0.f) != (v.w[j] > 0.f)) {
+            float t = v.w[k] / (v.w[k] - v.w[j]);
+            float dx = v.x[k] + t * (v.x[j] - v.x[k]), dy = v.y[k] + t * (v.y[j] - v.y[k]);
+            if (dx > 0.f) mxx = INFINITY;
+            if (dx < 0.f) mnx = -INFINITY;
+            if (dy > 0.f) mxy = INFINITY;
+            if (dy < 0.f) mny = -INFINITY;
+            if (dx == 0.f && dy == 0.f) { mnx = mny = -INFINITY; mxx = mxy = INFINITY; }
+        }
     }
-    int c0, c1, r0, r1;
-    if (degenerate) { c0 = 0; c1 = W - 1; r0 = 0; r1 = H - 1; }
-    else {
-        c0 = (int)floorf((mnx + 1.f) * 0.5f * (float)W) - 1;
-        c1 = (int)floorf((mxx + 1.f) * 0.5f * (float)W) + 1;
-        r0 = (int)floorf((mny + 1.f) * 0.5f * (float)H) - 1;
-        r1 = (int)floorf((mxy + 1.f) * 0.5f * (float)H) + 1;
-        c0 = max(c0, 0); r0 = max(r0, 0);
-        c1 = min(c1, W - 1); r1 = min(r1, H - 1);
-        if (c0 > c1 || r0 > r1) return;
-    }
+    mnx = fmaxf(mnx, -2.f); mxx = fminf(mxx, 2.f);
+    mny = fmaxf(mny, -2.f); mxy = fminf(mxy, 2.f);
+    int c0 = (int)floorf((mnx + 1.f) * 0.5f * (float)W) - 1;
+    int c1 = (int)floorf((mxx + 1.f) * 0.5f * (float)W) + 1;
+    int r0 = (int)floorf((mny + 1.f) * 0.5f * (float)H) - 1;
+    int r1 = (int)floorf((mxy + 1.f) * 0.5f * (float)H) + 1;
+    c0 = max(c0, 0); r0 = max(r0, 0);
+    c1 = min(c1, W - 1); r1 = min(r1, H - 1);
+    if (c0 > c1 || r0 > r1) return;
     s.c0 = c0; s.c1 = c1; s.r0 = r0; s.r1 = r1;
     s.cls = kHomog;
 }
@@ -271,6 +263,25 @@ __device__ __forceinline__ float4 bary_db(const TriVerts &v, const PixEval &p, f
     r.z = (fma_(-vv, sx, ex1) / p.s) * kx;
     r.w = (fma_(-vv, sy, ey1) / p.s) * ky;
     return r;
+}
+
+// Triangle-centric walks over a pixel box: load the id channel of kWalk box
+// pixels (row-major from index `base`) as independent loads, return the mask
+// of those equal to `want`.  idw points at the id channel of pixel 0 of the image.
+constexpr int kWalk = 16;
+__device__ __forceinline__ unsigned walk_ids(const float *__restrict__ idw, int W, int r0, int c0, int c1, int bw,
+                                             int base, int npx, float want) {
+    int r = r0 + base / bw, c = c0 + base % bw;
+    float w[kWalk];
+#pragma unroll
+    for (int j = 0; j < kWalk; ++j) {
+        w[j] = (base + j < npx) ? idw[4 * ((long long)r * W + c)] : 0.f;
+        if (++c > c1) { c = c0; ++r; }
+    }
+    unsigned m = 0;
+#pragma unroll
+    for (int j = 0; j < kWalk; ++j) m |= (unsigned)(w[j] == want) << j;
+    return m;
 }
 
 // XCD-aware block remap: consecutive logical blocks land on the same XCD

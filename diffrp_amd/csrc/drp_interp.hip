@@ -211,12 +211,17 @@ __global__ void __launch_bounds__(256) k_interp_bwd_tri(const float4 *__restrict
     for (int img = v0; img < v1; ++img) {
         int2 bx = boxes[(long long)img * F + t];
         int c0 = bx.x & 0xffff, c1 = (unsigned)bx.x >> 16, r0 = bx.y & 0xffff, r1 = (unsigned)bx.y >> 16;
+        if (c0 > c1) continue;
         long long base = (long long)img * H * W;
-        for (int r = r0; r <= r1; ++r) {
-            const float4 *row = rast + base + (long long)r * W;
-            for (int c = c0; c <= c1; ++c) {
-                float4 rv = row[c];
-                if (rv.w != want) continue;
+        int bw = c1 - c0 + 1, npx = bw * (r1 - r0 + 1);
+        const float *idw = reinterpret_cast<const float *>(rast + base) + 3;
+        for (int b0 = 0; b0 < npx; b0 += kWalk) {
+            unsigned mask = walk_ids(idw, W, r0, c0, c1, bw, b0, npx, want);
+            while (mask) {
+                int j = __builtin_ctz(mask);
+                mask &= mask - 1;
+                int k = b0 + j, r = r0 + k / bw, c = c0 + k % bw;
+                float4 rv = rast[base + (long long)r * W + c];
                 float u = rv.x, v = rv.y, w2 = (1.f - u) - v;
                 const float *g = grad_out + (base + (long long)r * W + c) * NC;
                 float gv[NC];

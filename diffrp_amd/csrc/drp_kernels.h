@@ -83,6 +83,13 @@ struct AALaunch {
     float boost;
 };
 
+struct EdgeLaunch {
+    const float *grad_out, *rgb, *rast, *frag_xy, *tris_xy;
+    float *grad_frag_xy;
+    int H, W, C, F;
+};
+
+int launch_edge_grad_bwd(const EdgeLaunch &L, hipStream_t st);
 int launch_rasterize(const RasterLaunch &L, hipStream_t st);
 int launch_rasterize_bwd(const RasterBwdLaunch &L, hipStream_t st);
 int launch_interpolate(const InterpLaunch &L, hipStream_t st);

@@ -414,19 +414,23 @@ __global__ void __launch_bounds__(256) k_raster_bwd_tri(Batch b, const float *__
     TriVerts v;
     if (c0 <= c1 && load_tri(pos, tri, t, d, v)) {
         float want = triidx_to_float(t + 1);
-        for (int r = r0; r <= r1; ++r) {
-            float py = pix_ndc(r, d.H);
-            const float4 *row = rast + d.pix_off + (long long)r * d.W;
-            for (int c = c0; c <= c1; ++c) {
-                if (row[c].w != want) continue;
+        int bw = c1 - c0 + 1, npx = bw * (r1 - r0 + 1);
+        const float *idw = reinterpret_cast<const float *>(rast + d.pix_off) + 3;  // id channel
+        for (int base = 0; base < npx; base += kWalk) {
+            // issue the id loads of up to kWalk box pixels at once, then visit the matches
+            unsigned mask = walk_ids(idw, d.W, r0, c0, c1, bw, base, npx, want);
+            while (mask) {
+                int j = __builtin_ctz(mask);
+                mask &= mask - 1;
+                int k = base + j, r = r0 + k / bw, c = c0 + k % bw;
                 long long pix = d.pix_off + (long long)r * d.W + c;
                 float4 g = grad_rast[pix];
                 float4 gd = DB ? db_grad_scaled(grad_db[pix], d.H, d.W) : make_float4(0.f, 0.f, 0.f, 0.f);
                 float4 o[3];
-                raster_pixel_grad<DB>(v, pix_ndc(c, d.W), py, g, gd, o);
+                raster_pixel_grad<DB>(v, pix_ndc(c, d.W), pix_ndc(r, d.H), g, gd, o);
 #pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    acc[k].x += o[k].x; acc[k].y += o[k].y; acc[k].z += o[k].z; acc[k].w += o[k].w;
+                for (int q = 0; q < 3; ++q) {
+                    acc[q].x += o[q].x; acc[q].y += o[q].y; acc[q].z += o[q].z; acc[q].w += o[q].w;
                 }
             }
         }

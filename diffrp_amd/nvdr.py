@@ -98,10 +98,11 @@ def default_tile(H, W, F):
 
 
 def default_bin_capacity(tile, H, W, F):
-    # expected triangles per tile for a closed mesh seen from outside (both
-    # windings rasterised), with generous headroom; overflow only costs speed
+    # a closed mesh seen from outside puts ~2 F (tiles per triangle) / tiles
+    # triangles in a tile on average; silhouette tiles reach ~3x the mean.
+    # Overflow only costs speed; the autotuner sizes this from measured loads.
     tiles = max(1, ((W + tile[0] - 1) // tile[0]) * ((H + tile[1] - 1) // tile[1]))
-    est = 2.5 * F / tiles + 16
+    est = 6.0 * F / tiles + 32
     cap = 32
     while cap < est and cap < 4096:
         cap *= 2

@@ -128,6 +128,15 @@ int drp_antialias_bwd(void *stream, const float *color, int n_img, int H, int W,
                       const float *grad_out, float *grad_color, float *grad_pos, float pos_gradient_boost,
                       void *workspace, size_t workspace_bytes);
 
+/* ---- edge gradient -----------------------------------------------------------
+ * Replaces the backward of _EdgeGradient (diffrp/rendering/surface_deferred.py:
+ * 120-169, used by edge_gradient :777-804).  All images bottom row first,
+ * one view: grad_out / rgb [H,W,C] (C in {1,3,4}), rast [H,W,4], frag_xy [H,W,2]
+ * (per-pixel interpolated clip x/w, y/w), tris_xy [F,3,2] (projected corners);
+ * writes grad_frag_xy [H,W,2].  The colour gradient is grad_out unchanged. */
+int drp_edge_grad_bwd(void *stream, const float *grad_out, const float *rgb, int H, int W, int C, const float *rast,
+                      const float *frag_xy, const float *tris_xy, int F, float *grad_frag_xy);
+
 #ifdef __cplusplus
 }
 #endif

@@ -140,23 +140,6 @@ static inline int64_t floor_div(int64_t a, int64_t b) {
     return q;
 }
 
-/* clip polygon (x, y, w) against plane a*x + b*y + w >= 0 */
-static int SFX(clip_plane)(real (*in)[3], int n, real (*out)[3], real a, real b) {
-    int m = 0;
-    for (int i = 0; i < n; ++i) {
-        real *p = in[i], *q = in[(i + 1) % n];
-        real dp = a * p[0] + b * p[1] + p[2];
-        real dq = a * q[0] + b * q[1] + q[2];
-        if (dp >= 0) { out[m][0] = p[0]; out[m][1] = p[1]; out[m][2] = p[2]; ++m; }
-        if ((dp >= 0) != (dq >= 0)) {
-            real t = dp / (dp - dq);
-            for (int k = 0; k < 3; ++k) out[m][k] = p[k] + t * (q[k] - p[k]);
-            ++m;
-        }
-    }
-    return m;
-}
-
 static void SFX(setup_tri)(const real *pos, int V, const int32_t *t3, int H, int W, SFX(ora_setup) *s) {
     s->cls = 0;
     for (int k = 0; k < 3; ++k) {
@@ -208,37 +191,49 @@ static void SFX(setup_tri)(const real *pos, int V, const int32_t *t3, int H, int
         s->cls = 1;
         return;
     }
-    /* homogeneous: clip to the frustum side planes only to bound / cull */
-    real pa[8][3], pb[8][3];
-    for (int k = 0; k < 3; ++k) { pa[k][0] = s->x[k]; pa[k][1] = s->y[k]; pa[k][2] = s->w[k]; }
-    int n = 3;
-    n = SFX(clip_plane)(pa, n, pb, (real)-1, (real)0); if (!n) return;
-    n = SFX(clip_plane)(pb, n, pa, (real)1, (real)0);  if (!n) return;
-    n = SFX(clip_plane)(pa, n, pb, (real)0, (real)-1); if (!n) return;
-    n = SFX(clip_plane)(pb, n, pa, (real)0, (real)1);  if (!n) return;
-    real mnx = 1, mxx = -1, mny = 1, mxy = -1;
-    int degenerate = 0;
-    for (int i = 0; i < n; ++i) {
-        if (!(pa[i][2] > 0)) { degenerate = 1; break; }
-        real nx = pa[i][0] / pa[i][2], ny = pa[i][1] / pa[i][2];
-        if (nx < mnx) mnx = nx;
-        if (nx > mxx) mxx = nx;
-        if (ny < mny) mny = ny;
-        if (ny > mxy) mxy = ny;
+    /* homogeneous: a conservative pixel box of the w > 0 part (coverage itself is
+     * decided per pixel).  Reject triangles wholly beyond one frustum side (pixel
+     * rays have |x|, |y| < w) or wholly behind the eye; bound the rest by the
+     * projections of its w > 0 vertices, extended to infinity along the direction
+     * of every edge that crosses w = 0. */
+    const real *x = s->x, *y = s->y, *w = s->w;
+    if ((x[0] > w[0] && x[1] > w[1] && x[2] > w[2]) || (-x[0] > w[0] && -x[1] > w[1] && -x[2] > w[2]) ||
+        (y[0] > w[0] && y[1] > w[1] && y[2] > w[2]) || (-y[0] > w[0] && -y[1] > w[1] && -y[2] > w[2]))
+        return;
+    if (!(w[0] > 0) && !(w[1] > 0) && !(w[2] > 0)) return;
+    real mnx = INFINITY, mxx = -INFINITY, mny = INFINITY, mxy = -INFINITY;
+    for (int k = 0; k < 3; ++k) {
+        int j = (k + 1) % 3;
+        if (w[k] > 0) {
+            real nx = x[k] / w[k], ny = y[k] / w[k];
+            if (nx < mnx) mnx = nx;
+            if (nx > mxx) mxx = nx;
+            if (ny < mny) mny = ny;
+            if (ny > mxy) mxy = ny;
+        }
+        if ((w[k] > 0) != (w[j] > 0)) {
+            real t = w[k] / (w[k] - w[j]);
+            real dx = x[k] + t * (x[j] - x[k]), dy = y[k] + t * (y[j] - y[k]);
+            if (dx > 0) mxx = INFINITY;
+            if (dx < 0) mnx = -INFINITY;
+            if (dy > 0) mxy = INFINITY;
+            if (dy < 0) mny = -INFINITY;
+            if (dx == 0 && dy == 0) { mnx = mny = -INFINITY; mxx = mxy = INFINITY; }
+        }
     }
-    int c0, c1, r0, r1;
-    if (degenerate) { c0 = 0; c1 = W - 1; r0 = 0; r1 = H - 1; }
-    else {
-        c0 = (int)floor((mnx + 1) * (real)0.5 * (real)W) - 1;
-        c1 = (int)floor((mxx + 1) * (real)0.5 * (real)W) + 1;
-        r0 = (int)floor((mny + 1) * (real)0.5 * (real)H) - 1;
-        r1 = (int)floor((mxy + 1) * (real)0.5 * (real)H) + 1;
-        if (c0 < 0) c0 = 0;
-        if (r0 < 0) r0 = 0;
-        if (c1 > W - 1) c1 = W - 1;
-        if (r1 > H - 1) r1 = H - 1;
-        if (c0 > c1 || r0 > r1) return;
-    }
+    if (mnx < -2) mnx = -2;
+    if (mxx > 2) mxx = 2;
+    if (mny < -2) mny = -2;
+    if (mxy > 2) mxy = 2;
+    int c0 = (int)floor((mnx + 1) * (real)0.5 * (real)W) - 1;
+    int c1 = (int)floor((mxx + 1) * (real)0.5 * (real)W) + 1;
+    int r0 = (int)floor((mny + 1) * (real)0.5 * (real)H) - 1;
+    int r1 = (int)floor((mxy + 1) * (real)0.5 * (real)H) + 1;
+    if (c0 < 0) c0 = 0;
+    if (r0 < 0) r0 = 0;
+    if (c1 > W - 1) c1 = W - 1;
+    if (r1 > H - 1) r1 = H - 1;
+    if (c0 > c1 || r0 > r1) return;
     s->c0 = c0; s->c1 = c1; s->r0 = r0; s->r1 = r1;
     s->cls = 2;
 }

@@ -120,6 +120,51 @@ def false_color_camera_space_normal(rast, verts, tris, normals, M, V):
     return torch.cat([n[..., :3] * 0.5 + 0.5, n[..., 3:]], -1)
 
 
+def sign2d(p1, p2, p3):
+    """diffrp/utils/geometry.py:68-72."""
+    return (p1[..., 0:1] - p3[..., 0:1]) * (p2[..., 1:2] - p3[..., 1:2]) - \
+        (p2[..., 0:1] - p3[..., 0:1]) * (p1[..., 1:2] - p3[..., 1:2])
+
+
+def point_in_tri2d(pt, v1, v2, v3):
+    """diffrp/utils/geometry.py:75-91: inside unless the three signs are mixed."""
+    d = [sign2d(pt, v1, v2), sign2d(pt, v2, v3), sign2d(pt, v3, v1)]
+    neg = (d[0] < 0) | (d[1] < 0) | (d[2] < 0)
+    pos = (d[0] > 0) | (d[1] > 0) | (d[2] > 0)
+    return ~(neg & pos)
+
+
+def edge_gradient_backward(grad_output, frag_xy, flat_tris_xy, rast, rgb):
+    """Backward of _EdgeGradient (surface_deferred.py:127-169), bottom-row-first
+    [H, W, .] tensors: returns (grad_frag_xy [H,W,2], grad_rgb).  Pairs along one
+    axis give e = sign * 0.5 <g_a + g_b, c_a - c_b> to a where b is empty or a's
+    fragment lies in b's triangle, and to b symmetrically; the zero padding of
+    the one-shorter pair arrays (torch_redstone.supercat in the reference) is
+    written as explicit pads."""
+    H, W = frag_xy.shape[:2]
+    tid = float_to_triidx(rast[..., 3:4])
+    empty = tid == 0
+    scr = flat_tris_xy[(tid.squeeze(-1) - 1).clamp(min=0).long()]          # [H, W, 3, 2]
+
+    def axis_grad(dim, sign):
+        n = frag_xy.shape[dim]
+        A = lambda t: t.narrow(dim, 0, n - 1)  # noqa: E731
+        B = lambda t: t.narrow(dim, 1, n - 1)  # noqa: E731
+        e = sign * 0.5 * ((A(grad_output) + B(grad_output)) * (A(rgb) - B(rgb))).sum(-1, keepdim=True)
+        edge = A(tid) != B(tid)
+        sb, sa = B(scr), A(scr)
+        a_in_b = B(empty) | point_in_tri2d(A(frag_xy), sb[..., 0, :], sb[..., 1, :], sb[..., 2, :])
+        b_in_a = A(empty) | point_in_tri2d(B(frag_xy), sa[..., 0, :], sa[..., 1, :], sa[..., 2, :])
+        ga = (edge & a_in_b).float() * e
+        gb = (edge & b_in_a).float() * e
+        zero = torch.zeros_like(ga.narrow(dim, 0, 1))
+        return torch.cat([ga, zero], dim) + torch.cat([zero, gb], dim)
+
+    gy = axis_grad(0, -1.0)
+    gx = axis_grad(1, 1.0)
+    return torch.cat([gx * (W / 2), gy * (H / 2)], -1), grad_output.clone()
+
+
 def to_uint8(x: torch.Tensor) -> np.ndarray:
     """diffrp/utils/exchange.py:7-18: (saturate(x) * 255).byte()  (truncation)."""
     return (torch.clamp(x.float(), 0.0, 1.0) * 255).byte().numpy()
