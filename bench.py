@@ -199,6 +199,7 @@ def cpu_baseline(S, views):
     """The CPU oracle (scalar C restatement + torch-CPU loss) on a bounded sample of views."""
     import oracle
     oracle.build()
+    oracle.set_threads(1)
     tri = S.tri_cpu.numpy()
     opp = oracle.topology(tri)
     attr = S.attr_cpu.numpy()
@@ -235,7 +236,7 @@ def main():
     ap.add_argument('--subdiv', type=int, default=6)
     ap.add_argument('--channels', type=int, default=8)
     ap.add_argument('--plain-steps', type=int, default=3)
-    ap.add_argument('--cpu-views', type=int, default=6)
+    ap.add_argument('--cpu-views', type=int, default=64)
     ap.add_argument('--no-plain', action='store_true')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-ops', action='store_true')

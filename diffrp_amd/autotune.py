@@ -44,7 +44,7 @@ def bin_load(pos, tri, resolution, tile, probe_cap=4096) -> Tuple[int, int]:
     ws_bytes = L.drp_rasterize_workspace_size(n, H, W, n * F, tile[0], tile[1], probe_cap)
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device=pos.device)
     rast = torch.empty((n, H, W, 4), dtype=torch.float32, device=pos.device)
-    boxes = torch.empty((n, F, 2), dtype=torch.int32, device=pos.device)
+    boxes = torch.empty((n, F, 3), dtype=torch.int32, device=pos.device)
     _run(pos, tri, H, W, tile, probe_cap, ws, boxes, rast)
     n_tiles = n * ((W + tile[0] - 1) // tile[0]) * ((H + tile[1] - 1) // tile[1])
     # workspace layout: int32 bin counts [n_tiles], then spill-list counts [n]
@@ -68,7 +68,7 @@ def tune_rasterize(ctx: 'nvdr.RasterizeCudaContext', pos: torch.Tensor, tri: tor
         ws_bytes = L.drp_rasterize_workspace_size(n, H, W, n * F, tile[0], tile[1], cap)
         ws = torch.empty(ws_bytes, dtype=torch.uint8, device=pos.device)
         rast = torch.empty((n, H, W, 4), dtype=torch.float32, device=pos.device)
-        boxes = torch.empty((n, F, 2), dtype=torch.int32, device=pos.device)
+        boxes = torch.empty((n, F, 3), dtype=torch.int32, device=pos.device)
         _run(pos, tri, H, W, tile, cap, ws, boxes, rast)
         torch.cuda.synchronize(pos.device)
         st = torch.cuda.current_stream(pos.device)

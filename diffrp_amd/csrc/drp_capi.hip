@@ -37,6 +37,13 @@ bool tile_ok(int tw, int th) {
 
 bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
 
+// views folded into one attribute partial: a power of two in [1, 8]
+int views_per_block(int v) {
+    int p = 1;
+    while (p * 2 <= v && p < 8) p *= 2;
+    return p;
+}
+
 int channel_variant(int C) { return (C == 1 || C == 2 || C == 3 || C == 4 || C == 8 || C == 16) ? C : 0; }
 
 struct RasterWs {
@@ -129,8 +136,8 @@ int drp_rasterize_fwd(void *stream, const float *pos, int n_img, int V, int pos_
     L.bins = ws.bins;
     L.lists = ws.lists;
     L.list_box = ws.list_box;
-    L.boxes = (int2 *)boxes;
-    if (boxes && !aligned16(boxes) && ((uintptr_t)boxes & 7)) return fail(-1, "boxes must be 8-byte aligned");
+    L.boxes = (ItemBox *)boxes;
+    if (boxes && ((uintptr_t)boxes & 3)) return fail(-1, "boxes must be 4-byte aligned");
     L.n_tiles = (int)n_tiles;
     L.max_tris = max_range;
     L.prev = prev;
@@ -162,7 +169,7 @@ int drp_rasterize_bwd(void *stream, const float *pos, int n_img, int V, int pos_
                     workspace_bytes);
     RasterBwdLaunch L{};
     if (tri_path) {
-        L.boxes = (const int2 *)boxes;
+        L.boxes = (const ItemBox *)boxes;
         L.partial = (float *)workspace;
         L.inc_off = inc_off;
         L.inc = inc;
@@ -222,7 +229,7 @@ int drp_interpolate_fwd(void *stream, const float *attr, int n_img, int V, int a
 
 size_t drp_interpolate_bwd_workspace_size(int n_img, int F, int C, int attr_batched, int views_per_group) {
     if (n_img <= 0 || F <= 0 || C <= 0) return 0;
-    int vpg = attr_batched ? 1 : (views_per_group < 1 ? 1 : views_per_group);
+    int vpg = attr_batched ? 1 : views_per_block(views_per_group);
     size_t groups = ((size_t)n_img + vpg - 1) / vpg;
     return align256(groups * (size_t)F * 3 * (size_t)C * 4);
 }
@@ -262,14 +269,14 @@ int drp_interpolate_bwd(void *stream, const float *attr, int n_img, int V, int a
     L.grad_db = grad_db;
     L.H = H;
     L.W = W;
-    L.views_per_group = views_per_group < 1 ? 1 : views_per_group;
+    L.views_per_group = attr_batched ? 1 : views_per_block(views_per_group);
     bool tri_path = boxes && inc_off && inc && n_diff == 0 && L.cv > 0;
     if (tri_path) {
         size_t need = drp_interpolate_bwd_workspace_size(n_img, F, C, attr_batched, L.views_per_group);
         if (!workspace || workspace_bytes < need)
             return fail(-1, "workspace too small: need %zu bytes, got %zu", need, workspace_bytes);
         if (C % 4 == 0 && !aligned16(grad_out)) return fail(-1, "grad_out must be 16-byte aligned");
-        L.boxes = (const int2 *)boxes;
+        L.boxes = (const ItemBox *)boxes;
         L.partial = (float *)workspace;
         L.inc_off = inc_off;
         L.inc = inc;
@@ -347,7 +354,7 @@ int drp_topology_build(void *stream, const int32_t *tri, int F, int32_t *opp, vo
 size_t drp_antialias_workspace_size(int n_img, int H, int W, int V, int F, int pos_batched) {
     if (n_img <= 0) return 0;
     size_t n_proj = pos_batched ? (size_t)n_img : 1;
-    return align256(n_proj * (size_t)V * 8) + align256(n_proj * (size_t)F) +
+    return align256(n_proj * (size_t)V * 8) + 2 * align256(n_proj * (size_t)F) +
            align256((size_t)n_img * H * W * 8);
 }
 
@@ -377,9 +384,12 @@ static int aa_common(AALaunch &L, void *stream, const float *color, int n_img, i
     L.F = F;
     L.pos_batched = pos_batched;
     L.cv = (C == 1 || C == 3 || C == 4 || C == 8 || C == 16) ? C : 0;
-    L.proj = (float2 *)b;
-    L.orient = (signed char *)(b + align256(n_proj * (size_t)V * 8));
-    L.weights = (float2 *)(b + align256(n_proj * (size_t)V * 8) + align256(n_proj * (size_t)F));
+    size_t o_proj = 0, o_orient = align256(n_proj * (size_t)V * 8);
+    size_t o_silh = o_orient + align256(n_proj * (size_t)F), o_w = o_silh + align256(n_proj * (size_t)F);
+    L.proj = (float2 *)(b + o_proj);
+    L.orient = (signed char *)(b + o_orient);
+    L.silh = (unsigned char *)(b + o_silh);
+    L.weights = (float2 *)(b + o_w);
     (void)stream;
     return 0;
 }

@@ -265,25 +265,6 @@ __device__ __forceinline__ float4 bary_db(const TriVerts &v, const PixEval &p, f
     return r;
 }
 
-// Triangle-centric walks over a pixel box: load the id channel of kWalk box
-// pixels (row-major from index `base`) as independent loads, return the mask
-// of those equal to `want`.  idw points at the id channel of pixel 0 of the image.
-constexpr int kWalk = 16;
-__device__ __forceinline__ unsigned walk_ids(const float *__restrict__ idw, int W, int r0, int c0, int c1, int bw,
-                                             int base, int npx, float want) {
-    int r = r0 + base / bw, c = c0 + base % bw;
-    float w[kWalk];
-#pragma unroll
-    for (int j = 0; j < kWalk; ++j) {
-        w[j] = (base + j < npx) ? idw[4 * ((long long)r * W + c)] : 0.f;
-        if (++c > c1) { c = c0; ++r; }
-    }
-    unsigned m = 0;
-#pragma unroll
-    for (int j = 0; j < kWalk; ++j) m |= (unsigned)(w[j] == want) << j;
-    return m;
-}
-
 // XCD-aware block remap: consecutive logical blocks land on the same XCD
 // (gfx950 dispatches workgroups round-robin over its 8 XCDs).
 __device__ __forceinline__ int xcd_remap(int bid, int nblocks_padded) {

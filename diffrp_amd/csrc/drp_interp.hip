@@ -162,8 +162,8 @@ int launch_interpolate(const InterpLaunch &L, hipStream_t st) {
 // ----------------------------------------------------------------------------
 // triangle-centric backward (no atomics, deterministic)
 //   k_interp_grad_rast: per pixel, grad_rast = (<g, a0-a2>, <g, a1-a2>, 0, 0)
-//   k_interp_bwd_tri:   per (view group, triangle), walk the triangle's pixel box
-//                       in each view and accumulate (u, v, 1-u-v) * g per corner
+//   k_tri_bwd (drp_tribwd.hip): per (view group, triangle), accumulate
+//                       (u, v, 1-u-v) * g per corner over the triangle's pixels
 //   k_gather_attr:      per vertex, sum the corner partials (CSR, fixed order)
 // ----------------------------------------------------------------------------
 template <int CV>
@@ -191,64 +191,6 @@ __global__ void __launch_bounds__(256) k_interp_grad_rast(const float *__restric
         sv += gc * (a1[ch] - x2);
     }
     grad_rast[gid] = make_float4(su, sv, 0.f, 0.f);
-}
-
-template <int NC>
-__global__ void __launch_bounds__(256) k_interp_bwd_tri(const float4 *__restrict__ rast, int F, int n_img, int H,
-                                                        int W, const float *__restrict__ grad_out,
-                                                        const int2 *__restrict__ boxes, int views_per_group,
-                                                        float *__restrict__ partial) {
-    int grp = blockIdx.y;
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= F) return;
-    float acc[3][NC];
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
-#pragma unroll
-        for (int ch = 0; ch < NC; ++ch) acc[k][ch] = 0.f;
-    float want = triidx_to_float(t + 1);
-    int v0 = grp * views_per_group, v1 = min(v0 + views_per_group, n_img);
-    for (int img = v0; img < v1; ++img) {
-        int2 bx = boxes[(long long)img * F + t];
-        int c0 = bx.x & 0xffff, c1 = (unsigned)bx.x >> 16, r0 = bx.y & 0xffff, r1 = (unsigned)bx.y >> 16;
-        if (c0 > c1) continue;
-        long long base = (long long)img * H * W;
-        int bw = c1 - c0 + 1, npx = bw * (r1 - r0 + 1);
-        const float *idw = reinterpret_cast<const float *>(rast + base) + 3;
-        for (int b0 = 0; b0 < npx; b0 += kWalk) {
-            unsigned mask = walk_ids(idw, W, r0, c0, c1, bw, b0, npx, want);
-            while (mask) {
-                int j = __builtin_ctz(mask);
-                mask &= mask - 1;
-                int k = b0 + j, r = r0 + k / bw, c = c0 + k % bw;
-                float4 rv = rast[base + (long long)r * W + c];
-                float u = rv.x, v = rv.y, w2 = (1.f - u) - v;
-                const float *g = grad_out + (base + (long long)r * W + c) * NC;
-                float gv[NC];
-                if ((NC % 4) == 0) {
-#pragma unroll
-                    for (int ch = 0; ch < NC; ch += 4) {
-                        float4 q = *reinterpret_cast<const float4 *>(g + ch);
-                        gv[ch] = q.x; gv[ch + 1] = q.y; gv[ch + 2] = q.z; gv[ch + 3] = q.w;
-                    }
-                } else {
-#pragma unroll
-                    for (int ch = 0; ch < NC; ++ch) gv[ch] = g[ch];
-                }
-#pragma unroll
-                for (int ch = 0; ch < NC; ++ch) {
-                    acc[0][ch] += u * gv[ch];
-                    acc[1][ch] += v * gv[ch];
-                    acc[2][ch] += w2 * gv[ch];
-                }
-            }
-        }
-    }
-    float *P = partial + ((long long)grp * F + t) * 3 * NC;
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
-#pragma unroll
-        for (int ch = 0; ch < NC; ++ch) P[k * NC + ch] = acc[k][ch];
 }
 
 // grad_attr[out][v] += sum over groups g in the output's range and incident corners
@@ -286,9 +228,17 @@ static void launch_interp_bwd_tri_t(const InterpLaunch &L, hipStream_t st) {
     int n_img = (int)(total / L.pix_per_img);
     int vpg = L.attr_batched ? 1 : L.views_per_group;
     int groups = (n_img + vpg - 1) / vpg;
-    hipLaunchKernelGGL(k_interp_bwd_tri<NC>, dim3((L.F + 255) / 256, groups), dim3(256), 0, st,
-                       (const float4 *)L.rast, L.F, n_img, L.H, L.W, L.grad_out, (const int2 *)L.boxes, vpg,
-                       L.partial);
+    TriBwdLaunch T{};
+    T.rast = L.rast;
+    T.grad_out = L.grad_out;
+    T.boxes = L.boxes;
+    T.partial = L.partial;
+    T.n_img = n_img;
+    T.F = L.F;
+    T.H = L.H;
+    T.W = L.W;
+    T.views_per_block = vpg;
+    launch_tri_bwd_attr(T, NC, st);
     int outs = L.attr_batched ? n_img : 1;
     hipLaunchKernelGGL(k_gather_attr<NC>, dim3((L.V + 255) / 256, outs), dim3(256), 0, st, L.partial, groups, L.F,
                        L.V, L.attr_batched, L.inc_off, L.inc, L.grad_attr);

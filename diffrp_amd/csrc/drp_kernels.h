@@ -5,6 +5,13 @@
 
 namespace drp {
 
+// Pixel box of one (image, triangle) from the forward setup: columns c0..c1 in
+// cc (c0 | c1 << 16), rows r0..r1 in rr; c0 > c1 when empty.  hit is set by the
+// forward when the triangle won at least one pixel of that image.
+struct ItemBox {
+    int cc, rr, hit;
+};
+
 struct RasterLaunch {
     Batch b;
     const float *pos;
@@ -15,7 +22,7 @@ struct RasterLaunch {
     int *bins;        // [n_tiles * bin_cap]
     int *lists;       // spill lists
     int2 *list_box;   // pixel box of each spill-list entry
-    int2 *boxes;      // optional [n_img, F] pixel box per (image, triangle)
+    ItemBox *boxes;   // optional [n_img, F] pixel box + hit flag per (image, triangle)
     int n_tiles;
     int max_tris;     // largest triangle range of any image
     const float *prev;
@@ -32,7 +39,7 @@ struct RasterBwdLaunch {
     long long total_pix;
     int pix_per_img;
     // triangle-centric path (all set) or per-pixel atomics (any null)
-    const int2 *boxes;
+    const ItemBox *boxes;
     float *partial;          // [n_img, F, 3] float4
     const int *inc_off, *inc;
 };
@@ -53,7 +60,7 @@ struct InterpLaunch {
     float *grad_attr, *grad_rast, *grad_db;
     // triangle-centric backward (all set) or per-pixel atomics (any null)
     int H, W;
-    const int2 *boxes;
+    const ItemBox *boxes;
     float *partial;          // [groups, F, 3, C]
     const int *inc_off, *inc;
     int views_per_group;
@@ -76,12 +83,27 @@ struct AALaunch {
     int cv;           // compiled channel variant (0 = generic loop)
     float2 *proj;
     signed char *orient;
+    unsigned char *silh;
     float2 *weights;
     float *out;
     const float *grad_out;
     float *grad_color, *grad_pos;
     float boost;
 };
+
+struct TriBwdLaunch {
+    Batch b;
+    const float *pos;
+    const int *tri;
+    const float *rast, *grad_rast, *grad_db, *grad_out;
+    const ItemBox *boxes;
+    float *partial;
+    int n_img, F, H, W;
+    int views_per_block;   // 1, 2, 4 or 8 (attribute mode); 1 for positions
+};
+
+int launch_tri_bwd_pos(const TriBwdLaunch &L, hipStream_t st);
+int launch_tri_bwd_attr(const TriBwdLaunch &L, int C, hipStream_t st);
 
 struct EdgeLaunch {
     const float *grad_out, *rgb, *rast, *frag_xy, *tris_xy;

@@ -9,12 +9,13 @@
 // test resolved by 64-bit LDS atomicMin on (orderable(z/w) << 32 | triangle);
 // (u, v) are recomputed for the winner only.
 //
-// Backward: triangle-centric.  One thread per (image, triangle) walks the
-// triangle's pixel box, accumulates the position gradient of every pixel it
-// won (no atomics), writes a per-corner partial, and a CSR vertex gather sums
-// the partials (deterministic).  A per-pixel atomic kernel handles inputs that
-// come without pixel boxes.
+// The setup pass also records each (image, triangle)'s pixel box, and k_fine
+// flags the triangles that won a pixel; the triangle-centric backward
+// (drp_tribwd.hip) walks the flagged boxes and writes per-corner partials that
+// a CSR vertex gather sums (deterministic, no atomics).  A per-pixel atomic
+// kernel handles inputs that come without pixel boxes.
 #include "drp_common.h"
+#include "drp_grad.h"
 #include "drp_kernels.h"
 
 namespace drp {
@@ -41,7 +42,7 @@ __global__ void __launch_bounds__(256) k_setup_bin(Batch b, const float *__restr
                                                    int bin_cap, int *__restrict__ bin_count,
                                                    int *__restrict__ bins, int *__restrict__ list_count,
                                                    int *__restrict__ lists, int2 *__restrict__ list_box,
-                                                   int2 *__restrict__ boxes) {
+                                                   ItemBox *__restrict__ boxes) {
     int img = blockIdx.y;
     ImageDesc d = image_desc(b, img);
     int t = d.f0 + blockIdx.x * blockDim.x + threadIdx.x;
@@ -51,7 +52,10 @@ __global__ void __launch_bounds__(256) k_setup_bin(Batch b, const float *__restr
     if (valid) {
         TriVerts v;
         if (load_tri(pos, tri, t, d, v)) setup_tri(v, d.H, d.W, s);
-        if (boxes) boxes[(long long)img * b.F + t] = s.cls == kCull ? empty_box() : pack_box(s.c0, s.c1, s.r0, s.r1);
+        if (boxes) {
+            int2 pb = s.cls == kCull ? empty_box() : pack_box(s.c0, s.c1, s.r0, s.r1);
+            boxes[(long long)img * b.F + t] = ItemBox{pb.x, pb.y, 0};  // hit: set by k_fine
+        }
     }
     bool live = valid && s.cls != kCull;
     int tx0 = 0, tx1 = -1, ty0 = 0, ty1 = -1, ntiles = 0;
@@ -150,7 +154,7 @@ __global__ void __launch_bounds__(TW *TH) k_fine(Batch b, const float *__restric
                                                  const int *__restrict__ list_count, const int *__restrict__ lists,
                                                  const int2 *__restrict__ list_box, int n_tiles, int n_tiles_pad,
                                                  const float4 *__restrict__ prev, float4 *__restrict__ rast,
-                                                 float4 *__restrict__ rast_db) {
+                                                 float4 *__restrict__ rast_db, ItemBox *__restrict__ boxes) {
     constexpr int NT = TW * TH;
     static_assert(NT % 64 == 0, "tile must be a whole number of waves");
     __shared__ FineRec rec[NT];
@@ -297,63 +301,7 @@ __global__ void __launch_bounds__(TW *TH) k_fine(Batch b, const float *__restric
     float u = p.e[0] / p.s, vv = p.e[1] / p.s;
     rast[pix] = make_float4(u, vv, zw, triidx_to_float(t + 1));
     if (DB) rast_db[pix] = bary_db(v, p, u, vv, d.H, d.W);
-}
-
-// ----------------------------------------------------------------------------
-// backward math of one pixel: d(u, v, z/w [, db]) -> d(x, y, z, w) of the corners
-// ----------------------------------------------------------------------------
-template <bool DB>
-__device__ __forceinline__ void raster_pixel_grad(const TriVerts &v, float px, float py, float4 g, float4 gd,
-                                                  float4 out[3]) {
-    PixEval p;
-    eval_pixel(v, px, py, p);
-    float u = p.e[0] / p.s, vv = p.e[1] / p.s, zw = p.zn / p.wn;
-    float gu = g.x, gv = g.y, gz = g.z;
-    float gX[3] = {0.f, 0.f, 0.f}, gY[3] = {0.f, 0.f, 0.f}, gw[3] = {0.f, 0.f, 0.f}, gs = 0.f;
-    if (DB) {
-        float ex[3], ey[3];
-        ex[0] = fma_(v.w[2], p.Y[1], -(v.w[1] * p.Y[2]));
-        ex[1] = fma_(v.w[0], p.Y[2], -(v.w[2] * p.Y[0]));
-        ex[2] = fma_(v.w[1], p.Y[0], -(v.w[0] * p.Y[1]));
-        ey[0] = fma_(v.w[1], p.X[2], -(v.w[2] * p.X[1]));
-        ey[1] = fma_(v.w[2], p.X[0], -(v.w[0] * p.X[2]));
-        ey[2] = fma_(v.w[0], p.X[1], -(v.w[1] * p.X[0]));
-        float sx = ex[0] + ex[1] + ex[2], sy = ey[0] + ey[1] + ey[2];
-        float Aux = ex[0] - u * sx, Auy = ey[0] - u * sy, Avx = ex[1] - vv * sx, Avy = ey[1] - vv * sy;
-        float a = gd.x / p.s, bb = gd.y / p.s, cc = gd.z / p.s, dd = gd.w / p.s;  // gd pre-scaled by kx / ky
-        float gsx = -(a * u + cc * vv), gsy = -(bb * u + dd * vv);
-        float gex[3] = {a + gsx, cc + gsx, gsx}, gey[3] = {bb + gsy, dd + gsy, gsy};
-        gu += -(a * sx + bb * sy);
-        gv += -(cc * sx + dd * sy);
-        gs += -(a * Aux + bb * Auy + cc * Avx + dd * Avy) / p.s;
-        gw[2] += gex[0] * p.Y[1]; gY[1] += gex[0] * v.w[2]; gw[1] -= gex[0] * p.Y[2]; gY[2] -= gex[0] * v.w[1];
-        gw[0] += gex[1] * p.Y[2]; gY[2] += gex[1] * v.w[0]; gw[2] -= gex[1] * p.Y[0]; gY[0] -= gex[1] * v.w[2];
-        gw[1] += gex[2] * p.Y[0]; gY[0] += gex[2] * v.w[1]; gw[0] -= gex[2] * p.Y[1]; gY[1] -= gex[2] * v.w[0];
-        gw[1] += gey[0] * p.X[2]; gX[2] += gey[0] * v.w[1]; gw[2] -= gey[0] * p.X[1]; gX[1] -= gey[0] * v.w[2];
-        gw[2] += gey[1] * p.X[0]; gX[0] += gey[1] * v.w[2]; gw[0] -= gey[1] * p.X[2]; gX[2] -= gey[1] * v.w[0];
-        gw[0] += gey[2] * p.X[1]; gX[1] += gey[2] * v.w[0]; gw[1] -= gey[2] * p.X[0]; gX[0] -= gey[2] * v.w[1];
-    }
-    float inv_s = 1.f / p.s;
-    float G0 = (gu * (1.f - u) + gv * (-vv)) * inv_s + gs + gz * (v.z[0] - zw * v.w[0]) / p.wn;
-    float G1 = (gu * (-u) + gv * (1.f - vv)) * inv_s + gs + gz * (v.z[1] - zw * v.w[1]) / p.wn;
-    float G2 = (gu * (-u) + gv * (-vv)) * inv_s + gs + gz * (v.z[2] - zw * v.w[2]) / p.wn;
-    gX[0] += G2 * p.Y[1] - G1 * p.Y[2];
-    gX[1] += G0 * p.Y[2] - G2 * p.Y[0];
-    gX[2] += G1 * p.Y[0] - G0 * p.Y[1];
-    gY[0] += G1 * p.X[2] - G2 * p.X[1];
-    gY[1] += G2 * p.X[0] - G0 * p.X[2];
-    gY[2] += G0 * p.X[1] - G1 * p.X[0];
-    float gzw = gz / p.wn;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        float ek = p.e[k];
-        out[k] = make_float4(gX[k], gY[k], gzw * ek, gw[k] - px * gX[k] - py * gY[k] - gzw * zw * ek);
-    }
-}
-
-__device__ __forceinline__ float4 db_grad_scaled(float4 gd, int H, int W) {
-    float kx = 2.f / (float)W, ky = 2.f / (float)H;
-    return make_float4(gd.x * kx, gd.y * ky, gd.z * kx, gd.w * ky);
+    if (boxes) boxes[(long long)img * b.F + t].hit = 1;  // same value from every winning pixel
 }
 
 // per-pixel fallback: atomics into grad_pos
@@ -393,54 +341,6 @@ __global__ void __launch_bounds__(256) k_raster_bwd(Batch b, const float *__rest
     }
 }
 
-// triangle-centric: one thread per (image, triangle) -> partial[img][t][corner]
-template <bool DB>
-__global__ void __launch_bounds__(256) k_raster_bwd_tri(Batch b, const float *__restrict__ pos,
-                                                        const int *__restrict__ tri,
-                                                        const float4 *__restrict__ rast,
-                                                        const float4 *__restrict__ grad_rast,
-                                                        const float4 *__restrict__ grad_db,
-                                                        const int2 *__restrict__ boxes,
-                                                        float4 *__restrict__ partial) {
-    int img = blockIdx.y;
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= b.F) return;
-    ImageDesc d = image_desc(b, img);
-    float4 acc[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) acc[k] = make_float4(0.f, 0.f, 0.f, 0.f);
-    int c0, c1, r0, r1;
-    unpack_box(boxes[(long long)img * b.F + t], c0, c1, r0, r1);
-    TriVerts v;
-    if (c0 <= c1 && load_tri(pos, tri, t, d, v)) {
-        float want = triidx_to_float(t + 1);
-        int bw = c1 - c0 + 1, npx = bw * (r1 - r0 + 1);
-        const float *idw = reinterpret_cast<const float *>(rast + d.pix_off) + 3;  // id channel
-        for (int base = 0; base < npx; base += kWalk) {
-            // issue the id loads of up to kWalk box pixels at once, then visit the matches
-            unsigned mask = walk_ids(idw, d.W, r0, c0, c1, bw, base, npx, want);
-            while (mask) {
-                int j = __builtin_ctz(mask);
-                mask &= mask - 1;
-                int k = base + j, r = r0 + k / bw, c = c0 + k % bw;
-                long long pix = d.pix_off + (long long)r * d.W + c;
-                float4 g = grad_rast[pix];
-                float4 gd = DB ? db_grad_scaled(grad_db[pix], d.H, d.W) : make_float4(0.f, 0.f, 0.f, 0.f);
-                float4 o[3];
-                raster_pixel_grad<DB>(v, pix_ndc(c, d.W), pix_ndc(r, d.H), g, gd, o);
-#pragma unroll
-                for (int q = 0; q < 3; ++q) {
-                    acc[q].x += o[q].x; acc[q].y += o[q].y; acc[q].z += o[q].z; acc[q].w += o[q].w;
-                }
-            }
-        }
-    }
-    float4 *P = partial + ((long long)img * b.F + t) * 3;
-    P[0] = acc[0];
-    P[1] = acc[1];
-    P[2] = acc[2];
-}
-
 // vertex gather: grad_pos[img][v] += sum over incident corners of partial[pimg][t][corner]
 // (pos_batched: one output per image; else all images sum into one buffer)
 __global__ void __launch_bounds__(256) k_gather_pos(const float4 *__restrict__ partial, int n_img, int F, int V,
@@ -477,7 +377,7 @@ static void launch_fine_v(const RasterLaunch &L, hipStream_t st) {
     int pad = (L.n_tiles + 7) & ~7;
     hipLaunchKernelGGL((k_fine<TW, TH, PEEL, DB>), dim3(pad), dim3(TW * TH), 0, st, L.b, L.pos, L.tri, L.bin_cap,
                        L.bin_count, L.bins, L.list_count, L.lists, L.list_box, L.n_tiles, pad,
-                       (const float4 *)L.prev, (float4 *)L.rast, (float4 *)L.rast_db);
+                       (const float4 *)L.prev, (float4 *)L.rast, (float4 *)L.rast_db, L.boxes);
 }
 
 template <int TW, int TH>
@@ -511,15 +411,21 @@ int launch_rasterize(const RasterLaunch &L, hipStream_t st) {
 int launch_rasterize_bwd(const RasterBwdLaunch &L, hipStream_t st) {
     if (L.total_pix == 0) return 0;
     if (L.boxes && L.partial && L.inc_off && L.inc) {
-        dim3 grid((L.b.F + 255) / 256, L.b.n_img);
-        if (L.grad_db)
-            hipLaunchKernelGGL(k_raster_bwd_tri<true>, grid, dim3(256), 0, st, L.b, L.pos, L.tri,
-                               (const float4 *)L.rast, (const float4 *)L.grad_rast, (const float4 *)L.grad_db,
-                               (const int2 *)L.boxes, (float4 *)L.partial);
-        else
-            hipLaunchKernelGGL(k_raster_bwd_tri<false>, grid, dim3(256), 0, st, L.b, L.pos, L.tri,
-                               (const float4 *)L.rast, (const float4 *)L.grad_rast, (const float4 *)nullptr,
-                               (const int2 *)L.boxes, (float4 *)L.partial);
+        TriBwdLaunch T{};
+        T.b = L.b;
+        T.pos = L.pos;
+        T.tri = L.tri;
+        T.rast = L.rast;
+        T.grad_rast = L.grad_rast;
+        T.grad_db = L.grad_db;
+        T.boxes = L.boxes;
+        T.partial = L.partial;
+        T.n_img = L.b.n_img;
+        T.F = L.b.F;
+        T.H = L.b.H;
+        T.W = L.b.W;
+        T.views_per_block = 1;
+        if (int rc = launch_tri_bwd_pos(T, st)) return rc;
         int n_out = L.b.pos_batched ? L.b.n_img : 1;
         dim3 gg((L.b.V + 255) / 256, n_out);
         hipLaunchKernelGGL(k_gather_pos, gg, dim3(256), 0, st, (const float4 *)L.partial, L.b.n_img, L.b.F, L.b.V,

@@ -175,7 +175,7 @@ def _rasterize_launch(ctx, pos, tri, H, W, rng, prev, want_db):
     if n == 0:
         return rast, rast_db, boxes
     if rng is None and F > 0 and max(H, W) <= 65535:
-        boxes = torch.empty((n, F, 2), dtype=torch.int32, device=dev)
+        boxes = torch.empty((n, F, 3), dtype=torch.int32, device=dev)
     (tw, th), cap = ctx.config(n, H, W, F)
     list_total = rng.list_total if rng is not None else n * F
     max_range = rng.max_range if rng is not None else F
@@ -336,10 +336,12 @@ _TRI_BWD_C = (1, 2, 3, 4, 8, 16)
 
 
 def _views_per_group(n, F):
-    """Views folded into one partial of the triangle-centric attribute backward:
-    enough (triangle, group) threads to fill the GPU, as few partials as that allows."""
-    groups = min(n, max(1, -(-(1 << 19) // max(F, 1))))
-    return -(-n // groups)
+    """Views folded into one partial of the triangle-centric attribute backward
+    (a power of two up to 8: fewer partials to gather, blocks still fill the GPU)."""
+    p = 1
+    while p * 2 <= min(n, 8):
+        p *= 2
+    return p
 
 
 class _InterpolateFunc(torch.autograd.Function):

@@ -51,10 +51,11 @@ int drp_abi_version(void);
  *   rast_db: nullptr unless derivatives are wanted.
  *   tile_w x tile_h in {8x8, 16x8, 16x16, 32x8}; bin_cap >= 1 (performance
  *   only: full bins spill to a per-image list, a full list to a full scan).
- *   boxes (nullable, int32 [n,F,2], 8-byte aligned): packed pixel box of each
- *   (image, triangle), (c0 | c1 << 16, r0 | r1 << 16), c0 > c1 when culled; the
- *   triangle-centric backward passes read it.  Only triangles inside an image's
- *   range are written. */
+ *   boxes (nullable, int32 [n,F,3], 4-byte aligned): per (image, triangle) the
+ *   packed pixel box (c0 | c1 << 16, r0 | r1 << 16), c0 > c1 when culled, and a
+ *   hit flag, nonzero when the triangle won at least one pixel of the image;
+ *   the triangle-centric backward passes read both.  Only triangles inside an
+ *   image's range are written. */
 size_t drp_rasterize_workspace_size(int n_img, int H, int W, long long list_total, int tile_w, int tile_h,
                                     int bin_cap);
 int drp_rasterize_fwd(void *stream, const float *pos, int n_img, int V, int pos_batched, const int32_t *tri, int F,

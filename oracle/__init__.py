@@ -46,7 +46,13 @@ def lib() -> ctypes.CDLL:
         _lib.ora_triidx_to_float.argtypes = [ctypes.c_int32]
         _lib.ora_float_to_triidx.restype = ctypes.c_int32
         _lib.ora_float_to_triidx.argtypes = [ctypes.c_float]
+        _lib.ora_set_threads.argtypes = [ctypes.c_int]
     return _lib
+
+
+def set_threads(n: int):
+    """Threads of the per-pixel forward loops (results do not depend on it)."""
+    lib().ora_set_threads(int(n))
 
 
 def _ptr(a):

@@ -39,6 +39,7 @@
  * ORA_DOUBLE, REAL=double (finite-difference oracle; symbols end in _f64).
  */
 #include <math.h>
+#include <omp.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -81,6 +82,11 @@ float ora_triidx_to_float(int32_t n) {
 int32_t ora_float_to_triidx(float x) {
     return (x <= 16777216.0f) ? (int32_t)x : (int32_t)(f2u(x) - 0x4a800000u);
 }
+/* Threads of the per-pixel forward loops (rasterize_fixed, interpolate,
+ * antialias); every pixel is computed on its own, so results do not depend
+ * on it.  The other passes are sequential. */
+void ora_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+
 void ora_encode_ids(const int32_t *n, float *out, int64_t count) {
     for (int64_t i = 0; i < count; ++i) out[i] = ora_triidx_to_float(n[i]);
 }
@@ -398,6 +404,7 @@ int SFX(ora_rasterize_fixed)(const real *pos, int n_img, int V, int pos_batched,
     int64_t npx = (int64_t)H * W;
     for (int img = 0; img < n_img; ++img) {
         const real *P = SFX(img_pos)(pos, pos_batched, img, V);
+#pragma omp parallel for schedule(static)
         for (int r = 0; r < H; ++r)
             for (int c = 0; c < W; ++c) {
                 int64_t pi = (int64_t)img * npx + (int64_t)r * W + c;
@@ -524,6 +531,7 @@ int SFX(ora_interpolate)(const real *attr, int n_img, int V, int attr_batched, i
     int64_t npx = (int64_t)H * W;
     for (int img = 0; img < n_img; ++img) {
         const real *A = attr_batched ? attr + (int64_t)img * V * C : attr;
+#pragma omp parallel for schedule(static)
         for (int64_t q = 0; q < npx; ++q) {
             int64_t pi = (int64_t)img * npx + q;
             real *o = out + pi * C;
@@ -728,6 +736,7 @@ int SFX(ora_antialias)(const real *color, int n_img, int H, int W, int C, const 
         const real *R = rast + (int64_t)img * npx * 4;
         const real *Cc = color + (int64_t)img * npx * C;
         real *O = out + (int64_t)img * npx * C;
+#pragma omp parallel for schedule(static)
         for (int r = 0; r < H; ++r)
             for (int c = 0; c < W; ++c) {
                 int64_t q = (int64_t)r * W + c;

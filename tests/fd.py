@@ -45,6 +45,42 @@ def pipeline_grad_f64(verts, attr, tri, VP, rast_ids, target, opp):
     return gverts, gattr
 
 
+def crop_vp(VP, H, W, r0, c0, h, w):
+    """Projection of the h x w window at (r0, c0) of an H x W image, as an image of
+    its own: pixel (r, c) of the window has the NDC centre of pixel (r0 + r, c0 + c).
+    x' = (W/w) x + ((W - 2 c0 - w)/w) w_clip (y alike); z and w are unchanged, so
+    barycentrics, depth and screen orientation are those of the full image."""
+    M = np.eye(4)
+    M[0, 0], M[0, 3] = W / w, (W - 2 * c0 - w) / w
+    M[1, 1], M[1, 3] = H / h, (H - 2 * r0 - h) / h
+    return M @ VP
+
+
+def vertex_window(ids, tri, k, margin=2):
+    """Pixel window [r0, r1) x [c0, c1) outside which the loss does not depend on
+    vertex k at frozen visibility: the pixels that show a triangle incident to k,
+    grown by ``margin`` (antialias pairs reach one pixel further).  ids: [H, W]
+    plain triangle + 1.  None when no such pixel exists."""
+    inc = np.flatnonzero((tri == k).any(1)) + 1
+    rr, cc = np.nonzero(np.isin(ids, inc))
+    if rr.size == 0:
+        return None
+    H, W = ids.shape
+    return (max(rr.min() - margin, 0), min(rr.max() + 1 + margin, H),
+            max(cc.min() - margin, 0), min(cc.max() + 1 + margin, W))
+
+
+def window_loss_fn(verts, attr, tri, VP, rast_ids, target, opp, win):
+    """Closure: ``pipeline_f64`` restricted to a window of a single view; it
+    differs from the full-image loss by a constant in the window's vertex."""
+    r0, r1, c0, c1 = win
+    H, W = rast_ids.shape[1:3]
+    VPw = crop_vp(VP[0], H, W, r0, c0, r1 - r0, c1 - c0)[None]
+    ids_w = np.ascontiguousarray(rast_ids[:, r0:r1, c0:c1])
+    tgt_w = np.ascontiguousarray(target[:, r0:r1, c0:c1])
+    return lambda: pipeline_f64(verts, attr, tri, VPw, ids_w, tgt_w, opp)  # noqa: E731
+
+
 def central_diff(f, x, idx, h):
     x0 = x[idx]
     x[idx] = x0 + h

@@ -151,3 +151,34 @@ def test_full_pipeline_fd_world_space():
                     floor=1e-3 * np.abs(gv).max())
     assert n >= 24
     fd_check(f, a, ga, R(16).choice(a.size, 16, replace=False), h=1e-4, rtol=1e-6, floor=1e-6)
+
+
+def test_window_fd_equals_full_image_fd():
+    """The windowed loss used for large-image finite differences moves exactly as
+    the full-image loss does when one vertex moves (visibility frozen)."""
+    from diffrp_amd.geometry import noisy_icosphere
+    from tests.fd import central_diff, clip_f64, vertex_window, window_loss_fn
+    verts_t, tri_t = noisy_icosphere(3, 0.8, 0.01, seed=1)
+    verts, tri = verts_t.double().numpy(), tri_t.numpy()
+    H, W = 96, 80
+    VP = vp_from_direction([0.2, 0.9, 0.4], H).double().numpy()[None]
+    VP[0, 0] *= H / W  # keep the aspect ratio of a W x H image
+    rng = R(31)
+    attr = rng.random((len(verts), 4))
+    target = rng.random((1, H, W, 4))
+    ids = ids_f64(oracle.rasterize(clip_f64(verts, VP).astype(np.float32), tri, (H, W)))
+    opp = oracle.topology(tri)
+    full = lambda: pipeline_f64(verts, attr, tri, VP, ids, target, opp)  # noqa: E731
+    visible = np.unique(tri[np.unique(ids[0, ..., 3]).astype(int)[1:] - 1])
+    checked = 0
+    for k in R(32).choice(visible, 12, replace=False):
+        win = vertex_window(ids[0, ..., 3], tri, k)
+        assert win is not None and (win[1] - win[0]) * (win[3] - win[2]) < H * W // 4
+        local = window_loss_fn(verts, attr, tri, VP, ids, target, opp, win)
+        for x, i in ((verts, 3 * k + int(rng.integers(3))), (attr, 4 * k + int(rng.integers(4)))):
+            flat = x.reshape(-1)
+            d_full = central_diff(full, flat, i, 1e-5)
+            d_win = central_diff(local, flat, i, 1e-5)
+            assert abs(d_full - d_win) <= 1e-6 * max(1.0, abs(d_full)), (k, i, d_full, d_win)
+            checked += 1
+    assert checked == 24
