@@ -91,7 +91,7 @@ class Scene:
         rast, _ = nvdr.rasterize(self.ctx, pos, self.tri, (self.H, self.W), grad_db=False)
         col, _ = nvdr.interpolate(self.attr, rast, self.tri)
         y = nvdr.antialias(col, rast, pos, self.tri, topology_hash=self.topo)
-        loss = ((y - self.target) ** 2).sum()
+        loss = torch.nn.functional.mse_loss(y, self.target, reduction='sum')
         loss.backward()
         return loss
 
@@ -102,7 +102,7 @@ class Scene:
             rast, _ = nvdr.rasterize(self.ctx, pos, self.tri, (self.H, self.W), grad_db=False)
             col, _ = nvdr.interpolate(self.attr, rast, self.tri)
             y = nvdr.antialias(col, rast, pos, self.tri)
-            loss = ((y - self.target[v:v + 1]) ** 2).sum()
+            loss = torch.nn.functional.mse_loss(y, self.target[v:v + 1], reduction='sum')
             loss.backward()
             total = total + loss.detach()
         return total
@@ -178,8 +178,8 @@ def per_op_times(S, iters=10):
         'antialias_fwd': lambda: L.drp_antialias_fwd(s, ptr(col), n, H, W, C, ptr(rast), ptr(pos), V, 1, ptr(S.tri),
                                                      F, ptr(opp), ptr(out), ptr(aws_t), aws),
         'antialias_bwd': lambda: L.drp_antialias_bwd(s, ptr(col), n, H, W, C, ptr(rast), ptr(pos), V, 1, ptr(S.tri),
-                                                     F, ptr(opp), ptr(gout), ptr(gcol), ptr(gpos), 1.0, ptr(aws_t),
-                                                     aws),
+                                                     F, ptr(opp), ptr(gout), ptr(gcol), ptr(gpos), 1.0, 1,
+                                                     ptr(aws_t), aws),
         'interpolate_bwd': lambda: L.drp_interpolate_bwd(s, ptr(attr), n, V, 0, C, ptr(rast), H, W, ptr(S.tri), F,
                                                          None, None, 0, ptr(gout), None, ptr(gattr), ptr(grast),
                                                          None, ptr(boxes), vpg, ptr(off), ptr(inc), ptr(iws_t), iws),

@@ -49,7 +49,7 @@ _SIGS = {
     'drp_antialias_fwd': (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_int, c_int,
                                   c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_size]),
     'drp_antialias_bwd': (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_int, c_int,
-                                  c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_void_p,
+                                  c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_int, c_void_p,
                                   c_size]),
 }
 

@@ -11,8 +11,7 @@
 // t = clamp(t_e, 0, 1); t < 1/2 moves `from` towards `to` by (1/2 - t),
 // otherwise `to` towards `from` by (t - 1/2).  Else the next candidate is tried.
 //
-// Kernels: k_aa_project (per-vertex pixel coordinates), k_aa_orient
-// (per-triangle orientation), k_aa_silhouette (triangles with a possible
+// Kernels: k_aa_orient (per-triangle screen orientation), k_aa_silhouette (triangles with a possible
 // silhouette edge), k_aa_pairs (one thread per pixel evaluates the pairs to its
 // right and above; signed weight per pair), k_aa_gather (output colours in a
 // fixed order), and the backward counterparts.
@@ -101,28 +100,32 @@ int launch_topology(const TopoLaunch &L, hipStream_t st) {
 }
 
 // ----------------------------------------------------------------------------
-// projection / orientation precompute
+// orientation / silhouette precompute
 // ----------------------------------------------------------------------------
+__device__ __forceinline__ double dfma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 __device__ __forceinline__ int orient3(float4 a, float4 b, float4 c) {
-    // sign of det[[x0,x1,x2],[y0,y1,y2],[w0,w1,w2]]
-    float m0 = fma_(b.y, c.w, -(c.y * b.w));
-    float m1 = fma_(a.y, c.w, -(c.y * a.w));
-    float m2 = fma_(a.y, b.w, -(b.y * a.w));
-    float d = fma_(a.x, m0, fma_(-b.x, m1, c.x * m2));
-    return d > 0.f ? 1 : (d < 0.f ? -1 : 0);
+    // sign of det[[x0,x1,x2],[y0,y1,y2],[w0,w1,w2]], in double (edge-on triangles decide silhouettes)
+    double m0 = dfma(b.y, c.w, -((double)c.y * b.w));
+    double m1 = dfma(a.y, c.w, -((double)c.y * a.w));
+    double m2 = dfma(a.y, b.w, -((double)b.y * a.w));
+    double d = dfma(a.x, m0, dfma(-(double)b.x, m1, (double)c.x * m2));
+    return d > 0.0 ? 1 : (d < 0.0 ? -1 : 0);
 }
 
+// pixel coordinates of every vertex, in double (NaN when w <= 0): the pair
+// search reads them for every candidate triangle
 __global__ void k_aa_project(const float *__restrict__ pos, int V, int n_proj, int H, int W,
-                             float2 *__restrict__ proj) {
+                             double2 *__restrict__ proj) {
     long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= (long long)n_proj * V) return;
     float4 p = reinterpret_cast<const float4 *>(pos)[g];
-    float2 o;
+    double2 o;
     if (p.w > 0.f) {
-        o.x = (p.x / p.w + 1.0f) * ((float)W * 0.5f);
-        o.y = (p.y / p.w + 1.0f) * ((float)H * 0.5f);
+        o.x = ((double)p.x / (double)p.w + 1.0) * ((double)W * 0.5);
+        o.y = ((double)p.y / (double)p.w + 1.0) * ((double)H * 0.5);
     } else {
-        o.x = __int_as_float(0x7fc00000);
+        o.x = __longlong_as_double(0x7ff8000000000000ll);
         o.y = o.x;
     }
     proj[g] = o;
@@ -172,12 +175,11 @@ struct PairRes {
     int recv;   // -1 none, 0: a receives, 1: b receives
     float wgt;
     int tri, edge, from_a, clamped;
-    float fa, fb;
 };
 
 struct AAImg {
     const float4 *rast;         // image rast
-    const float2 *proj;         // image's projected vertices
+    const double2 *proj;        // image's vertices in pixel coordinates (NaN: w <= 0)
     const signed char *orient;  // image's triangle orientations
     const unsigned char *silh;  // image's silhouette-capable triangles
     int V, F, H, W;
@@ -206,40 +208,48 @@ __device__ __forceinline__ PairRes eval_pair(const AAImg &I, const int *__restri
         int vi[3] = {t3[0], t3[1], t3[2]};
         if ((unsigned)vi[0] >= (unsigned)I.V || (unsigned)vi[1] >= (unsigned)I.V || (unsigned)vi[2] >= (unsigned)I.V)
             continue;
-        float2 q0 = I.proj[vi[0]], q1 = I.proj[vi[1]], q2 = I.proj[vi[2]];
-        if (isnan(q0.x) || isnan(q1.x) || isnan(q2.x)) continue;
+        // pair geometry in double: a float clip coordinate fixes a vertex to
+        // ~W * 2^-25 pixel, and the crossing needs differences of nearby
+        // pixel positions
+        double X[3], Y[3];
+        bool front = true;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            double2 q = I.proj[vi[k]];
+            front = front && !isnan(q.x);
+            X[k] = q.x;
+            Y[k] = q.y;
+        }
+        if (!front) continue;
         int o = I.orient[t];
         if (o == 0) continue;
-        float X[3] = {q0.x, q1.x, q2.x}, Y[3] = {q0.y, q1.y, q2.y};
-        float fx = (float)(use_a ? ca : cb) + 0.5f, fy = (float)(use_a ? ra : rb) + 0.5f;
-        float tx = (float)(use_a ? cb : ca) + 0.5f, ty = (float)(use_a ? rb : ra) + 0.5f;
+        double fx = (double)(use_a ? ca : cb) + 0.5, fy = (double)(use_a ? ra : rb) + 0.5;
+        double tx = (double)(use_a ? cb : ca) + 0.5, ty = (double)(use_a ? rb : ra) + 0.5;
         int best = -1;
-        float bt = 0.f, bfa = 0.f, bfb = 0.f;
+        double bt = 0.0;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             int j = k == 2 ? 0 : k + 1;
-            float ex = X[j] - X[k], ey = Y[j] - Y[k];
-            float ff = fma_(ex, fy - Y[k], -(ey * (fx - X[k])));
-            float ft = fma_(ex, ty - Y[k], -(ey * (tx - X[k])));
+            double ex = X[j] - X[k], ey = Y[j] - Y[k];
+            double ff = dfma(ex, fy - Y[k], -(ey * (fx - X[k])));
+            double ft = dfma(ex, ty - Y[k], -(ey * (tx - X[k])));
             if (o < 0) { ff = -ff; ft = -ft; }
-            if (!(ft < 0.f)) continue;
-            float te = ff / (ff - ft);
-            if (best < 0 || te < bt) { best = k; bt = te; bfa = ff; bfb = ft; }
+            if (!(ft < 0.0)) continue;
+            double te = ff / (ff - ft);
+            if (best < 0 || te < bt) { best = k; bt = te; }
         }
         if (best < 0) continue;
         int n = opp[3 * (long long)t + best];
         if (n >= 0 && n < I.F && I.orient[n] == o) continue;
         int clamped = 0;
-        if (bt < 0.f) { bt = 0.f; clamped = 1; }
-        if (bt > 1.f) { bt = 1.f; clamped = 1; }
+        if (bt < 0.0) { bt = 0.0; clamped = 1; }
+        if (bt > 1.0) { bt = 1.0; clamped = 1; }
         res.tri = t;
         res.edge = best;
         res.from_a = use_a;
-        res.fa = bfa;
-        res.fb = bfb;
         res.clamped = clamped;
-        if (bt < 0.5f) { res.recv = use_a ? 0 : 1; res.wgt = 0.5f - bt; }
-        else { res.recv = use_a ? 1 : 0; res.wgt = bt - 0.5f; }
+        if (bt < 0.5) { res.recv = use_a ? 0 : 1; res.wgt = (float)(0.5 - bt); }
+        else { res.recv = use_a ? 1 : 0; res.wgt = (float)(bt - 0.5); }
         return res;
     }
     return res;
@@ -265,8 +275,27 @@ __device__ __forceinline__ AAImg aa_img(const AALaunch &L, int img) {
     return I;
 }
 
-// weights[pix] = (signed weight of pair (pix, right), of pair (pix, above))
-__global__ void __launch_bounds__(256) k_aa_pairs(AALaunch L, float2 *__restrict__ weights) {
+// Decision of one pair as the backward needs it: 0 = no blend, else
+// (tri + 1) << 5 | recv << 4 | clamped << 3 | from_a << 2 | edge.
+__device__ __forceinline__ int pack_decision(const PairRes &p) {
+    if (p.recv < 0) return 0;
+    return ((p.tri + 1) << 5) | (p.recv << 4) | (p.clamped << 3) | (p.from_a << 2) | p.edge;
+}
+__device__ __forceinline__ PairRes unpack_decision(int d) {
+    PairRes p;
+    p.recv = d == 0 ? -1 : ((d >> 4) & 1);
+    p.tri = (int)((unsigned)d >> 5) - 1;
+    p.clamped = (d >> 3) & 1;
+    p.from_a = (d >> 2) & 1;
+    p.edge = d & 3;
+    p.wgt = 0.f;
+    return p;
+}
+
+// weights[pix] = (signed weight of pair (pix, right), of pair (pix, above));
+// decisions[pix] (when kept) the two pairs' decisions for the backward
+__global__ void __launch_bounds__(256) k_aa_pairs(AALaunch L, float2 *__restrict__ weights,
+                                                  int2 *__restrict__ decisions) {
     long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     long long npx = (long long)L.H * L.W;
     if (g >= (long long)L.n_img * npx) return;
@@ -275,9 +304,19 @@ __global__ void __launch_bounds__(256) k_aa_pairs(AALaunch L, float2 *__restrict
     int r = (int)(q / L.W), c = (int)(q % L.W);
     AAImg I = aa_img(L, img);
     float2 w = make_float2(0.f, 0.f);
-    if (c + 1 < L.W) w.x = signed_w(eval_pair(I, L.tri, L.opp, r, c, r, c + 1));
-    if (r + 1 < L.H) w.y = signed_w(eval_pair(I, L.tri, L.opp, r, c, r + 1, c));
+    int2 dec = make_int2(0, 0);
+    if (c + 1 < L.W) {
+        PairRes p = eval_pair(I, L.tri, L.opp, r, c, r, c + 1);
+        w.x = signed_w(p);
+        dec.x = pack_decision(p);
+    }
+    if (r + 1 < L.H) {
+        PairRes p = eval_pair(I, L.tri, L.opp, r, c, r + 1, c);
+        w.y = signed_w(p);
+        dec.y = pack_decision(p);
+    }
     weights[g] = w;
+    if (decisions) decisions[g] = dec;
 }
 
 // channel chunk handled by one thread: 4 (one float4) when C is a multiple of 4
@@ -349,72 +388,153 @@ __global__ void __launch_bounds__(256) k_aa_gather(AALaunch L, const float2 *__r
     st<CH>(L.out + ch + g * C, o);
 }
 
-// backward pairs: recompute each pair once, store its weight, scatter the
-// position gradient of the crossing edge's two vertices
-__global__ void __launch_bounds__(256) k_aa_pairs_bwd(AALaunch L, float2 *__restrict__ weights) {
-    long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    long long npx = (long long)L.H * L.W;
-    if (g >= (long long)L.n_img * npx) return;
-    int img = (int)(g / npx);
-    long long q = g - img * npx;
-    int r = (int)(q / L.W), c = (int)(q % L.W);
-    AAImg I = aa_img(L, img);
-    float2 w = make_float2(0.f, 0.f);
-    const float4 *P = reinterpret_cast<const float4 *>(L.pos) + (L.pos_batched ? (long long)img * L.V : 0);
-    float *GP = L.grad_pos + (L.pos_batched ? (long long)img * L.V * 4 : 0);
-    for (int dir = 0; dir < 2; ++dir) {
-        int rb = dir ? r + 1 : r, cb = dir ? c : c + 1;
-        if (rb >= L.H || cb >= L.W) continue;
-        PairRes pr = eval_pair(I, L.tri, L.opp, r, c, rb, cb);
-        float sw = signed_w(pr);
-        if (dir) w.y = sw; else w.x = sw;
-        if (pr.recv < 0 || pr.clamped || L.grad_pos == nullptr) continue;
-        long long ga = g, gb = dir ? g + L.W : g + 1;
-        long long gr = pr.recv == 0 ? ga : gb, gs = pr.recv == 0 ? gb : ga;
-        float dLdw = 0.f;
-        if ((L.C & 3) == 0) {
-            const float4 *g4 = reinterpret_cast<const float4 *>(L.grad_out + gr * L.C);
-            const float4 *cs = reinterpret_cast<const float4 *>(L.color + gs * L.C);
-            const float4 *cr = reinterpret_cast<const float4 *>(L.color + gr * L.C);
-            for (int k4 = 0; k4 < (L.C >> 2); ++k4) {
-                float4 a = g4[k4], b = cs[k4], e = cr[k4];
-                dLdw += a.x * (b.x - e.x) + a.y * (b.y - e.y) + a.z * (b.z - e.z) + a.w * (b.w - e.w);
+// Per-block accumulation of vertex gradients before the global atomics.
+// Device-scope float atomics on a multi-XCD GPU are performed beyond the
+// per-XCD L2 caches, so every one is a memory transaction; neighbouring
+// silhouette pairs share vertices, and summing them per 16 x 16 tile first
+// leaves one atomic per distinct vertex and component.
+constexpr int kAAHash = 1024;
+
+struct VertexAcc {
+    int key[kAAHash];
+    float gx[kAAHash], gy[kAAHash], gw[kAAHash];
+};
+
+__device__ __forceinline__ void vacc_add(VertexAcc &A, float *GP, int v, float gx, float gy, float gw) {
+    unsigned h = ((unsigned)v * 2654435761u) >> 22;  // top 10 bits
+    for (int probe = 0; probe < 32; ++probe) {
+        int k = atomicCAS(&A.key[h], -1, v);
+        if (k == -1 || k == v) {
+            atomicAdd(&A.gx[h], gx);
+            atomicAdd(&A.gy[h], gy);
+            atomicAdd(&A.gw[h], gw);
+            return;
+        }
+        h = (h + 1) & (kAAHash - 1);
+    }
+    float *gp = GP + 4 * (long long)v;  // table crowded: straight to memory
+    atomicAdd(gp + 0, gx);
+    atomicAdd(gp + 1, gy);
+    atomicAdd(gp + 3, gw);
+}
+
+// Position gradient of one blending pair: dL/dw (from the receiver's output
+// gradient and the colour difference) times dw/dt times dt/d(edge vertices).
+// The pair's decisions are the forward's; the derivative is taken in double:
+// t = fa / (fa - fb) is ill-conditioned for edges nearly parallel to the pair.
+__device__ __forceinline__ void pair_pos_grad(const AALaunch &L, const AAImg &I, const float4 *P, float *GP,
+                                              VertexAcc &A, long long g, int r, int c, int dir, const PairRes &pr) {
+    int rb = dir ? r + 1 : r, cb = dir ? c : c + 1;
+    long long ga = g, gb = dir ? g + L.W : g + 1;
+    long long gr = pr.recv == 0 ? ga : gb, gs = pr.recv == 0 ? gb : ga;
+    double dLdw = 0.0;
+    if ((L.C & 3) == 0) {
+        const float4 *g4 = reinterpret_cast<const float4 *>(L.grad_out + gr * L.C);
+        const float4 *cs = reinterpret_cast<const float4 *>(L.color + gs * L.C);
+        const float4 *cr = reinterpret_cast<const float4 *>(L.color + gr * L.C);
+        for (int k4 = 0; k4 < (L.C >> 2); ++k4) {
+            float4 a = g4[k4], b = cs[k4], e = cr[k4];
+            dLdw += (double)a.x * (b.x - e.x) + (double)a.y * (b.y - e.y) + (double)a.z * (b.z - e.z) +
+                    (double)a.w * (b.w - e.w);
+        }
+    } else {
+        for (int ch = 0; ch < L.C; ++ch)
+            dLdw += (double)L.grad_out[gr * L.C + ch] * (L.color[gs * L.C + ch] - L.color[gr * L.C + ch]);
+    }
+    if (dLdw == 0.0) return;
+    bool recv_is_from = (pr.recv == 0) == (pr.from_a != 0);
+    double dLdt = recv_is_from ? -dLdw : dLdw;
+    const int *t3 = L.tri + 3 * (long long)pr.tri;
+    int i = pr.edge, j = pr.edge == 2 ? 0 : pr.edge + 1;
+    int vv[2] = {t3[i], t3[j]};
+    double2 q0 = I.proj[vv[0]], q1 = I.proj[vv[1]];
+    double fxp = (double)(pr.from_a ? c : cb) + 0.5, fyp = (double)(pr.from_a ? r : rb) + 0.5;
+    double txp = (double)(pr.from_a ? cb : c) + 0.5, typ = (double)(pr.from_a ? rb : r) + 0.5;
+    double so = I.orient[pr.tri] < 0 ? -1.0 : 1.0;
+    double ex = q1.x - q0.x, ey = q1.y - q0.y;
+    double fa = so * (ex * (fyp - q0.y) - ey * (fxp - q0.x));
+    double fb = so * (ex * (typ - q0.y) - ey * (txp - q0.x));
+    double den = fa - fb;
+    double dt_dfa = -fb / (den * den), dt_dfb = fa / (den * den);
+    double k = dLdt * so * (double)L.boost;
+    double gx2[2], gy2[2];
+    gx2[0] = k * (dt_dfa * (q1.y - fyp) + dt_dfb * (q1.y - typ));
+    gy2[0] = k * (dt_dfa * (fxp - q1.x) + dt_dfb * (txp - q1.x));
+    gx2[1] = k * (dt_dfa * (fyp - q0.y) + dt_dfb * (typ - q0.y));
+    gy2[1] = k * (dt_dfa * (q0.x - fxp) + dt_dfb * (q0.x - txp));
+    const double hw = (double)L.W * 0.5, hh = (double)L.H * 0.5;
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+        float4 pv = P[vv[m]];
+        double iw = 1.0 / (double)pv.w;
+        double gx = gx2[m] * hw * iw, gy = gy2[m] * hh * iw;
+        vacc_add(A, GP, vv[m], (float)gx, (float)gy, (float)(-(gx * pv.x + gy * pv.y) * iw));
+    }
+}
+
+// Backward pairs, one block per 16 x 16 tile of one image.  With the
+// forward's decisions, read them; otherwise evaluate each pair and store its
+// weight.  Silhouette pairs add the position gradient of their edge's two
+// vertices to the tile's table, flushed with one atomic per vertex component.
+__global__ void __launch_bounds__(256) k_aa_pairs_bwd(AALaunch L, float2 *__restrict__ weights,
+                                                      const int2 *__restrict__ decisions, int tiles_x,
+                                                      int tiles_per_img, int n_blocks_pad) {
+    __shared__ VertexAcc A;
+    const int tid = threadIdx.x;
+    const int bid = xcd_remap(blockIdx.x, n_blocks_pad);
+    const int img = bid / tiles_per_img, lt = bid - img * tiles_per_img;
+    const bool live_block = img < L.n_img;
+    const bool want_pos = L.grad_pos != nullptr;
+    for (int e = tid; e < kAAHash; e += 256) {
+        A.key[e] = -1;
+        A.gx[e] = 0.f;
+        A.gy[e] = 0.f;
+        A.gw[e] = 0.f;
+    }
+    __syncthreads();
+    const int r = (lt / tiles_x) * 16 + (tid >> 4), c = (lt % tiles_x) * 16 + (tid & 15);
+    const long long npx = (long long)L.H * L.W;
+    float *GP = nullptr;
+    if (live_block && r < L.H && c < L.W) {
+        long long g = (long long)img * npx + (long long)r * L.W + c;
+        AAImg I = aa_img(L, img);
+        const float4 *P = reinterpret_cast<const float4 *>(L.pos) + (L.pos_batched ? (long long)img * L.V : 0);
+        GP = want_pos ? L.grad_pos + (L.pos_batched ? (long long)img * L.V * 4 : 0) : nullptr;
+        if (decisions) {
+            int2 d = want_pos ? decisions[g] : make_int2(0, 0);
+            if (d.x != 0) {
+                PairRes pr = unpack_decision(d.x);
+                if (!pr.clamped) pair_pos_grad(L, I, P, GP, A, g, r, c, 0, pr);
+            }
+            if (d.y != 0) {
+                PairRes pr = unpack_decision(d.y);
+                if (!pr.clamped) pair_pos_grad(L, I, P, GP, A, g, r, c, 1, pr);
             }
         } else {
-            for (int ch = 0; ch < L.C; ++ch)
-                dLdw += L.grad_out[gr * L.C + ch] * (L.color[gs * L.C + ch] - L.color[gr * L.C + ch]);
-        }
-        if (dLdw == 0.f) continue;
-        bool recv_is_from = (pr.recv == 0) == (pr.from_a != 0);
-        float dLdt = recv_is_from ? -dLdw : dLdw;
-        float den = pr.fa - pr.fb;
-        float dt_dfa = -pr.fb / (den * den), dt_dfb = pr.fa / (den * den);
-        const int *t3 = L.tri + 3 * (long long)pr.tri;
-        int i = pr.edge, j = pr.edge == 2 ? 0 : pr.edge + 1;
-        float so = I.orient[pr.tri] < 0 ? -1.f : 1.f;
-        float2 qi = I.proj[t3[i]], qj = I.proj[t3[j]];
-        int ra = r, ca = c;
-        float fxp = (float)(pr.from_a ? ca : cb) + 0.5f, fyp = (float)(pr.from_a ? ra : rb) + 0.5f;
-        float txp = (float)(pr.from_a ? cb : ca) + 0.5f, typ = (float)(pr.from_a ? rb : ra) + 0.5f;
-        float k = dLdt * so * L.boost;
-        float gXi = k * (dt_dfa * (qj.y - fyp) + dt_dfb * (qj.y - typ));
-        float gYi = k * (dt_dfa * (fxp - qj.x) + dt_dfb * (txp - qj.x));
-        float gXj = k * (dt_dfa * (fyp - qi.y) + dt_dfb * (typ - qi.y));
-        float gYj = k * (dt_dfa * (qi.x - fxp) + dt_dfb * (qi.x - txp));
-        float hw = (float)L.W * 0.5f, hh = (float)L.H * 0.5f;
-        int vv[2] = {t3[i], t3[j]};
-        float gx2[2] = {gXi, gXj}, gy2[2] = {gYi, gYj};
-#pragma unroll
-        for (int m = 0; m < 2; ++m) {
-            float4 pv = P[vv[m]];
-            float *gp = GP + 4 * (long long)vv[m];
-            float iw = 1.f / pv.w;
-            atomicAdd(gp + 0, gx2[m] * hw * iw);
-            atomicAdd(gp + 1, gy2[m] * hh * iw);
-            atomicAdd(gp + 3, -(gx2[m] * hw * pv.x + gy2[m] * hh * pv.y) * iw * iw);
+            float2 w = make_float2(0.f, 0.f);
+            for (int dir = 0; dir < 2; ++dir) {
+                int rb = dir ? r + 1 : r, cb = dir ? c : c + 1;
+                if (rb >= L.H || cb >= L.W) continue;
+                PairRes pr = eval_pair(I, L.tri, L.opp, r, c, rb, cb);
+                float sw = signed_w(pr);
+                if (dir) w.y = sw; else w.x = sw;
+                if (pr.recv < 0 || pr.clamped || !want_pos) continue;
+                pair_pos_grad(L, I, P, GP, A, g, r, c, dir, pr);
+            }
+            weights[g] = w;
         }
     }
-    weights[g] = w;
+    if (!want_pos || !live_block) return;  // uniform across the block
+    __syncthreads();
+    float *G = L.grad_pos + (L.pos_batched ? (long long)img * L.V * 4 : 0);
+    for (int e = tid; e < kAAHash; e += 256) {
+        int v = A.key[e];
+        if (v < 0) continue;
+        float *gp = G + 4 * (long long)v;
+        atomicAdd(gp + 0, A.gx[e]);
+        atomicAdd(gp + 1, A.gy[e]);
+        atomicAdd(gp + 3, A.gw[e]);
+    }
 }
 
 template <int CV>
@@ -477,7 +597,7 @@ __global__ void __launch_bounds__(256) k_aa_gather_bwd(AALaunch L, const float2 
         default: hipLaunchKernelGGL(KERNEL<0>, grid, block, 0, st, __VA_ARGS__); break;      \
     }
 
-static void launch_project(const AALaunch &L, hipStream_t st) {
+static void launch_orient(const AALaunch &L, hipStream_t st) {
     int n_proj = L.pos_batched ? L.n_img : 1;
     long long nv = (long long)n_proj * L.V, nf = (long long)n_proj * L.F;
     if (nv) hipLaunchKernelGGL(k_aa_project, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, st, L.pos, L.V, n_proj,
@@ -498,9 +618,9 @@ static dim3 gather_grid(const AALaunch &L, long long total) {
 int launch_antialias(const AALaunch &L, hipStream_t st) {
     long long total = (long long)L.n_img * L.H * L.W;
     if (total == 0) return 0;
-    launch_project(L, st);
+    launch_orient(L, st);
     dim3 grid((unsigned)((total + 255) / 256)), block(256);
-    hipLaunchKernelGGL(k_aa_pairs, grid, block, 0, st, L, L.weights);
+    hipLaunchKernelGGL(k_aa_pairs, grid, block, 0, st, L, L.weights, L.decisions);
     grid = gather_grid(L, total);
     DRP_AA_DISPATCH(k_aa_gather, L, (const float2 *)L.weights)
     return hipGetLastError() == hipSuccess ? 0 : -2;
@@ -509,10 +629,15 @@ int launch_antialias(const AALaunch &L, hipStream_t st) {
 int launch_antialias_bwd(const AALaunch &L, hipStream_t st) {
     long long total = (long long)L.n_img * L.H * L.W;
     if (total == 0) return 0;
-    launch_project(L, st);
-    dim3 grid((unsigned)((total + 255) / 256)), block(256);
-    hipLaunchKernelGGL(k_aa_pairs_bwd, grid, block, 0, st, L, L.weights);
-    grid = gather_grid(L, total);
+    // with the forward's state, projections, orientations, weights and decisions are in place
+    if (!L.reuse) launch_orient(L, st);
+    int tiles_x = (L.W + 15) / 16, tiles_per_img = tiles_x * ((L.H + 15) / 16);
+    long long nb = (long long)tiles_per_img * L.n_img, nb_pad = (nb + 7) & ~7ll;
+    dim3 block(256);
+    hipLaunchKernelGGL(k_aa_pairs_bwd, dim3((unsigned)nb_pad), block, 0, st, L, L.weights,
+                       L.reuse ? (const int2 *)L.decisions : (const int2 *)nullptr, tiles_x, tiles_per_img,
+                       (int)nb_pad);
+    dim3 grid = gather_grid(L, total);
     DRP_AA_DISPATCH(k_aa_gather_bwd, L, (const float2 *)L.weights)
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }

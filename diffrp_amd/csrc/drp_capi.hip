@@ -31,6 +31,8 @@ int fail(int code, const char *fmt, ...) {
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+constexpr int kMaxDecisionTris = (1 << 27) - 1;
+
 bool tile_ok(int tw, int th) {
     return (tw == 8 && th == 8) || (tw == 16 && th == 16) || (tw == 32 && th == 8) || (tw == 16 && th == 8);
 }
@@ -354,8 +356,8 @@ int drp_topology_build(void *stream, const int32_t *tri, int F, int32_t *opp, vo
 size_t drp_antialias_workspace_size(int n_img, int H, int W, int V, int F, int pos_batched) {
     if (n_img <= 0) return 0;
     size_t n_proj = pos_batched ? (size_t)n_img : 1;
-    return align256(n_proj * (size_t)V * 8) + 2 * align256(n_proj * (size_t)F) +
-           align256((size_t)n_img * H * W * 8);
+    size_t npx = (size_t)n_img * H * W;
+    return align256(n_proj * (size_t)V * 16) + 2 * align256(n_proj * (size_t)F) + 2 * align256(npx * 8);
 }
 
 static int aa_common(AALaunch &L, void *stream, const float *color, int n_img, int H, int W, int C,
@@ -384,12 +386,16 @@ static int aa_common(AALaunch &L, void *stream, const float *color, int n_img, i
     L.F = F;
     L.pos_batched = pos_batched;
     L.cv = (C == 1 || C == 3 || C == 4 || C == 8 || C == 16) ? C : 0;
-    size_t o_proj = 0, o_orient = align256(n_proj * (size_t)V * 8);
+    size_t npx = (size_t)n_img * H * W;
+    size_t o_proj = 0, o_orient = align256(n_proj * (size_t)V * 16);
     size_t o_silh = o_orient + align256(n_proj * (size_t)F), o_w = o_silh + align256(n_proj * (size_t)F);
-    L.proj = (float2 *)(b + o_proj);
+    size_t o_dec = o_w + align256(npx * 8);
+    L.proj = (double2 *)(b + o_proj);
     L.orient = (signed char *)(b + o_orient);
     L.silh = (unsigned char *)(b + o_silh);
     L.weights = (float2 *)(b + o_w);
+    // pair decisions pack the triangle into 27 bits; larger meshes recompute them in the backward
+    L.decisions = F < kMaxDecisionTris ? (int2 *)(b + o_dec) : nullptr;
     (void)stream;
     return 0;
 }
@@ -413,7 +419,7 @@ int drp_antialias_fwd(void *stream, const float *color, int n_img, int H, int W,
 int drp_antialias_bwd(void *stream, const float *color, int n_img, int H, int W, int C, const float *rast,
                       const float *pos, int V, int pos_batched, const int32_t *tri, int F, const int32_t *opp,
                       const float *grad_out, float *grad_color, float *grad_pos, float pos_gradient_boost,
-                      void *workspace, size_t workspace_bytes) {
+                      int reuse_forward, void *workspace, size_t workspace_bytes) {
     if (n_img == 0) return 0;
     AALaunch L{};
     if (int e = aa_common(L, stream, color, n_img, H, W, C, rast, pos, V, pos_batched, tri, F, opp, workspace,
@@ -426,6 +432,7 @@ int drp_antialias_bwd(void *stream, const float *color, int n_img, int H, int W,
     L.grad_color = grad_color;
     L.grad_pos = grad_pos;
     L.boost = pos_gradient_boost;
+    L.reuse = reuse_forward && L.decisions != nullptr;
     int rc = launch_antialias_bwd(L, (hipStream_t)stream);
     if (rc) return fail(rc, "antialias_bwd launch failed: %s", hipGetErrorString(hipGetLastError()));
     return 0;

@@ -81,10 +81,12 @@ struct AALaunch {
     const int *tri, *opp;
     int n_img, H, W, C, V, F, pos_batched;
     int cv;           // compiled channel variant (0 = generic loop)
-    float2 *proj;
+    double2 *proj;               // [n_proj, V] pixel coordinates (NaN: w <= 0)
     signed char *orient;
     unsigned char *silh;
     float2 *weights;
+    int2 *decisions;             // [n, H, W] pair decisions for the backward (nullptr: not kept)
+    int reuse;                   // backward: workspace holds the forward's state
     float *out;
     const float *grad_out;
     float *grad_color, *grad_pos;

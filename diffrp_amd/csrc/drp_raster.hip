@@ -328,7 +328,7 @@ __global__ void __launch_bounds__(256) k_raster_bwd(Batch b, const float *__rest
     long long q = gid - d.pix_off;
     int r = (int)(q / d.W), c = (int)(q % d.W);
     float4 o[3];
-    raster_pixel_grad<DB>(v, pix_ndc(c, d.W), pix_ndc(r, d.H), g, gd, o);
+    raster_pixel_grad<DB>(v, c, r, d.H, d.W, g, gd, o);
     const int *t3 = tri + 3 * (long long)t;
     float *G = grad_pos + d.pos_off;
 #pragma unroll

@@ -133,7 +133,7 @@ __global__ void __launch_bounds__(256) k_tri_bwd(TriBwdLaunch L) {
                     float4 gd = make_float4(0.f, 0.f, 0.f, 0.f);
                     if (DB) gd = db_grad_scaled(reinterpret_cast<const float4 *>(L.grad_db)[pix], L.H, L.W);
                     float4 o[3];
-                    raster_pixel_grad<DB>(iverts[k], pix_ndc(c, L.W), pix_ndc(r, L.H), g, gd, o);
+                    raster_pixel_grad<DB>(iverts[k], c, r, L.H, L.W, g, gd, o);
 #pragma unroll
                     for (int q = 0; q < 3; ++q) {
                         val[4 * q] = o[q].x; val[4 * q + 1] = o[q].y; val[4 * q + 2] = o[q].z; val[4 * q + 3] = o[q].w;

@@ -483,6 +483,9 @@ class _AntialiasFunc(torch.autograd.Function):
         check(rc, 'antialias')
         fctx.save_for_backward(color, rast, pos, tri, opp)
         fctx.boost = boost
+        # the workspace keeps the pair decisions; the backward reads them
+        # (saved tensors cannot change in place before backward, torch checks)
+        fctx.ws = ws
         return out
 
     @staticmethod
@@ -495,10 +498,11 @@ class _AntialiasFunc(torch.autograd.Function):
         gcol = torch.empty_like(color)
         gpos = torch.zeros_like(pos) if fctx.needs_input_grad[2] else None
         L = _lib.lib()
-        ws_bytes = L.drp_antialias_workspace_size(n, H, W, V, F, batched)
-        ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=color.device)
+        ws = fctx.ws
         rc = L.drp_antialias_bwd(stream_of(color), ptr(color), n, H, W, C, ptr(rast), ptr(pos), V, batched, ptr(tri),
-                                 F, ptr(opp), ptr(dy), ptr(gcol), ptr(gpos), float(fctx.boost), ptr(ws), ws_bytes)
+                                 F, ptr(opp), ptr(dy), ptr(gcol), ptr(gpos), float(fctx.boost), 1, ptr(ws),
+                                 ws.numel())
+        fctx.ws = None
         check(rc, 'antialias backward')
         return gcol, None, gpos, None, None, None
 

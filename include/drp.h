@@ -123,11 +123,15 @@ int drp_antialias_fwd(void *stream, const float *color, int n_img, int H, int W,
                       float *out, void *workspace, size_t workspace_bytes);
 
 /* Backward of antialias: grad_color written; grad_pos accumulated (caller
- * zero-fills; may be null to skip), scaled by pos_gradient_boost. */
+ * zero-fills; may be null to skip), scaled by pos_gradient_boost.
+ * reuse_forward != 0: the workspace still holds what drp_antialias_fwd left
+ * there for the same rast, pos and tri (colour may differ), so the pair
+ * search is not repeated (ignored for F >= 2^27 - 1, where the forward keeps
+ * no pair decisions). */
 int drp_antialias_bwd(void *stream, const float *color, int n_img, int H, int W, int C, const float *rast,
                       const float *pos, int V, int pos_batched, const int32_t *tri, int F, const int32_t *opp,
                       const float *grad_out, float *grad_color, float *grad_pos, float pos_gradient_boost,
-                      void *workspace, size_t workspace_bytes);
+                      int reuse_forward, void *workspace, size_t workspace_bytes);
 
 /* ---- edge gradient -----------------------------------------------------------
  * Replaces the backward of _EdgeGradient (diffrp/rendering/surface_deferred.py:

@@ -130,6 +130,72 @@ int ora_topology(const int32_t *tri, int F, int32_t *opp) {
 #endif
 
 /* ------------------------------------------------------------------------- */
+/* rasterize backward of one pixel, in double precision (both variants)        */
+/* ------------------------------------------------------------------------- */
+/* d(u, v, z/w) at pixel (r, c) of the triangle with clip corners (x, y, z, w)_k,
+ * plus d rast_db when gd (already scaled by 2/W, 2/H) is given, chained to
+ * d(x, y, z, w)_k.  u = e0/s, v = e1/s, z/w = sum e_k z_k / sum e_k w_k with
+ * e_k the 2-D homogeneous edge functions of X_k = x_k - w_k px, Y_k = y_k - w_k py. */
+static void pix_grad_d(const double x[3], const double y[3], const double z[3], const double w[3], int r, int c,
+                       int H, int W, double gu, double gv, double gz, const double *gd, double out[3][4]) {
+    double px = (double)(2 * c + 1 - W) / (double)W, py = (double)(2 * r + 1 - H) / (double)H;
+    double X[3], Y[3], e[3];
+    for (int k = 0; k < 3; ++k) {
+        X[k] = fma(-w[k], px, x[k]);
+        Y[k] = fma(-w[k], py, y[k]);
+    }
+    e[0] = fma(X[1], Y[2], -(X[2] * Y[1]));
+    e[1] = fma(X[2], Y[0], -(X[0] * Y[2]));
+    e[2] = fma(X[0], Y[1], -(X[1] * Y[0]));
+    double s = (e[0] + e[1]) + e[2];
+    double zn = fma(e[0], z[0], fma(e[1], z[1], e[2] * z[2]));
+    double wn = fma(e[0], w[0], fma(e[1], w[1], e[2] * w[2]));
+    double u = e[0] / s, v = e[1] / s, zw = zn / wn;
+    double gX[3] = {0, 0, 0}, gY[3] = {0, 0, 0}, gw[3] = {0, 0, 0}, gs = 0;
+    if (gd) {
+        /* rast_db = (e_kx - u sx)/s etc. (scaled); chain into e_kx, e_ky, u, v, s */
+        double ex[3], ey[3];
+        ex[0] = fma(w[2], Y[1], -(w[1] * Y[2]));
+        ex[1] = fma(w[0], Y[2], -(w[2] * Y[0]));
+        ex[2] = fma(w[1], Y[0], -(w[0] * Y[1]));
+        ey[0] = fma(w[1], X[2], -(w[2] * X[1]));
+        ey[1] = fma(w[2], X[0], -(w[0] * X[2]));
+        ey[2] = fma(w[0], X[1], -(w[1] * X[0]));
+        double sx = ex[0] + ex[1] + ex[2], sy = ey[0] + ey[1] + ey[2];
+        double Aux = ex[0] - u * sx, Auy = ey[0] - u * sy, Avx = ex[1] - v * sx, Avy = ey[1] - v * sy;
+        double a = gd[0] / s, b = gd[1] / s, cc = gd[2] / s, d = gd[3] / s;
+        double gex[3] = {a, cc, 0}, gey[3] = {b, d, 0};
+        double gsx = -(a * u + cc * v), gsy = -(b * u + d * v);
+        for (int k = 0; k < 3; ++k) { gex[k] += gsx; gey[k] += gsy; }
+        gu += -(a * sx + b * sy);
+        gv += -(cc * sx + d * sy);
+        gs += -(a * Aux + b * Auy + cc * Avx + d * Avy) / s;
+        /* e_kx, e_ky in terms of w, X, Y */
+        gw[2] += gex[0] * Y[1]; gY[1] += gex[0] * w[2]; gw[1] -= gex[0] * Y[2]; gY[2] -= gex[0] * w[1];
+        gw[0] += gex[1] * Y[2]; gY[2] += gex[1] * w[0]; gw[2] -= gex[1] * Y[0]; gY[0] -= gex[1] * w[2];
+        gw[1] += gex[2] * Y[0]; gY[0] += gex[2] * w[1]; gw[0] -= gex[2] * Y[1]; gY[1] -= gex[2] * w[0];
+        gw[1] += gey[0] * X[2]; gX[2] += gey[0] * w[1]; gw[2] -= gey[0] * X[1]; gX[1] -= gey[0] * w[2];
+        gw[2] += gey[1] * X[0]; gX[0] += gey[1] * w[2]; gw[0] -= gey[1] * X[2]; gX[2] -= gey[1] * w[0];
+        gw[0] += gey[2] * X[1]; gX[1] += gey[2] * w[0]; gw[1] -= gey[2] * X[0]; gX[0] -= gey[2] * w[1];
+    }
+    double G0 = (gu * (1 - u) + gv * (-v)) / s + gs + gz * (z[0] - zw * w[0]) / wn;
+    double G1 = (gu * (-u) + gv * (1 - v)) / s + gs + gz * (z[1] - zw * w[1]) / wn;
+    double G2 = (gu * (-u) + gv * (-v)) / s + gs + gz * (z[2] - zw * w[2]) / wn;
+    gX[0] += G2 * Y[1] - G1 * Y[2];
+    gX[1] += G0 * Y[2] - G2 * Y[0];
+    gX[2] += G1 * Y[0] - G0 * Y[1];
+    gY[0] += G1 * X[2] - G2 * X[1];
+    gY[1] += G2 * X[0] - G0 * X[2];
+    gY[2] += G0 * X[1] - G1 * X[0];
+    for (int k = 0; k < 3; ++k) {
+        out[k][0] = gX[k];
+        out[k][1] = gY[k];
+        out[k][2] = gz * e[k] / wn;
+        out[k][3] = gw[k] - px * gX[k] - py * gY[k] - gz * zw * e[k] / wn;
+    }
+}
+
+/* ------------------------------------------------------------------------- */
 /* triangle setup                                                              */
 /* ------------------------------------------------------------------------- */
 typedef struct {
@@ -447,11 +513,14 @@ int SFX(ora_rasterize_fixed)(const real *pos, int n_img, int V, int pos_batched,
 /* ------------------------------------------------------------------------- */
 /* grad_pos accumulates (+=): [n, V, 4] when pos_batched else [V, 4].
  * grad_rast channels (u, v, z/w) are used; grad_db (nullable) adds the
- * derivative of rast_db. */
+ * derivative of rast_db.  The per-pixel chain runs in double precision in
+ * both variants (pix_grad_d): thin triangles seen edge-on lose ~|e|/s of the
+ * relative precision of their derivatives in float. */
 int SFX(ora_rasterize_bwd)(const real *pos, int n_img, int V, int pos_batched, const int32_t *tri, int F,
                            int H, int W, const real *rast, const real *grad_rast, const real *grad_db,
                            real *grad_pos) {
     int64_t npx = (int64_t)H * W;
+    const real kx = (real)2 / (real)W, ky = (real)2 / (real)H;
     for (int img = 0; img < n_img; ++img) {
         const real *P = SFX(img_pos)(pos, pos_batched, img, V);
         real *G = pos_batched ? grad_pos + (int64_t)img * V * 4 : grad_pos;
@@ -461,57 +530,23 @@ int SFX(ora_rasterize_bwd)(const real *pos, int n_img, int V, int pos_batched, c
                 int32_t id = rast[4 * pi + 3] > 0 ? SFX(decode_id)(rast[4 * pi + 3]) : 0;
                 if (id <= 0 || id > F) continue;
                 const int32_t *t3 = tri + 3 * (int64_t)(id - 1);
-                SFX(ora_setup) s;
-                SFX(setup_tri)(P, V, t3, H, W, &s);
-                SFX(ora_pix) p;
-                SFX(eval_pixel)(&s, r, c, H, W, &p);
-                real gu = grad_rast[4 * pi + 0], gv = grad_rast[4 * pi + 1], gz = grad_rast[4 * pi + 2];
-                real u = p.e[0] / p.s, v = p.e[1] / p.s, zw = p.zn / p.wn;
-                real gX[3] = {0, 0, 0}, gY[3] = {0, 0, 0}, gw[3] = {0, 0, 0}, gs = 0;
-                if (grad_db) {
-                    /* rast_db = k * (e_kx - u sx)/s etc.; chain into e_kx, e_ky, u, v, s */
-                    const real *gd = grad_db + 4 * pi;
-                    real ex[3], ey[3];
-                    ex[0] = FMA(s.w[2], p.Y[1], -(s.w[1] * p.Y[2]));
-                    ex[1] = FMA(s.w[0], p.Y[2], -(s.w[2] * p.Y[0]));
-                    ex[2] = FMA(s.w[1], p.Y[0], -(s.w[0] * p.Y[1]));
-                    ey[0] = FMA(s.w[1], p.X[2], -(s.w[2] * p.X[1]));
-                    ey[1] = FMA(s.w[2], p.X[0], -(s.w[0] * p.X[2]));
-                    ey[2] = FMA(s.w[0], p.X[1], -(s.w[1] * p.X[0]));
-                    real sx = ex[0] + ex[1] + ex[2], sy = ey[0] + ey[1] + ey[2];
-                    real kx = (real)2 / (real)W, ky = (real)2 / (real)H;
-                    real Aux = ex[0] - u * sx, Auy = ey[0] - u * sy, Avx = ex[1] - v * sx, Avy = ey[1] - v * sy;
-                    real a = gd[0] * kx / p.s, b = gd[1] * ky / p.s, cc = gd[2] * kx / p.s, d = gd[3] * ky / p.s;
-                    real gex[3] = {a, cc, 0}, gey[3] = {b, d, 0};
-                    real gsx = -(a * u + cc * v), gsy = -(b * u + d * v);
-                    for (int k = 0; k < 3; ++k) { gex[k] += gsx; gey[k] += gsy; }
-                    gu += -(a * sx + b * sy);
-                    gv += -(cc * sx + d * sy);
-                    gs += -(a * Aux + b * Auy + cc * Avx + d * Avy) / p.s;
-                    /* e_kx, e_ky in terms of w, X, Y */
-                    gw[2] += gex[0] * p.Y[1]; gY[1] += gex[0] * s.w[2]; gw[1] -= gex[0] * p.Y[2]; gY[2] -= gex[0] * s.w[1];
-                    gw[0] += gex[1] * p.Y[2]; gY[2] += gex[1] * s.w[0]; gw[2] -= gex[1] * p.Y[0]; gY[0] -= gex[1] * s.w[2];
-                    gw[1] += gex[2] * p.Y[0]; gY[0] += gex[2] * s.w[1]; gw[0] -= gex[2] * p.Y[1]; gY[1] -= gex[2] * s.w[0];
-                    gw[1] += gey[0] * p.X[2]; gX[2] += gey[0] * s.w[1]; gw[2] -= gey[0] * p.X[1]; gX[1] -= gey[0] * s.w[2];
-                    gw[2] += gey[1] * p.X[0]; gX[0] += gey[1] * s.w[2]; gw[0] -= gey[1] * p.X[2]; gX[2] -= gey[1] * s.w[0];
-                    gw[0] += gey[2] * p.X[1]; gX[1] += gey[2] * s.w[0]; gw[1] -= gey[2] * p.X[0]; gX[0] -= gey[2] * s.w[1];
+                if (t3[0] < 0 || t3[0] >= V || t3[1] < 0 || t3[1] >= V || t3[2] < 0 || t3[2] >= V) continue;
+                double cx[3], cy[3], cz[3], cw[3];
+                for (int k = 0; k < 3; ++k) {
+                    const real *pv = P + 4 * (int64_t)t3[k];
+                    cx[k] = pv[0]; cy[k] = pv[1]; cz[k] = pv[2]; cw[k] = pv[3];
                 }
-                real G0 = (gu * ((real)1 - u) + gv * (-v)) / p.s + gs + gz * (s.z[0] - zw * s.w[0]) / p.wn;
-                real G1 = (gu * (-u) + gv * ((real)1 - v)) / p.s + gs + gz * (s.z[1] - zw * s.w[1]) / p.wn;
-                real G2 = (gu * (-u) + gv * (-v)) / p.s + gs + gz * (s.z[2] - zw * s.w[2]) / p.wn;
-                gX[0] += G2 * p.Y[1] - G1 * p.Y[2];
-                gX[1] += G0 * p.Y[2] - G2 * p.Y[0];
-                gX[2] += G1 * p.Y[0] - G0 * p.Y[1];
-                gY[0] += G1 * p.X[2] - G2 * p.X[1];
-                gY[1] += G2 * p.X[0] - G0 * p.X[2];
-                gY[2] += G0 * p.X[1] - G1 * p.X[0];
+                double gd[4], go[3][4];
+                if (grad_db) {
+                    const real *q = grad_db + 4 * pi;
+                    gd[0] = (double)(q[0] * kx); gd[1] = (double)(q[1] * ky);
+                    gd[2] = (double)(q[2] * kx); gd[3] = (double)(q[3] * ky);
+                }
+                pix_grad_d(cx, cy, cz, cw, r, c, H, W, grad_rast[4 * pi + 0], grad_rast[4 * pi + 1],
+                           grad_rast[4 * pi + 2], grad_db ? gd : NULL, go);
                 for (int k = 0; k < 3; ++k) {
                     real *g = G + 4 * (int64_t)t3[k];
-                    real ek = p.e[k];
-                    g[0] += gX[k];
-                    g[1] += gY[k];
-                    g[2] += gz * ek / p.wn;
-                    g[3] += gw[k] - p.px * gX[k] - p.py * gY[k] - gz * zw * ek / p.wn;
+                    for (int j = 0; j < 4; ++j) g[j] += (real)go[k][j];
                 }
             }
     }
@@ -641,34 +676,38 @@ typedef struct {
     real wgt;
     int tri, edge;  /* triangle and edge (k -> k+1) that produced the blend  */
     int from_a;     /* the candidate triangle was seen at a                  */
-    real fa, fb;    /* edge function at from / to                             */
+    double fa, fb;  /* edge function at from / to                             */
     int clamped;    /* t was clamped: no position gradient                   */
 } SFX(ora_pair);
 
-static int SFX(proj_tri)(const real *P, int V, const int32_t *t3, int H, int W, real X[3], real Y[3], real cw[3][4]) {
+/* Pair geometry (projection, orientation, edge functions, crossing) is
+ * evaluated in double from the clip coordinates in both variants: a float clip
+ * coordinate fixes a vertex to ~W * 2^-25 pixel, and the differences of
+ * nearby pixel positions that the crossing needs keep few of those digits. */
+static int SFX(proj_tri)(const real *P, int V, const int32_t *t3, int H, int W, double X[3], double Y[3]) {
     for (int k = 0; k < 3; ++k) {
         int32_t vi = t3[k];
         if (vi < 0 || vi >= V) return 0;
-        for (int j = 0; j < 4; ++j) cw[k][j] = P[4 * (int64_t)vi + j];
-        if (!(cw[k][3] > 0)) return 0;
-        X[k] = (cw[k][0] / cw[k][3] + (real)1) * ((real)W * (real)0.5);
-        Y[k] = (cw[k][1] / cw[k][3] + (real)1) * ((real)H * (real)0.5);
+        double x = P[4 * (int64_t)vi], y = P[4 * (int64_t)vi + 1], w = P[4 * (int64_t)vi + 3];
+        if (!(w > 0)) return 0;
+        X[k] = (x / w + 1.0) * ((double)W * 0.5);
+        Y[k] = (y / w + 1.0) * ((double)H * 0.5);
     }
     return 1;
 }
 
 /* sign of det[[x0,x1,x2],[y0,y1,y2],[w0,w1,w2]] (screen orientation for w > 0) */
 static int SFX(orient)(const real *P, int V, const int32_t *t3) {
-    real a[3][3];
+    double a[3][3];
     for (int k = 0; k < 3; ++k) {
         int32_t vi = t3[k];
         if (vi < 0 || vi >= V) return 0;
         a[k][0] = P[4 * (int64_t)vi]; a[k][1] = P[4 * (int64_t)vi + 1]; a[k][2] = P[4 * (int64_t)vi + 3];
     }
-    real m0 = FMA(a[1][1], a[2][2], -(a[2][1] * a[1][2]));
-    real m1 = FMA(a[0][1], a[2][2], -(a[2][1] * a[0][2]));
-    real m2 = FMA(a[0][1], a[1][2], -(a[1][1] * a[0][2]));
-    real d = FMA(a[0][0], m0, FMA(-a[1][0], m1, a[2][0] * m2));
+    double m0 = fma(a[1][1], a[2][2], -(a[2][1] * a[1][2]));
+    double m1 = fma(a[0][1], a[2][2], -(a[2][1] * a[0][2]));
+    double m2 = fma(a[0][1], a[1][2], -(a[1][1] * a[0][2]));
+    double d = fma(a[0][0], m0, fma(-a[1][0], m1, a[2][0] * m2));
     return d > 0 ? 1 : (d < 0 ? -1 : 0);
 }
 
@@ -690,22 +729,22 @@ static void SFX(eval_pair)(const real *P, int V, const int32_t *tri, int F, cons
         if (id == 0) continue;
         int t = id - 1;
         const int32_t *t3 = tri + 3 * (int64_t)t;
-        real X[3], Y[3], cw[3][4];
-        if (!SFX(proj_tri)(P, V, t3, H, W, X, Y, cw)) continue;
+        double X[3], Y[3];
+        if (!SFX(proj_tri)(P, V, t3, H, W, X, Y)) continue;
         int o = SFX(orient)(P, V, t3);
         if (o == 0) continue;
-        real fx = (real)(use_a ? ca : cb) + (real)0.5, fy = (real)(use_a ? ra : rb) + (real)0.5;
-        real tx = (real)(use_a ? cb : ca) + (real)0.5, ty = (real)(use_a ? rb : ra) + (real)0.5;
+        double fx = (double)(use_a ? ca : cb) + 0.5, fy = (double)(use_a ? ra : rb) + 0.5;
+        double tx = (double)(use_a ? cb : ca) + 0.5, ty = (double)(use_a ? rb : ra) + 0.5;
         int best = -1;
-        real bt = 0, bfa = 0, bfb = 0;
+        double bt = 0, bfa = 0, bfb = 0;
         for (int k = 0; k < 3; ++k) {
             int j = (k + 1) % 3;
-            real ex = X[j] - X[k], ey = Y[j] - Y[k];
-            real ff = FMA(ex, fy - Y[k], -(ey * (fx - X[k])));
-            real ft = FMA(ex, ty - Y[k], -(ey * (tx - X[k])));
+            double ex = X[j] - X[k], ey = Y[j] - Y[k];
+            double ff = fma(ex, fy - Y[k], -(ey * (fx - X[k])));
+            double ft = fma(ex, ty - Y[k], -(ey * (tx - X[k])));
             if (o < 0) { ff = -ff; ft = -ft; }
             if (!(ft < 0)) continue;
-            real te = ff / (ff - ft);
+            double te = ff / (ff - ft);
             if (best < 0 || te < bt) { best = k; bt = te; bfa = ff; bfb = ft; }
         }
         if (best < 0) continue;
@@ -720,8 +759,8 @@ static void SFX(eval_pair)(const real *P, int V, const int32_t *tri, int F, cons
         res->fa = bfa;
         res->fb = bfb;
         res->clamped = clamped;
-        if (bt < (real)0.5) { res->recv = use_a ? 0 : 1; res->wgt = (real)0.5 - bt; }
-        else { res->recv = use_a ? 1 : 0; res->wgt = bt - (real)0.5; }
+        if (bt < 0.5) { res->recv = use_a ? 0 : 1; res->wgt = (real)(0.5 - bt); }
+        else { res->recv = use_a ? 1 : 0; res->wgt = (real)(bt - 0.5); }
         return;
     }
 }
@@ -785,42 +824,54 @@ int SFX(ora_antialias_bwd)(const real *color, int n_img, int H, int W, int C, co
                     if (pr.recv < 0) continue;
                     int64_t qa = (int64_t)ra * W + ca, qb = (int64_t)rb * W + cb;
                     int64_t qr = pr.recv == 0 ? qa : qb, qs = pr.recv == 0 ? qb : qa;
-                    real dLdw = 0;
+                    double dLdw = 0;
                     for (int ch = 0; ch < C; ++ch) {
                         real g = Go[qr * C + ch];
                         Gc[qs * C + ch] += pr.wgt * g;
                         Gc[qr * C + ch] -= pr.wgt * g;
-                        dLdw += g * (Cc[qs * C + ch] - Cc[qr * C + ch]);
+                        dLdw += (double)g * (double)(Cc[qs * C + ch] - Cc[qr * C + ch]);
                     }
                     if (pr.clamped) continue;
+                    /* The decisions above are the forward's (in `real`); the
+                     * derivative of the crossing t = fa / (fa - fb) is taken in
+                     * double from the clip coordinates in both variants: it is
+                     * ill-conditioned for edges nearly parallel to the pair. */
                     /* w = 1/2 - t when the receiver is `from`, t - 1/2 otherwise */
                     int recv_is_from = (pr.recv == 0) == (pr.from_a != 0);
-                    real dLdt = recv_is_from ? -dLdw : dLdw;
-                    real den = pr.fa - pr.fb;
-                    real dt_dfa = -pr.fb / (den * den), dt_dfb = pr.fa / (den * den);
+                    double dLdt = recv_is_from ? -dLdw : dLdw;
                     const int32_t *t3 = tri + 3 * (int64_t)pr.tri;
-                    real X[3], Y[3], cw[3][4];
-                    SFX(proj_tri)(P, V, t3, H, W, X, Y, cw);
                     int o = SFX(orient)(P, V, t3);
-                    real so = o < 0 ? (real)-1 : (real)1;
+                    double so = o < 0 ? -1.0 : 1.0;
                     int i = pr.edge, j = (pr.edge + 1) % 3;
-                    real fxp = (real)(pr.from_a ? ca : cb) + (real)0.5, fyp = (real)(pr.from_a ? ra : rb) + (real)0.5;
-                    real txp = (real)(pr.from_a ? cb : ca) + (real)0.5, typ = (real)(pr.from_a ? rb : ra) + (real)0.5;
-                    /* d f / d(Xi, Yi, Xj, Yj) at P: so*(Yj-PY), so*(PX-Xj), so*(PY-Yi), so*(Xi-PX) */
-                    real gXi = dLdt * so * (dt_dfa * (Y[j] - fyp) + dt_dfb * (Y[j] - typ));
-                    real gYi = dLdt * so * (dt_dfa * (fxp - X[j]) + dt_dfb * (txp - X[j]));
-                    real gXj = dLdt * so * (dt_dfa * (fyp - Y[i]) + dt_dfb * (typ - Y[i]));
-                    real gYj = dLdt * so * (dt_dfa * (X[i] - fxp) + dt_dfb * (X[i] - txp));
-                    real hw = (real)W * (real)0.5, hh = (real)H * (real)0.5;
                     int vv[2] = {t3[i], t3[j]};
-                    real gx2[2] = {gXi, gXj}, gy2[2] = {gYi, gYj};
-                    int kk[2] = {i, j};
+                    double hw = (double)W * 0.5, hh = (double)H * 0.5;
+                    double cx[2], cy[2], cw[2], qx[2], qy[2];
                     for (int m = 0; m < 2; ++m) {
-                        real x = cw[kk[m]][0], y = cw[kk[m]][1], w = cw[kk[m]][3];
+                        const real *pv = P + 4 * (int64_t)vv[m];
+                        cx[m] = pv[0]; cy[m] = pv[1]; cw[m] = pv[3];
+                        qx[m] = (cx[m] / cw[m] + 1.0) * hw;
+                        qy[m] = (cy[m] / cw[m] + 1.0) * hh;
+                    }
+                    double fxp = (double)(pr.from_a ? ca : cb) + 0.5, fyp = (double)(pr.from_a ? ra : rb) + 0.5;
+                    double txp = (double)(pr.from_a ? cb : ca) + 0.5, typ = (double)(pr.from_a ? rb : ra) + 0.5;
+                    double ex = qx[1] - qx[0], ey = qy[1] - qy[0];
+                    double fa = so * (ex * (fyp - qy[0]) - ey * (fxp - qx[0]));
+                    double fb = so * (ex * (typ - qy[0]) - ey * (txp - qx[0]));
+                    double den = fa - fb;
+                    double dt_dfa = -fb / (den * den), dt_dfb = fa / (den * den);
+                    /* d f / d(Xi, Yi, Xj, Yj) at P: so*(Yj-PY), so*(PX-Xj), so*(PY-Yi), so*(Xi-PX) */
+                    double k = dLdt * so * (double)boost;
+                    double gx2[2], gy2[2];
+                    gx2[0] = k * (dt_dfa * (qy[1] - fyp) + dt_dfb * (qy[1] - typ));
+                    gy2[0] = k * (dt_dfa * (fxp - qx[1]) + dt_dfb * (txp - qx[1]));
+                    gx2[1] = k * (dt_dfa * (fyp - qy[0]) + dt_dfb * (typ - qy[0]));
+                    gy2[1] = k * (dt_dfa * (qx[0] - fxp) + dt_dfb * (qx[0] - txp));
+                    for (int m = 0; m < 2; ++m) {
                         real *g = GP + 4 * (int64_t)vv[m];
-                        g[0] += boost * gx2[m] * hw / w;
-                        g[1] += boost * gy2[m] * hh / w;
-                        g[3] += boost * (-(gx2[m] * hw * x + gy2[m] * hh * y) / (w * w));
+                        double gx = gx2[m] * hw / cw[m], gy = gy2[m] * hh / cw[m];
+                        g[0] += (real)gx;
+                        g[1] += (real)gy;
+                        g[3] += (real)(-(gx * cx[m] + gy * cy[m]) / cw[m]);
                     }
                 }
     }

@@ -237,3 +237,35 @@ def test_rasterize_backward_range_mode_uses_atomics(cuda):
     rast.backward(gr.cuda())
     o = oracle.rasterize_bwd(pos.numpy(), tri.numpy(), rast.detach().cpu().numpy(), gr.numpy())
     _close_rel(p.grad.cpu().numpy(), o, 1e-4, 'grad pos range mode')
+
+
+def test_antialias_backward_with_and_without_forward_state(cuda):
+    """The backward that reads the forward's pair decisions equals the one that searches again."""
+    from diffrp_amd import _lib
+    from diffrp_amd._lib import ptr, stream_of
+    pos, tri, _ = noisy_clip(5, 0.01, 128, views=2)
+    g = torch.Generator().manual_seed(12)
+    n, H, W, C = 2, 128, 128, 4
+    color = torch.rand(n, H, W, C, generator=g).cuda()
+    go = torch.randn(n, H, W, C, generator=g).cuda()
+    p, t = pos.cuda().contiguous(), tri.cuda().int().contiguous()
+    rast, _ = _rast(p, t, (H, W))
+    opp = nvdr.antialias_construct_topology_hash(t).opp
+    L = _lib.lib()
+    V, F = p.shape[1], t.shape[0]
+    ws_bytes = L.drp_antialias_workspace_size(n, H, W, V, F, 1)
+    out = torch.empty_like(color)
+    res = []
+    for reuse in (0, 1):
+        ws = torch.empty(ws_bytes, dtype=torch.uint8, device='cuda')
+        s = stream_of(color)
+        _lib.check(L.drp_antialias_fwd(s, ptr(color), n, H, W, C, ptr(rast), ptr(p), V, 1, ptr(t), F, ptr(opp),
+                                       ptr(out), ptr(ws), ws_bytes), 'antialias')
+        gc, gp = torch.empty_like(color), torch.zeros_like(p)
+        _lib.check(L.drp_antialias_bwd(s, ptr(color), n, H, W, C, ptr(rast), ptr(p), V, 1, ptr(t), F, ptr(opp),
+                                       ptr(go), ptr(gc), ptr(gp), 1.0, reuse, ptr(ws), ws_bytes), 'antialias bwd')
+        torch.cuda.synchronize()
+        res.append((gc.cpu(), gp.cpu()))
+    assert torch.equal(res[0][0], res[1][0])
+    assert res[0][1].abs().max() > 0
+    _close_rel(res[1][1].numpy(), res[0][1].numpy(), 1e-6, 'grad pos with forward state')
