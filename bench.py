@@ -165,9 +165,9 @@ def per_op_times(S, iters=10):
     opp = S.topo.opp
     off, inc = nvdr.incidence(S.tri, V)
     vpg = nvdr._views_per_group(n, F)
-    iws = L.drp_interpolate_bwd_workspace_size(n, F, C, 0, vpg)
+    iws = L.drp_interpolate_bwd_workspace_size(n, H, W, F, C, 0, vpg)
     iws_t = torch.empty(iws, dtype=torch.uint8, device=pos.device)
-    pws = L.drp_rasterize_bwd_workspace_size(n, F)
+    pws = L.drp_rasterize_bwd_workspace_size(n, H, W, F)
     pws_t = torch.empty(pws, dtype=torch.uint8, device=pos.device)
     s = stream_of(pos)
     fns = {

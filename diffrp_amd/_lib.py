@@ -29,7 +29,7 @@ _SIGS = {
     'drp_rasterize_fwd': (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p, c_int, c_void_p, c_int, c_int,
                                   c_void_p, c_void_p, c_void_p, c_void_p, c_size, c_int, c_int, c_int, c_ll, c_int,
                                   c_void_p]),
-    'drp_rasterize_bwd_workspace_size': (c_size, [c_int, c_int]),
+    'drp_rasterize_bwd_workspace_size': (c_size, [c_int, c_int, c_int, c_int]),
     'drp_rasterize_bwd': (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p, c_int, c_void_p, c_int, c_int,
                                   c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                   c_size]),
@@ -37,7 +37,7 @@ _SIGS = {
     'drp_incidence_build': (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_size]),
     'drp_interpolate_fwd': (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_int, c_int,
                                     c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
-    'drp_interpolate_bwd_workspace_size': (c_size, [c_int, c_int, c_int, c_int, c_int]),
+    'drp_interpolate_bwd_workspace_size': (c_size, [c_int, c_int, c_int, c_int, c_int, c_int, c_int]),
     'drp_interpolate_bwd': (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_int, c_int,
                                     c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
                                     c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_size]),

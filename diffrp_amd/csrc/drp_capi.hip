@@ -150,9 +150,9 @@ int drp_rasterize_fwd(void *stream, const float *pos, int n_img, int V, int pos_
     return 0;
 }
 
-size_t drp_rasterize_bwd_workspace_size(int n_img, int F) {
-    if (n_img <= 0 || F <= 0) return 0;
-    return align256((size_t)n_img * (size_t)F * 3 * 16);
+size_t drp_rasterize_bwd_workspace_size(int n_img, int H, int W, int F) {
+    if (n_img <= 0 || F <= 0 || H <= 0 || W <= 0) return 0;
+    return align256((size_t)n_img * (size_t)F * 3 * 16) + tile_records_bytes(n_img, H, W, 12);
 }
 
 int drp_rasterize_bwd(void *stream, const float *pos, int n_img, int V, int pos_batched, const int32_t *tri, int F,
@@ -166,13 +166,15 @@ int drp_rasterize_bwd(void *stream, const float *pos, int n_img, int V, int pos_
         !aligned16(grad_pos))
         return fail(-1, "pos/rast/grad buffers must be 16-byte aligned");
     bool tri_path = boxes && inc_off && inc && !ranges_dev;
-    if (tri_path && (!workspace || workspace_bytes < drp_rasterize_bwd_workspace_size(n_img, F)))
-        return fail(-1, "workspace too small: need %zu bytes, got %zu", drp_rasterize_bwd_workspace_size(n_img, F),
+    if (tri_path && (!workspace || workspace_bytes < drp_rasterize_bwd_workspace_size(n_img, H, W, F)))
+        return fail(-1, "workspace too small: need %zu bytes, got %zu",
+                    drp_rasterize_bwd_workspace_size(n_img, H, W, F),
                     workspace_bytes);
     RasterBwdLaunch L{};
     if (tri_path) {
         L.boxes = (const ItemBox *)boxes;
         L.partial = (float *)workspace;
+        L.records = (char *)workspace + align256((size_t)n_img * (size_t)F * 3 * 16);
         L.inc_off = inc_off;
         L.inc = inc;
     }
@@ -229,11 +231,16 @@ int drp_interpolate_fwd(void *stream, const float *attr, int n_img, int V, int a
     return 0;
 }
 
-size_t drp_interpolate_bwd_workspace_size(int n_img, int F, int C, int attr_batched, int views_per_group) {
-    if (n_img <= 0 || F <= 0 || C <= 0) return 0;
+static size_t interp_partial_bytes(int n_img, int F, int C, int attr_batched, int views_per_group) {
     int vpg = attr_batched ? 1 : views_per_block(views_per_group);
     size_t groups = ((size_t)n_img + vpg - 1) / vpg;
     return align256(groups * (size_t)F * 3 * (size_t)C * 4);
+}
+
+size_t drp_interpolate_bwd_workspace_size(int n_img, int H, int W, int F, int C, int attr_batched,
+                                          int views_per_group) {
+    if (n_img <= 0 || F <= 0 || C <= 0 || H <= 0 || W <= 0) return 0;
+    return interp_partial_bytes(n_img, F, C, attr_batched, views_per_group) + tile_records_bytes(n_img, H, W, 3 * C);
 }
 
 int drp_interpolate_bwd(void *stream, const float *attr, int n_img, int V, int attr_batched, int C,
@@ -274,12 +281,13 @@ int drp_interpolate_bwd(void *stream, const float *attr, int n_img, int V, int a
     L.views_per_group = attr_batched ? 1 : views_per_block(views_per_group);
     bool tri_path = boxes && inc_off && inc && n_diff == 0 && L.cv > 0;
     if (tri_path) {
-        size_t need = drp_interpolate_bwd_workspace_size(n_img, F, C, attr_batched, L.views_per_group);
+        size_t need = drp_interpolate_bwd_workspace_size(n_img, H, W, F, C, attr_batched, L.views_per_group);
         if (!workspace || workspace_bytes < need)
             return fail(-1, "workspace too small: need %zu bytes, got %zu", need, workspace_bytes);
         if (C % 4 == 0 && !aligned16(grad_out)) return fail(-1, "grad_out must be 16-byte aligned");
         L.boxes = (const ItemBox *)boxes;
         L.partial = (float *)workspace;
+        L.records = (char *)workspace + interp_partial_bytes(n_img, F, C, attr_batched, L.views_per_group);
         L.inc_off = inc_off;
         L.inc = inc;
     }

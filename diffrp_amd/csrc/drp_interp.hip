@@ -160,39 +160,11 @@ int launch_interpolate(const InterpLaunch &L, hipStream_t st) {
 }
 
 // ----------------------------------------------------------------------------
-// triangle-centric backward (no atomics, deterministic)
-//   k_interp_grad_rast: per pixel, grad_rast = (<g, a0-a2>, <g, a1-a2>, 0, 0)
-//   k_tri_bwd (drp_tribwd.hip): per (view group, triangle), accumulate
-//                       (u, v, 1-u-v) * g per corner over the triangle's pixels
-//   k_gather_attr:      per vertex, sum the corner partials (CSR, fixed order)
+// deterministic backward (no atomics)
+//   k_tile_reduce / k_tile_gather (drp_tribwd.hip): grad_rast per pixel and,
+//       per (view group, triangle), (u, v, 1-u-v) * g per corner
+//   k_gather_attr: per vertex, sum the corner partials (CSR, fixed order)
 // ----------------------------------------------------------------------------
-template <int CV>
-__global__ void __launch_bounds__(256) k_interp_grad_rast(const float *__restrict__ attr, int V, int attr_batched,
-                                                          const float4 *__restrict__ rast,
-                                                          const int *__restrict__ tri, int F, long long total_pix,
-                                                          long long pix_per_img, const float *__restrict__ grad_out,
-                                                          float4 *__restrict__ grad_rast) {
-    constexpr int NC = CV;
-    long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= total_pix) return;
-    InterpPix p = interp_pix(rast, tri, gid, F, V);
-    if (!p.hit) {
-        grad_rast[gid] = make_float4(0.f, 0.f, 0.f, 0.f);
-        return;
-    }
-    const float *A = attr + (attr_batched ? (gid / pix_per_img) * (long long)V * NC : 0);
-    const float *a0 = A + (long long)p.i0 * NC, *a1 = A + (long long)p.i1 * NC, *a2 = A + (long long)p.i2 * NC;
-    const float *g = grad_out + gid * NC;
-    float su = 0.f, sv = 0.f;
-#pragma unroll
-    for (int ch = 0; ch < NC; ++ch) {
-        float gc = g[ch], x2 = a2[ch];
-        su += gc * (a0[ch] - x2);
-        sv += gc * (a1[ch] - x2);
-    }
-    grad_rast[gid] = make_float4(su, sv, 0.f, 0.f);
-}
-
 // grad_attr[out][v] += sum over groups g in the output's range and incident corners
 template <int NC>
 __global__ void __launch_bounds__(256) k_gather_attr(const float *__restrict__ partial, int n_groups, int F, int V,
@@ -221,16 +193,17 @@ __global__ void __launch_bounds__(256) k_gather_attr(const float *__restrict__ p
 
 template <int NC>
 static void launch_interp_bwd_tri_t(const InterpLaunch &L, hipStream_t st) {
-    long long total = L.total_pix;
-    hipLaunchKernelGGL(k_interp_grad_rast<NC>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, L.attr, L.V,
-                       L.attr_batched, (const float4 *)L.rast, L.tri, L.F, total, L.pix_per_img, L.grad_out,
-                       (float4 *)L.grad_rast);
-    int n_img = (int)(total / L.pix_per_img);
+    int n_img = (int)(L.total_pix / L.pix_per_img);
     int vpg = L.attr_batched ? 1 : L.views_per_group;
     int groups = (n_img + vpg - 1) / vpg;
     TriBwdLaunch T{};
+    T.tri = L.tri;
     T.rast = L.rast;
     T.grad_out = L.grad_out;
+    T.attr = L.attr;
+    T.V = L.V;
+    T.attr_batched = L.attr_batched;
+    T.grad_rast_out = L.grad_rast;
     T.boxes = L.boxes;
     T.partial = L.partial;
     T.n_img = n_img;
@@ -238,6 +211,7 @@ static void launch_interp_bwd_tri_t(const InterpLaunch &L, hipStream_t st) {
     T.H = L.H;
     T.W = L.W;
     T.views_per_block = vpg;
+    tile_records_layout(T, L.records, n_img, L.H, L.W, 3 * NC);
     launch_tri_bwd_attr(T, NC, st);
     int outs = L.attr_batched ? n_img : 1;
     hipLaunchKernelGGL(k_gather_attr<NC>, dim3((L.V + 255) / 256, outs), dim3(256), 0, st, L.partial, groups, L.F,
@@ -246,7 +220,7 @@ static void launch_interp_bwd_tri_t(const InterpLaunch &L, hipStream_t st) {
 
 int launch_interpolate_bwd(const InterpLaunch &L, hipStream_t st) {
     if (L.total_pix == 0) return 0;
-    if (L.boxes && L.partial && L.inc_off && L.inc && L.n_diff == 0 && L.cv > 0) {
+    if (L.boxes && L.partial && L.records && L.inc_off && L.inc && L.n_diff == 0 && L.cv > 0) {
         switch (L.cv) {
             case 1: launch_interp_bwd_tri_t<1>(L, st); break;
             case 2: launch_interp_bwd_tri_t<2>(L, st); break;

@@ -41,6 +41,7 @@ struct RasterBwdLaunch {
     // triangle-centric path (all set) or per-pixel atomics (any null)
     const ItemBox *boxes;
     float *partial;          // [n_img, F, 3] float4
+    void *records;           // tile records (tile_records_bytes(n_img, H, W, 12))
     const int *inc_off, *inc;
 };
 
@@ -62,6 +63,7 @@ struct InterpLaunch {
     int H, W;
     const ItemBox *boxes;
     float *partial;          // [groups, F, 3, C]
+    void *records;           // tile records (tile_records_bytes(n_img, H, W, 3 C))
     const int *inc_off, *inc;
     int views_per_group;
 };
@@ -98,14 +100,24 @@ struct TriBwdLaunch {
     const float *pos;
     const int *tri;
     const float *rast, *grad_rast, *grad_db, *grad_out;
+    const float *attr;       // attribute mode: attributes (for grad_rast)
+    int V, attr_batched;
+    float *grad_rast_out;    // attribute mode: grad_rast written by the tile pass
     const ItemBox *boxes;
     float *partial;
     int n_img, F, H, W;
-    int views_per_block;   // 1, 2, 4 or 8 (attribute mode); 1 for positions
+    int views_per_block;     // images summed per partial (attribute mode); 1 for positions
+    // per-tile records (16 x 16 tiles): count, triangle, sums
+    int *rec_count, *rec_t;
+    float *rec_v;
+    int tiles_x, tiles_per_img, n_tiles, n_tiles_pad;
 };
 
 int launch_tri_bwd_pos(const TriBwdLaunch &L, hipStream_t st);
 int launch_tri_bwd_attr(const TriBwdLaunch &L, int C, hipStream_t st);
+// workspace of the tile records for nv floats per record, and its layout
+size_t tile_records_bytes(int n_img, int H, int W, int nv);
+void tile_records_layout(TriBwdLaunch &T, void *ws, int n_img, int H, int W, int nv);
 
 struct EdgeLaunch {
     const float *grad_out, *rgb, *rast, *frag_xy, *tris_xy;

@@ -410,7 +410,7 @@ int launch_rasterize(const RasterLaunch &L, hipStream_t st) {
 
 int launch_rasterize_bwd(const RasterBwdLaunch &L, hipStream_t st) {
     if (L.total_pix == 0) return 0;
-    if (L.boxes && L.partial && L.inc_off && L.inc) {
+    if (L.boxes && L.partial && L.records && L.inc_off && L.inc) {
         TriBwdLaunch T{};
         T.b = L.b;
         T.pos = L.pos;
@@ -425,6 +425,7 @@ int launch_rasterize_bwd(const RasterBwdLaunch &L, hipStream_t st) {
         T.H = L.b.H;
         T.W = L.b.W;
         T.views_per_block = 1;
+        tile_records_layout(T, L.records, L.b.n_img, L.b.H, L.b.W, 12);
         if (int rc = launch_tri_bwd_pos(T, st)) return rc;
         int n_out = L.b.pos_batched ? L.b.n_img : 1;
         dim3 gg((L.b.V + 255) / 256, n_out);

@@ -1,45 +1,36 @@
-// Triangle-centric backward passes, lane-balanced (gfx950).
+// Deterministic backward passes of rasterize (positions) and interpolate
+// (attributes), pixel-centric in two kernels (gfx950).
 //
-// A block owns 256 items, item = (view, triangle): TB triangles x VB views.
-// Items whose triangle won no pixel in the forward (hit flag clear) are
-// skipped.  The pixel boxes of the others are expanded into candidates, one
-// lane per candidate (block prefix sum + search), so lanes stay busy however
-// uneven the boxes are.  Each round of 256 candidates:
-//   A  a candidate whose pixel shows the item's triangle writes its
-//      contribution to LDS (component-major rows, padded: conflict-free);
-//   B  one thread per (16-candidate chunk, component) sums its chunk in
-//      order, restarting at every item head, and stores each segment sum at
-//      the segment's first slot (fixed trip count: no divergence);
-//   C  each item adds, in order, the segment sums that start inside its range.
-// The summation order depends only on the boxes, so results are reproducible
-// bit for bit (no float atomics anywhere).
+// k_tile_reduce, one workgroup per 16 x 16 pixel tile of one image: every
+// pixel computes its contribution once (coalesced loads, no atomics) and the
+// 256 (triangle, pixel) keys are sorted in LDS (bitonic); each run of equal
+// triangles -- a segment -- is summed in pixel order, per component, and
+// written as a record (triangle, sums).  The records of a tile come out
+// sorted by triangle.
 //
-//   POS  (VB = 1): d rast (u, v, z/w [, db]) -> d clip (x, y, z, w) of the 3 corners,
-//                  partial [n, F, 3] float4, reduced per vertex by k_gather_pos.
-//   ATTR (VB <= 8): (u, v, 1-u-v) * g of the 3 corners, summed over the block's
-//                  VB views, partial [groups, F, 3, C], reduced by k_gather_attr.
+// k_tile_gather, one thread per (image [group], triangle) that won a pixel
+// in the forward: the triangle's pixel box names the tiles it touches; the
+// thread binary-searches each tile's records for its triangle and adds them in
+// tile order.  All sums run in a fixed order, so results are reproducible bit
+// for bit.
+//
+//   POS  (NV = 12): d rast (u, v, z/w [, db]) -> d clip (x, y, z, w) of the 3
+//        corners (double-precision chain, drp_grad.h); partial [n, F, 3] float4,
+//        reduced per vertex by k_gather_pos.
+//   ATTR (NV = 3 C): (u, v, 1-u-v) * g per corner; the same pass writes
+//        grad_rast = (<g, a0-a2>, <g, a1-a2>, 0, 0); k_tile_gather sums the
+//        views of a group, partial [groups, F, 3, C], reduced by k_gather_attr.
 #include "drp_common.h"
 #include "drp_grad.h"
 #include "drp_kernels.h"
 
 namespace drp {
 
-constexpr int kItems = 256;
-constexpr int kRound = 256;          // candidates per round (one per lane)
-constexpr int kChunk = 16;           // candidates per reduction thread
-constexpr int kRowStride = kRound + 1;
+constexpr int kBT = 16;            // backward tile edge
+constexpr int kBTP = kBT * kBT;    // pixels (and records) per tile
+constexpr int kRowStride = kBTP + 1;
 
-__device__ __forceinline__ int search_start(const int *cstart, int i) {
-    int lo = 0, hi = kItems - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (cstart[mid] <= i) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
-
-__device__ __forceinline__ int block_scan256(int v, int *wsum, int &total) {
+__device__ __forceinline__ int block_excl_scan256(int v, int *wsum, int &total) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int x = v;
 #pragma unroll
@@ -60,86 +51,55 @@ __device__ __forceinline__ int block_scan256(int v, int *wsum, int &total) {
     return pre + x - v;
 }
 
-// floor(a / b) for 0 <= a < b * 65536, 1 <= b <= 65536: float estimate, one fix-up step
-// (the estimate's error is below 1/64, so it is off by at most one).
-__device__ __forceinline__ int div_box(int a, int b) {
-    int q = (int)((float)a * __builtin_amdgcn_rcpf((float)b));
-    int r = a - q * b;
-    if (r < 0) --q;
-    else if (r >= b) ++q;
-    return q;
-}
-
 // MODE 0: POS (NV = 12), MODE 1: ATTR (NV = 3 C)
 template <int MODE, int C, bool DB>
-__global__ void __launch_bounds__(256) k_tri_bwd(TriBwdLaunch L) {
+__global__ void __launch_bounds__(256) k_tile_reduce(TriBwdLaunch L) {
     constexpr int NV = MODE == 0 ? 12 : 3 * C;
-    __shared__ int cstart[kItems + 1];
-    __shared__ int wsum[4];
-    __shared__ int2 ibox[kItems];
-    __shared__ unsigned char head[kRound];
-    __shared__ TriVerts iverts[MODE == 0 ? kItems : 1];
+    __shared__ unsigned long long key[kBTP];
     __shared__ float contrib[NV * kRowStride];
-    const int VB = L.views_per_block, TB = kItems / VB;
+    __shared__ int seg_start[kBTP + 1];
+    __shared__ int wsum[4];
     const int tid = threadIdx.x;
-    const int tblk = blockIdx.x, vgrp = blockIdx.y;
-    // this thread's item
-    int it_t = tblk * TB + tid % TB, it_v = vgrp * VB + tid / TB;
-    bool valid = it_t < L.F && it_v < L.n_img;
-    int c0 = 1, c1 = 0, r0 = 0, r1 = -1;
-    if (valid) {
-        ItemBox b = L.boxes[(long long)it_v * L.F + it_t];
-        if (b.hit) {
-            c0 = b.cc & 0xffff; c1 = (unsigned)b.cc >> 16; r0 = b.rr & 0xffff; r1 = (unsigned)b.rr >> 16;
-        }
-    }
-    int count = (c0 <= c1) ? (c1 - c0 + 1) * (r1 - r0 + 1) : 0;
-    if (MODE == 0 && count > 0) {
-        ImageDesc d = image_desc(L.b, it_v);
-        if (!load_tri(L.pos, L.tri, it_t, d, iverts[tid])) count = 0;
-    }
-    ibox[tid] = make_int2(c0 | (c1 << 16), r0 | (r1 << 16));
-    int total;
-    int ex = block_scan256(count, wsum, total);
-    cstart[tid] = ex;
-    if (tid == kItems - 1) cstart[kItems] = ex + count;
-    __syncthreads();
-    float acc[NV];
-#pragma unroll
-    for (int j = 0; j < NV; ++j) acc[j] = 0.f;
-    const int my0 = ex, my1 = ex + count;
+    const int tile = xcd_remap(blockIdx.x, L.n_tiles_pad);
+    if (tile >= L.n_tiles) return;  // uniform per block
+    const int img = tile / L.tiles_per_img, lt = tile - img * L.tiles_per_img;
+    const int r = (lt / L.tiles_x) * kBT + (tid >> 4), c = (lt % L.tiles_x) * kBT + (tid & 15);
     const long long npx = (long long)L.H * L.W;
-    for (int base = 0; base < total; base += kRound) {
-        // ---- A: contributions of this round's candidates
-        int i = base + tid;
-        float val[NV];
+    float val[NV];
 #pragma unroll
-        for (int j = 0; j < NV; ++j) val[j] = 0.f;
-        bool is_head = true;
-        if (i < total) {
-            int k = search_start(cstart, i);
-            int2 b = ibox[k];
-            int kc0 = b.x & 0xffff, kc1 = (unsigned)b.x >> 16, kr0 = b.y & 0xffff;
-            int bw = kc1 - kc0 + 1, local = i - cstart[k];
-            is_head = local == 0;
-            int dr = div_box(local, bw);
-            int r = kr0 + dr, c = kc0 + (local - dr * bw);
-            int t = tblk * TB + k % TB, view = vgrp * VB + k / TB;
-            long long pix = (long long)view * npx + (long long)r * L.W + c;
-            float4 rv = reinterpret_cast<const float4 *>(L.rast)[pix];
-            if (rv.w == triidx_to_float(t + 1)) {
-                if (MODE == 0) {
+    for (int j = 0; j < NV; ++j) val[j] = 0.f;
+    int t = -1;
+    if (r < L.H && c < L.W) {
+        const long long pix = (long long)img * npx + (long long)r * L.W + c;
+        float4 rv = reinterpret_cast<const float4 *>(L.rast)[pix];
+        if (rv.w > 0.f) {
+            int id = float_to_triidx(rv.w);
+            if (id >= 1 && id <= L.F) t = id - 1;
+        }
+        if (MODE == 0) {
+            if (t >= 0) {
+                ImageDesc d = image_desc(L.b, img);
+                TriVerts tv;
+                if (load_tri(L.pos, L.tri, t, d, tv)) {
                     float4 g = reinterpret_cast<const float4 *>(L.grad_rast)[pix];
                     float4 gd = make_float4(0.f, 0.f, 0.f, 0.f);
                     if (DB) gd = db_grad_scaled(reinterpret_cast<const float4 *>(L.grad_db)[pix], L.H, L.W);
                     float4 o[3];
-                    raster_pixel_grad<DB>(iverts[k], c, r, L.H, L.W, g, gd, o);
+                    raster_pixel_grad<DB>(tv, c, r, L.H, L.W, g, gd, o);
 #pragma unroll
                     for (int q = 0; q < 3; ++q) {
                         val[4 * q] = o[q].x; val[4 * q + 1] = o[q].y; val[4 * q + 2] = o[q].z; val[4 * q + 3] = o[q].w;
                     }
                 } else {
-                    float u = rv.x, v = rv.y, w2 = (1.f - u) - v;
+                    t = -1;
+                }
+            }
+        } else {
+            float4 gr = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (t >= 0) {
+                const int *t3 = L.tri + 3 * (long long)t;
+                int i0 = t3[0], i1 = t3[1], i2 = t3[2];
+                if ((unsigned)i0 < (unsigned)L.V && (unsigned)i1 < (unsigned)L.V && (unsigned)i2 < (unsigned)L.V) {
                     float gv[C];
                     if (C % 4 == 0) {
                         const float4 *g4 = reinterpret_cast<const float4 *>(L.grad_out + pix * C);
@@ -152,100 +112,175 @@ __global__ void __launch_bounds__(256) k_tri_bwd(TriBwdLaunch L) {
 #pragma unroll
                         for (int ch = 0; ch < C; ++ch) gv[ch] = L.grad_out[pix * C + ch];
                     }
+                    const float *A = L.attr + (L.attr_batched ? (long long)img * L.V * C : 0);
+                    const float *a0 = A + (long long)i0 * C, *a1 = A + (long long)i1 * C, *a2 = A + (long long)i2 * C;
+                    float u = rv.x, v = rv.y, w2 = (1.f - u) - v, su = 0.f, sv = 0.f;
 #pragma unroll
                     for (int ch = 0; ch < C; ++ch) {
-                        val[ch] = u * gv[ch];
-                        val[C + ch] = v * gv[ch];
-                        val[2 * C + ch] = w2 * gv[ch];
+                        float gc = gv[ch], x2 = a2[ch];
+                        su += gc * (a0[ch] - x2);
+                        sv += gc * (a1[ch] - x2);
+                        val[ch] = u * gc;
+                        val[C + ch] = v * gc;
+                        val[2 * C + ch] = w2 * gc;
                     }
-                }
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < NV; ++j) contrib[j * kRowStride + tid] = val[j];
-        head[tid] = is_head;
-        __syncthreads();
-        // ---- B: segment sums per (chunk, component), stored at each segment's first slot
-        for (int w = tid; w < (kRound / kChunk) * NV; w += 256) {
-            int j = w % NV, ch = w / NV;  // neighbouring lanes: neighbouring rows (distinct banks)
-            float *row = contrib + j * kRowStride;
-            int p0 = ch * kChunk, seg = p0;
-            float s = row[p0];
-#pragma unroll
-            for (int p = p0 + 1; p < p0 + kChunk; ++p) {
-                float x = row[p];
-                if (head[p]) {
-                    row[seg] = s;
-                    seg = p;
-                    s = x;
+                    gr = make_float4(su, sv, 0.f, 0.f);
                 } else {
-                    s += x;
+                    t = -1;
                 }
             }
-            row[seg] = s;
+            reinterpret_cast<float4 *>(L.grad_rast_out)[pix] = gr;
         }
-        __syncthreads();
-        // ---- C: the item adds the segments that start in its range, in order
-        int lo = max(my0, base), hi = min(my1, base + kRound);
-        if (lo < hi) {
-            int p = lo - base, pe = hi - base;
+    }
 #pragma unroll
-            for (int j = 0; j < NV; ++j) acc[j] += contrib[j * kRowStride + p];
-            for (int q = (p / kChunk + 1) * kChunk; q < pe; q += kChunk) {
+    for (int j = 0; j < NV; ++j) contrib[j * kRowStride + tid] = val[j];
+    key[tid] = t >= 0 ? (((unsigned long long)t << 8) | (unsigned)tid) : ~0ull;
+    __syncthreads();
+    // bitonic sort of the 256 keys, ascending (empty pixels last)
+    for (int k = 2; k <= kBTP; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            int ixj = tid ^ j;
+            if (ixj > tid) {
+                unsigned long long a = key[tid], b = key[ixj];
+                bool up = (tid & k) == 0;
+                if ((a > b) == up) {
+                    key[tid] = b;
+                    key[ixj] = a;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    // segments: runs of one triangle in sorted order
+    unsigned long long sk = key[tid];
+    bool valid = sk != ~0ull;
+    bool head = valid && (tid == 0 || (key[tid - 1] >> 8) != (sk >> 8));
+    int nseg;
+    int seg = block_excl_scan256(head ? 1 : 0, wsum, nseg);
+    int nvalid = __syncthreads_count(valid);
+    if (head) seg_start[seg] = tid;
+    if (tid == 0) seg_start[nseg] = nvalid;
+    __syncthreads();
+    const long long rbase = (long long)tile * kBTP;
+    if (head) L.rec_t[rbase + seg] = (int)(sk >> 8);
+    if (tid == 0) L.rec_count[tile] = nseg;
+    // one thread per (segment, component): consecutive threads -> consecutive record floats
+    for (int w = tid; w < nseg * NV; w += kBTP) {
+        int sg = w / NV, j = w - sg * NV;
+        int p0 = seg_start[sg], p1 = seg_start[sg + 1];
+        const float *row = contrib + j * kRowStride;
+        float s = row[(int)(key[p0] & 255)];
+        for (int p = p0 + 1; p < p1; ++p) s += row[(int)(key[p] & 255)];
+        L.rec_v[(rbase + sg) * NV + j] = s;
+    }
+}
+
+// add the records of triangle t in the tiles of its box, in tile order
+template <int NV>
+__device__ __forceinline__ void gather_records(const TriBwdLaunch &L, int img, int t, int cc, int rr, float *acc) {
+    int c0 = cc & 0xffff, c1 = (unsigned)cc >> 16, r0 = rr & 0xffff, r1 = (unsigned)rr >> 16;
+    if (c0 > c1 || r0 > r1) return;
+    for (int ty = r0 / kBT; ty <= r1 / kBT; ++ty)
+        for (int tx = c0 / kBT; tx <= c1 / kBT; ++tx) {
+            int tile = img * L.tiles_per_img + ty * L.tiles_x + tx;
+            long long rbase = (long long)tile * kBTP;
+            int lo = 0, hi = L.rec_count[tile] - 1;
+            while (lo <= hi) {  // records sorted by triangle
+                int mid = (lo + hi) >> 1;
+                int tm = L.rec_t[rbase + mid];
+                if (tm < t) lo = mid + 1;
+                else if (tm > t) hi = mid - 1;
+                else {
+                    const float *q = L.rec_v + (rbase + mid) * NV;
 #pragma unroll
-                for (int j = 0; j < NV; ++j) acc[j] += contrib[j * kRowStride + q];
+                    for (int j = 0; j < NV; ++j) acc[j] += q[j];
+                    break;
+                }
             }
         }
-        __syncthreads();
+}
+
+template <int MODE, int C>
+__global__ void __launch_bounds__(256) k_tile_gather(TriBwdLaunch L) {
+    constexpr int NV = MODE == 0 ? 12 : 3 * C;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int grp = blockIdx.y;
+    if (t >= L.F) return;
+    float acc[NV];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) acc[j] = 0.f;
+    const int VB = L.views_per_block;
+    const int v0 = grp * VB, v1 = min(v0 + VB, L.n_img);
+    for (int img = v0; img < v1; ++img) {
+        ItemBox b = L.boxes[(long long)img * L.F + t];
+        if (b.hit) gather_records<NV>(L, img, t, b.cc, b.rr, acc);
     }
     if (MODE == 0) {
-        if (!valid) return;
-        float4 *P = reinterpret_cast<float4 *>(L.partial) + ((long long)it_v * L.F + it_t) * 3;
+        float4 *P = reinterpret_cast<float4 *>(L.partial) + ((long long)grp * L.F + t) * 3;
         P[0] = make_float4(acc[0], acc[1], acc[2], acc[3]);
         P[1] = make_float4(acc[4], acc[5], acc[6], acc[7]);
         P[2] = make_float4(acc[8], acc[9], acc[10], acc[11]);
     } else {
-        // combine the VB views of each triangle in view order
-        float *stage = contrib;  // kItems * NV floats fit in NV * kRowStride
+        float *P = L.partial + ((long long)grp * L.F + t) * NV;
 #pragma unroll
-        for (int j = 0; j < NV; ++j) stage[tid * NV + j] = acc[j];
-        __syncthreads();
-        if (tid < TB && tblk * TB + tid < L.F) {
-            float s[NV];
-#pragma unroll
-            for (int j = 0; j < NV; ++j) s[j] = 0.f;
-            for (int vi = 0; vi < VB; ++vi) {
-                const float *q = stage + (vi * TB + tid) * NV;
-#pragma unroll
-                for (int j = 0; j < NV; ++j) s[j] += q[j];
-            }
-            float *P = L.partial + ((long long)vgrp * L.F + tblk * TB + tid) * NV;
-#pragma unroll
-            for (int j = 0; j < NV; ++j) P[j] = s[j];
-        }
+        for (int j = 0; j < NV; ++j) P[j] = acc[j];
     }
+}
+
+static void tile_grid(const TriBwdLaunch &L, dim3 &grid) {
+    grid = dim3((unsigned)L.n_tiles_pad);
 }
 
 int launch_tri_bwd_pos(const TriBwdLaunch &L, hipStream_t st) {
-    dim3 grid((L.F + kItems - 1) / kItems, L.n_img);
-    if (L.grad_db) hipLaunchKernelGGL((k_tri_bwd<0, 1, true>), grid, dim3(256), 0, st, L);
-    else hipLaunchKernelGGL((k_tri_bwd<0, 1, false>), grid, dim3(256), 0, st, L);
+    dim3 grid;
+    tile_grid(L, grid);
+    if (L.grad_db) hipLaunchKernelGGL((k_tile_reduce<0, 1, true>), grid, dim3(kBTP), 0, st, L);
+    else hipLaunchKernelGGL((k_tile_reduce<0, 1, false>), grid, dim3(kBTP), 0, st, L);
+    hipLaunchKernelGGL((k_tile_gather<0, 1>), dim3((L.F + 255) / 256, L.n_img), dim3(256), 0, st, L);
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
+template <int C>
+static void launch_attr_c(const TriBwdLaunch &L, hipStream_t st) {
+    dim3 grid;
+    tile_grid(L, grid);
+    hipLaunchKernelGGL((k_tile_reduce<1, C, false>), grid, dim3(kBTP), 0, st, L);
+    int groups = (L.n_img + L.views_per_block - 1) / L.views_per_block;
+    hipLaunchKernelGGL((k_tile_gather<1, C>), dim3((L.F + 255) / 256, groups), dim3(256), 0, st, L);
+}
+
 int launch_tri_bwd_attr(const TriBwdLaunch &L, int C, hipStream_t st) {
-    int VB = L.views_per_block, TB = kItems / VB;
-    dim3 grid((L.F + TB - 1) / TB, (L.n_img + VB - 1) / VB);
     switch (C) {
-        case 1: hipLaunchKernelGGL((k_tri_bwd<1, 1, false>), grid, dim3(256), 0, st, L); break;
-        case 2: hipLaunchKernelGGL((k_tri_bwd<1, 2, false>), grid, dim3(256), 0, st, L); break;
-        case 3: hipLaunchKernelGGL((k_tri_bwd<1, 3, false>), grid, dim3(256), 0, st, L); break;
-        case 4: hipLaunchKernelGGL((k_tri_bwd<1, 4, false>), grid, dim3(256), 0, st, L); break;
-        case 8: hipLaunchKernelGGL((k_tri_bwd<1, 8, false>), grid, dim3(256), 0, st, L); break;
-        case 16: hipLaunchKernelGGL((k_tri_bwd<1, 16, false>), grid, dim3(256), 0, st, L); break;
+        case 1: launch_attr_c<1>(L, st); break;
+        case 2: launch_attr_c<2>(L, st); break;
+        case 3: launch_attr_c<3>(L, st); break;
+        case 4: launch_attr_c<4>(L, st); break;
+        case 8: launch_attr_c<8>(L, st); break;
+        case 16: launch_attr_c<16>(L, st); break;
         default: return -1;
     }
     return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+size_t tile_records_bytes(int n_img, int H, int W, int nv) {
+    long long tiles = (long long)n_img * ((W + kBT - 1) / kBT) * ((H + kBT - 1) / kBT);
+    auto a256 = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    return a256((size_t)tiles * 4) + a256((size_t)tiles * kBTP * 4) + a256((size_t)tiles * kBTP * nv * 4);
+}
+
+void tile_records_layout(TriBwdLaunch &T, void *ws, int n_img, int H, int W, int nv) {
+    T.tiles_x = (W + kBT - 1) / kBT;
+    T.tiles_per_img = T.tiles_x * ((H + kBT - 1) / kBT);
+    T.n_tiles = n_img * T.tiles_per_img;
+    T.n_tiles_pad = (T.n_tiles + 7) & ~7;
+    auto a256 = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    char *b = (char *)ws;
+    T.rec_count = (int *)b;
+    b += a256((size_t)T.n_tiles * 4);
+    T.rec_t = (int *)b;
+    b += a256((size_t)T.n_tiles * kBTP * 4);
+    T.rec_v = (float *)b;
+    (void)nv;
 }
 
 }  // namespace drp

@@ -238,7 +238,7 @@ class _RasterizeFunc(torch.autograd.Function):
         ws_bytes = 0
         if boxes is not None:
             off, inc = incidence(tri, V)
-            ws_bytes = L.drp_rasterize_bwd_workspace_size(n, F)
+            ws_bytes = L.drp_rasterize_bwd_workspace_size(n, H, W, F)
             ws = torch.empty(ws_bytes, dtype=torch.uint8, device=pos.device)
         rc = L.drp_rasterize_bwd(stream_of(pos), ptr(pos), n, V, int(pos.dim() == 3), ptr(tri), F,
                                  ptr(rng.dev) if rng is not None else None, H, W, ptr(rast), ptr(dy), ptr(gdb),
@@ -382,7 +382,7 @@ class _InterpolateFunc(torch.autograd.Function):
         if boxes is not None and nd == 0 and C in _TRI_BWD_C:
             off, inc = incidence(tri, V)
             vpg = _views_per_group(n, F)
-            ws_bytes = L.drp_interpolate_bwd_workspace_size(n, F, C, batched, vpg)
+            ws_bytes = L.drp_interpolate_bwd_workspace_size(n, H, W, F, C, batched, vpg)
             ws = torch.empty(ws_bytes, dtype=torch.uint8, device=attr.device)
         else:
             boxes = None

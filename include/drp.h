@@ -68,9 +68,10 @@ int drp_rasterize_fwd(void *stream, const float *pos, int n_img, int V, int pos_
  * into grad_pos (same layout as pos; caller zero-fills).  grad_db may be null.
  * With boxes (from the forward), the incidence table of tri and a workspace
  * of drp_rasterize_bwd_workspace_size bytes, and no ranges, the gradient is
- * gathered per triangle and per vertex without atomics (deterministic);
- * otherwise per-pixel float atomics are used. */
-size_t drp_rasterize_bwd_workspace_size(int n_img, int F);
+ * summed per 16x16 tile and triangle, per triangle and per vertex in fixed
+ * orders without atomics (deterministic); otherwise per-pixel float atomics
+ * are used.  The per-pixel chain is evaluated in double precision. */
+size_t drp_rasterize_bwd_workspace_size(int n_img, int H, int W, int F);
 int drp_rasterize_bwd(void *stream, const float *pos, int n_img, int V, int pos_batched, const int32_t *tri, int F,
                       const int32_t *ranges_dev, int H, int W, const float *rast, const float *grad_rast,
                       const float *grad_db, float *grad_pos, const int32_t *boxes, const int32_t *inc_off,
@@ -98,7 +99,8 @@ int drp_interpolate_fwd(void *stream, const float *attr, int n_img, int V, int a
  * diff_attrs, C in {1,2,3,4,8,16} and a workspace of
  * drp_interpolate_bwd_workspace_size bytes, grad_attr is gathered per triangle
  * (views_per_group views per partial) and per vertex without atomics. */
-size_t drp_interpolate_bwd_workspace_size(int n_img, int F, int C, int attr_batched, int views_per_group);
+size_t drp_interpolate_bwd_workspace_size(int n_img, int H, int W, int F, int C, int attr_batched,
+                                          int views_per_group);
 int drp_interpolate_bwd(void *stream, const float *attr, int n_img, int V, int attr_batched, int C,
                         const float *rast, int H, int W, const int32_t *tri, int F, const float *rast_db,
                         const int32_t *diff_idx, int n_diff, const float *grad_out, const float *grad_da,

@@ -56,10 +56,13 @@ def test_workspace_queries(L):
     assert ws >= tiles * 512 * 4 + 64 * 81920 * 12
     assert ws % 256 == 0
     assert L.drp_rasterize_workspace_size(0, 512, 512, 0, 16, 8, 512) == 0
-    assert L.drp_rasterize_bwd_workspace_size(64, 81920) >= 64 * 81920 * 48
-    assert L.drp_interpolate_bwd_workspace_size(64, 81920, 8, 0, 8) >= 8 * 81920 * 3 * 8 * 4
-    assert L.drp_interpolate_bwd_workspace_size(64, 81920, 8, 0, 8) < L.drp_interpolate_bwd_workspace_size(
-        64, 81920, 8, 0, 1)
+    # partials plus one record slot per pixel (triangle + sums)
+    px = 64 * 512 * 512
+    assert L.drp_rasterize_bwd_workspace_size(64, 512, 512, 81920) >= 64 * 81920 * 48 + px * (4 + 48)
+    assert L.drp_interpolate_bwd_workspace_size(64, 512, 512, 81920, 8, 0, 8) >= 8 * 81920 * 96 + px * (4 + 96)
+    assert L.drp_interpolate_bwd_workspace_size(64, 512, 512, 81920, 8, 0, 8) < L.drp_interpolate_bwd_workspace_size(
+        64, 512, 512, 81920, 8, 0, 1)
+    assert L.drp_rasterize_bwd_workspace_size(64, 0, 512, 81920) == 0
     assert L.drp_topology_workspace_size(81920) > 0
     assert L.drp_incidence_workspace_size(40962) > 0
     assert L.drp_antialias_workspace_size(1, 64, 64, 3, 100, 50) > 0
