@@ -267,6 +267,16 @@ __device__ __forceinline__ float4 bary_db(const TriVerts &v, const PixEval &p, f
 
 // XCD-aware block remap: consecutive logical blocks land on the same XCD
 // (gfx950 dispatches workgroups round-robin over its 8 XCDs).
+// floor(a / b) for 0 <= a < b * 65536, 1 <= b <= 65536: float estimate, one fix-up step
+// (the estimate's error is below 1/64, so it is off by at most one).
+__device__ __forceinline__ int div_box(int a, int b) {
+    int q = (int)((float)a * __builtin_amdgcn_rcpf((float)b));
+    int r = a - q * b;
+    if (r < 0) --q;
+    else if (r >= b) ++q;
+    return q;
+}
+
 __device__ __forceinline__ int xcd_remap(int bid, int nblocks_padded) {
     int per = nblocks_padded >> 3;
     return (bid & 7) * per + (bid >> 3);

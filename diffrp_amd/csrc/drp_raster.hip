@@ -5,15 +5,16 @@
 // screen-tile bins (full bins and large triangles spill, with their box, to a
 // per-image list); (2) k_fine, one workgroup per tile: triangle records staged
 // in LDS, then fragment-parallel coverage -- lanes take (record, pixel of its
-// box inside the tile) candidates from a block prefix sum -- with the depth
-// test resolved by 64-bit LDS atomicMin on (orderable(z/w) << 32 | triangle);
-// (u, v) are recomputed for the winner only.
+// box inside the tile) candidates from a block prefix sum, a chunk of
+// consecutive candidates per lane -- with the depth test resolved by 64-bit
+// LDS atomicMin on (orderable(z/w) << 32 | triangle); (u, v) are recomputed for
+// the winner only.
 //
 // The setup pass also records each (image, triangle)'s pixel box, and k_fine
-// flags the triangles that won a pixel; the triangle-centric backward
-// (drp_tribwd.hip) walks the flagged boxes and writes per-corner partials that
-// a CSR vertex gather sums (deterministic, no atomics).  A per-pixel atomic
-// kernel handles inputs that come without pixel boxes.
+// flags the triangles that won a pixel; the deterministic backward
+// (drp_tribwd.hip) uses both to find each triangle's per-tile sums, and a CSR
+// vertex gather (k_gather_pos) sums the per-corner partials.  A per-pixel
+// atomic kernel handles inputs that come without pixel boxes.
 #include "drp_common.h"
 #include "drp_grad.h"
 #include "drp_kernels.h"
@@ -21,6 +22,7 @@
 namespace drp {
 
 constexpr int kMaxBinTiles = 16;
+constexpr int kChunk = 4;  // fine raster: consecutive candidates per lane
 
 __device__ __forceinline__ int2 pack_box(int c0, int c1, int r0, int r1) {
     return make_int2(c0 | (c1 << 16), r0 | (r1 << 16));
@@ -159,6 +161,8 @@ __global__ void __launch_bounds__(TW *TH) k_fine(Batch b, const float *__restric
     static_assert(NT % 64 == 0, "tile must be a whole number of waves");
     __shared__ FineRec rec[NT];
     __shared__ int cstart[NT + 1];
+    __shared__ int queue[NT * kChunk];  // covered candidates: record | x << 8 | y << 16 (tile-local)
+    __shared__ int qcount[2];
     __shared__ unsigned long long zbuf[NT];
     __shared__ unsigned long long pkey[PEEL ? NT : 1];
     __shared__ int wsum[NT / 64];
@@ -174,6 +178,7 @@ __global__ void __launch_bounds__(TW *TH) k_fine(Batch b, const float *__restric
     bool inimg = c < d.W && r < d.H;
     long long pix = d.pix_off + (long long)r * d.W + c;
     zbuf[tid] = ~0ull;
+    if (tid < 2) qcount[tid] = 0;
     if (PEEL) {
         unsigned long long pk = ~0ull;  // empty previous layer: nothing can follow
         if (inimg) {
@@ -191,6 +196,7 @@ __global__ void __launch_bounds__(TW *TH) k_fine(Batch b, const float *__restric
     int nlist = full_scan ? (d.f1 - d.f0) : lcnt;
     int total = nbin + nlist;
 
+    int par = 0;  // queue counter in use; alternates every sub-round, reset one sub-round ahead
     for (int base = 0; base < total; base += NT) {
         int idx = base + tid;
         int count = 0;
@@ -246,41 +252,108 @@ __global__ void __launch_bounds__(TW *TH) k_fine(Batch b, const float *__restric
         cstart[tid] = ex;
         if (tid == NT - 1) cstart[NT] = ex + count;
         __syncthreads();
-        for (int i = tid; i < ntot; i += NT) {
-            // record k: the last slot whose start is <= i (empty slots share starts)
-            int lo = 0, hi = NT - 1;
-            while (lo < hi) {
-                int mid = (lo + hi + 1) >> 1;
-                if (cstart[mid] <= i) lo = mid;
-                else hi = mid - 1;
+        // Candidates in sub-rounds of NT * kChunk.  Test: each lane takes
+        // kChunk consecutive candidates (one search and one division per
+        // chunk, the record's fields in registers, E += 16 A along a row) and
+        // queues the covered ones (wave-aggregated append).  Evaluate: the
+        // queued pixels are spread over all lanes, so the barycentric and depth
+        // work runs without divergence.  The depth test is a minimum, so the
+        // queue order does not matter.
+        for (int sb = 0; sb < ntot; sb += NT * kChunk, par ^= 1) {
+            int ents[kChunk];
+            int nent = 0;
+            const int i0 = sb + tid * kChunk;
+            if (i0 < ntot) {
+                // record k: the last slot whose start is <= i0 (empty slots share starts)
+                int k = 0, hi = NT - 1;
+                while (k < hi) {
+                    int mid = (k + hi + 1) >> 1;
+                    if (cstart[mid] <= i0) k = mid;
+                    else hi = mid - 1;
+                }
+                int local = i0 - cstart[k], cnt = cstart[k + 1] - cstart[k];
+                int bw = rec[k].bw;
+                int dy = div_box(local, bw), dx = local - dy * bw;
+                int bx0 = 0, by0 = 0, cls = 0, a16[3] = {0, 0, 0};
+                long long E[3] = {0, 0, 0};
+                bool fresh = true;
+                const int iend = min(i0 + kChunk, ntot);
+#pragma unroll
+                for (int s = 0; s < kChunk; ++s) {
+                    if (i0 + s >= iend) break;
+                    while (local >= cnt) {  // next record with candidates
+                        ++k;
+                        local = 0;
+                        dx = dy = 0;
+                        cnt = cstart[k + 1] - cstart[k];
+                        bw = rec[k].bw;
+                        fresh = true;
+                    }
+                    if (fresh) {  // new record or new row: reload fields, recompute E
+                        const FineRec &R = rec[k];
+                        bx0 = R.bx0;
+                        by0 = R.by0;
+                        cls = R.cls;
+                        if (cls == kDirect) {
+                            int ox = (bx0 + dx) * kSubpix, oy = (by0 + dy) * kSubpix;
+#pragma unroll
+                            for (int e = 0; e < 3; ++e) {
+                                a16[e] = R.A[e] * kSubpix;
+                                E[e] = R.E0[e] + (long long)R.A[e] * ox + (long long)R.B[e] * oy;
+                            }
+                        }
+                        fresh = false;
+                    }
+                    if (cls != kDirect || (E[0] | E[1] | E[2]) >= 0)
+                        ents[nent++] = k | ((bx0 + dx) << 8) | ((by0 + dy) << 16);
+                    ++local;
+                    if (++dx == bw) {
+                        dx = 0;
+                        ++dy;
+                        fresh = true;
+                    } else {
+#pragma unroll
+                        for (int e = 0; e < 3; ++e) E[e] += a16[e];
+                    }
+                }
             }
-            const FineRec &R = rec[lo];
-            int local = i - cstart[lo];
-            int bw = R.bw;
-            int dy = local / bw, dx = local - dy * bw;
-            int lx = R.bx0 + dx, ly = R.by0 + dy;
-            int cls = R.cls;
-            bool cov = true;
-            if (cls == kDirect) {
-                int ox = lx * kSubpix, oy = ly * kSubpix;
-                long long E0 = R.E0[0] + (long long)R.A[0] * ox + (long long)R.B[0] * oy;
-                long long E1 = R.E0[1] + (long long)R.A[1] * ox + (long long)R.B[1] * oy;
-                long long E2 = R.E0[2] + (long long)R.A[2] * ox + (long long)R.B[2] * oy;
-                cov = (E0 | E1 | E2) >= 0;
+            // wave-aggregated append
+            {
+                const int lane = tid & 63;
+                int x = nent;
+#pragma unroll
+                for (int o = 1; o < 64; o <<= 1) {
+                    int y = __shfl_up(x, o, 64);
+                    if (lane >= o) x += y;
+                }
+                int wtot = __shfl(x, 63, 64), base = 0;
+                if (lane == 63 && wtot > 0) base = atomicAdd(&qcount[par], wtot);
+                base = __shfl(base, 63, 64) + x - nent;
+#pragma unroll
+                for (int s = 0; s < kChunk; ++s)
+                    if (s < nent) queue[base + s] = ents[s];
             }
-            if (!cov) continue;
-            TriVerts v = R.v;
-            float px = pix_ndc(tc0 + lx, d.W), py = pix_ndc(tr0 + ly, d.H);
-            PixEval p;
-            eval_pixel(v, px, py, p);
-            if (cls == kHomog && !homog_inside(p)) continue;
-            if (p.s == 0.f) continue;
-            float zw = p.zn / p.wn;
-            if (!(zw >= -1.f && zw <= 1.f)) continue;
-            unsigned long long key = frag_key(zw, R.t);
-            int q = ly * TW + lx;
-            if (PEEL && key <= pkey[q]) continue;
-            atomicMin(&zbuf[q], key);
+            __syncthreads();
+            const int nq = qcount[par];
+            if (tid == 0) qcount[par ^ 1] = 0;  // the next sub-round's counter
+            for (int qi = tid; qi < nq; qi += NT) {
+                int en = queue[qi];
+                const FineRec &R = rec[en & 255];
+                int lx = (en >> 8) & 255, ly = en >> 16;
+                TriVerts v = R.v;
+                float px = pix_ndc(tc0 + lx, d.W), py = pix_ndc(tr0 + ly, d.H);
+                PixEval p;
+                eval_pixel(v, px, py, p);
+                if ((R.cls != kHomog || homog_inside(p)) && p.s != 0.f) {
+                    float zw = p.zn / p.wn;
+                    if (zw >= -1.f && zw <= 1.f) {
+                        unsigned long long key = frag_key(zw, R.t);
+                        int q = ly * TW + lx;
+                        if (!PEEL || key > pkey[q]) atomicMin(&zbuf[q], key);
+                    }
+                }
+            }
+            __syncthreads();
         }
         __syncthreads();
     }
