@@ -54,7 +54,7 @@ struct RasterWs {
     size_t bytes;
 };
 
-RasterWs raster_ws_layout(void *base, long long n_tiles, int n_img, long long list_total, int bin_cap) {
+RasterWs raster_ws_layout(void *base, long long n_tiles, int n_img, long long list_total, long long bin_slots) {
     RasterWs w;
     size_t off = 0;
     char *b = (char *)base;
@@ -63,7 +63,7 @@ RasterWs raster_ws_layout(void *base, long long n_tiles, int n_img, long long li
     w.list_count = w.bin_count + n_tiles;
     off = align256(sizeof(int) * (size_t)(n_tiles + n_img));
     w.bins = (int *)(b + off);
-    off += align256(sizeof(int) * (size_t)n_tiles * (size_t)bin_cap);
+    off += align256(sizeof(int) * (size_t)bin_slots);
     w.lists = (int *)(b + off);
     off += align256(sizeof(int) * lt);
     w.list_box = (int2 *)(b + off);
@@ -90,7 +90,7 @@ size_t drp_rasterize_workspace_size(int n_img, int H, int W, long long list_tota
                                     int bin_cap) {
     if (n_img <= 0 || H <= 0 || W <= 0 || tile_w <= 0 || tile_h <= 0 || bin_cap <= 0) return 0;
     long long tx = (W + tile_w - 1) / tile_w, ty = (H + tile_h - 1) / tile_h;
-    return raster_ws_layout(nullptr, tx * ty * n_img, n_img, list_total, bin_cap).bytes;
+    return raster_ws_layout(nullptr, tx * ty * n_img, n_img, list_total, tx * ty * n_img * (long long)bin_cap).bytes;
 }
 
 int drp_rasterize_fwd(void *stream, const float *pos, int n_img, int V, int pos_batched, const int32_t *tri, int F,
@@ -123,7 +123,8 @@ int drp_rasterize_fwd(void *stream, const float *pos, int n_img, int V, int pos_
         list_total = (long long)n_img * F;
         max_range = F;
     }
-    RasterWs ws = raster_ws_layout(workspace, n_tiles, n_img, list_total, bin_cap);
+    b.bin_cap = bin_cap;
+    RasterWs ws = raster_ws_layout(workspace, n_tiles, n_img, list_total, n_tiles * (long long)bin_cap);
     if (!workspace || workspace_bytes < ws.bytes)
         return fail(-1, "workspace too small: need %zu bytes, got %zu", ws.bytes, workspace_bytes);
     RasterLaunch L{};
@@ -132,7 +133,6 @@ int drp_rasterize_fwd(void *stream, const float *pos, int n_img, int V, int pos_
     L.tri = tri;
     L.tile_w = tile_w;
     L.tile_h = tile_h;
-    L.bin_cap = bin_cap;
     L.bin_count = ws.bin_count;
     L.list_count = ws.list_count;
     L.bins = ws.bins;
@@ -147,6 +147,157 @@ int drp_rasterize_fwd(void *stream, const float *pos, int n_img, int V, int pos_
     L.rast_db = rast_db;
     int rc = launch_rasterize(L, (hipStream_t)stream);
     if (rc) return fail(rc, "rasterize launch failed: %s", hipGetErrorString(hipGetLastError()));
+    return 0;
+}
+
+// ---- uneven batches --------------------------------------------------------
+// Plan (host and device copies byte-identical): header, ImageDesc[n_img]
+// (16-byte aligned), tile -> image map int32[n_tiles].
+struct BatchPlanHeader {
+    int n_img, tile_w, tile_h, max_count;
+    long long n_tiles, n_pix, n_tri, n_vert, bin_slots;
+};
+
+// bin capacity of an item when none is given: ~6 F / tiles + 32 (a closed mesh
+// covers ~2 tiles per triangle, dense tiles ~3x the mean), a power of two
+static int auto_bin_cap(int F, long long tiles) {
+    double est = 6.0 * F / (double)(tiles > 0 ? tiles : 1) + 32.0;
+    int cap = 32;
+    while (cap < est && cap < (1 << 16)) cap *= 2;
+    return cap;
+}
+
+static size_t plan_desc_off() { return (sizeof(BatchPlanHeader) + 15) & ~(size_t)15; }
+static size_t plan_t2i_off(int n_img) { return plan_desc_off() + (size_t)n_img * sizeof(ImageDesc); }
+
+static long long plan_tiles(const int32_t *items, int n_img, int tile_w, int tile_h) {
+    long long n = 0;
+    for (int i = 0; i < n_img; ++i)
+        n += (long long)((items[5 * i + 3] + tile_w - 1) / tile_w) * ((items[5 * i + 2] + tile_h - 1) / tile_h);
+    return n;
+}
+
+size_t drp_batch_plan_size(const int32_t *items, int n_img, int tile_w, int tile_h) {
+    if (!items || n_img <= 0 || !tile_ok(tile_w, tile_h)) return 0;
+    return plan_t2i_off(n_img) + sizeof(int32_t) * (size_t)plan_tiles(items, n_img, tile_w, tile_h);
+}
+
+int drp_batch_plan(const int32_t *items, int n_img, int tile_w, int tile_h, void *plan, size_t plan_bytes) {
+    if (!items || n_img <= 0) return fail(-1, "empty batch");
+    if (!tile_ok(tile_w, tile_h)) return fail(-1, "unsupported tile shape %dx%d", tile_w, tile_h);
+    long long vb = 0, fb = 0, pb = 0, tb = 0, sb = 0;
+    int max_count = 0;
+    for (int i = 0; i < n_img; ++i) {
+        int V = items[5 * i], F = items[5 * i + 1], H = items[5 * i + 2], W = items[5 * i + 3];
+        int cap = items[5 * i + 4];
+        if (V < 0 || F < 0 || cap < 0) return fail(-1, "item %d: negative V, F or bin capacity", i);
+        if (int e = check_dims(1, H, W)) return e;
+        long long nt = (long long)((W + tile_w - 1) / tile_w) * ((H + tile_h - 1) / tile_h);
+        vb += V;
+        fb += F;
+        pb += (long long)H * W;
+        tb += nt;
+        sb += nt * (cap > 0 ? cap : auto_bin_cap(F, nt));
+        max_count = F > max_count ? F : max_count;
+    }
+    if (tb > 0x7fffffffll || fb > 0x7fffffffll || vb > 0x7fffffffll) return fail(-1, "batch too large");
+    size_t need = plan_t2i_off(n_img) + sizeof(int32_t) * (size_t)tb;
+    if (!plan || plan_bytes < need) return fail(-1, "plan buffer too small: need %zu bytes", need);
+    char *base = (char *)plan;
+    BatchPlanHeader h{n_img, tile_w, tile_h, max_count, tb, pb, fb, vb, sb};
+    memcpy(base, &h, sizeof(h));
+    ImageDesc *D = (ImageDesc *)(base + plan_desc_off());
+    int32_t *t2i = (int32_t *)(base + plan_t2i_off(n_img));
+    vb = fb = pb = tb = sb = 0;
+    for (int i = 0; i < n_img; ++i) {
+        int V = items[5 * i], F = items[5 * i + 1], H = items[5 * i + 2], W = items[5 * i + 3];
+        ImageDesc d{};
+        d.pos_off = 4 * vb;
+        d.pix_off = pb;
+        d.V = V;
+        d.f0 = (int)fb;
+        d.f1 = (int)(fb + F);
+        d.H = H;
+        d.W = W;
+        d.tiles_x = (W + tile_w - 1) / tile_w;
+        d.tiles_y = (H + tile_h - 1) / tile_h;
+        d.tile_off = (int)tb;
+        d.list_off = (int)fb;
+        d.list_cap = F;
+        long long nt = (long long)d.tiles_x * d.tiles_y;
+        d.bin_cap = items[5 * i + 4] > 0 ? items[5 * i + 4] : auto_bin_cap(F, nt);
+        d.bin_off = sb;
+        memcpy(D + i, &d, sizeof(d));
+        for (long long k = 0; k < nt; ++k) t2i[tb + k] = i;
+        vb += V;
+        fb += F;
+        pb += (long long)H * W;
+        tb += nt;
+        sb += nt * d.bin_cap;
+    }
+    return 0;
+}
+
+int drp_batch_plan_info(const void *plan, long long *n_pix, long long *n_tiles, long long *n_tri, long long *n_vert) {
+    if (!plan) return fail(-1, "null plan");
+    BatchPlanHeader h;
+    memcpy(&h, plan, sizeof(h));
+    if (n_pix) *n_pix = h.n_pix;
+    if (n_tiles) *n_tiles = h.n_tiles;
+    if (n_tri) *n_tri = h.n_tri;
+    if (n_vert) *n_vert = h.n_vert;
+    return 0;
+}
+
+size_t drp_rasterize_batch_workspace_size(const void *plan) {
+    if (!plan) return 0;
+    BatchPlanHeader h;
+    memcpy(&h, plan, sizeof(h));
+    return raster_ws_layout(nullptr, h.n_tiles, h.n_img, h.n_tri, h.bin_slots).bytes;
+}
+
+int drp_rasterize_batch_fwd(void *stream, const float *pos, const int32_t *tri, const void *plan,
+                            const void *plan_dev, const float *prev, float *rast, float *rast_db, void *workspace,
+                            size_t workspace_bytes, int32_t *boxes) {
+    if (!plan || !plan_dev) return fail(-1, "null plan");
+    BatchPlanHeader h;
+    memcpy(&h, plan, sizeof(h));
+    if (h.n_img <= 0 || !tile_ok(h.tile_w, h.tile_h)) return fail(-1, "invalid plan");
+    if (!rast) return fail(-1, "rast output is null");
+    if ((h.n_vert > 0 && !pos) || (h.n_tri > 0 && !tri)) return fail(-1, "pos/tri pointer is null");
+    if ((pos && !aligned16(pos)) || !aligned16(rast) || (rast_db && !aligned16(rast_db)) ||
+        (prev && !aligned16(prev)) || !aligned16(plan_dev))
+        return fail(-1, "pos/rast/rast_db/prev/plan must be 16-byte aligned");
+    if (boxes && ((uintptr_t)boxes & 3)) return fail(-1, "boxes must be 4-byte aligned");
+    RasterWs ws = raster_ws_layout(workspace, h.n_tiles, h.n_img, h.n_tri, h.bin_slots);
+    if (!workspace || workspace_bytes < ws.bytes)
+        return fail(-1, "workspace too small: need %zu bytes, got %zu", ws.bytes, workspace_bytes);
+    Batch b{};
+    b.descs = (const ImageDesc *)((const char *)plan_dev + plan_desc_off());
+    b.tile_to_img = (const int *)((const char *)plan_dev + plan_t2i_off(h.n_img));
+    b.n_img = h.n_img;
+    b.V = (int)h.n_vert;
+    b.F = (int)h.n_tri;
+    b.local_ids = 1;
+    RasterLaunch L{};
+    L.b = b;
+    L.pos = pos;
+    L.tri = tri;
+    L.tile_w = h.tile_w;
+    L.tile_h = h.tile_h;
+    L.bin_count = ws.bin_count;
+    L.list_count = ws.list_count;
+    L.bins = ws.bins;
+    L.lists = ws.lists;
+    L.list_box = ws.list_box;
+    L.boxes = (ItemBox *)boxes;
+    L.n_tiles = (int)h.n_tiles;
+    L.max_tris = h.max_count;
+    L.prev = prev;
+    L.rast = rast;
+    L.rast_db = rast_db;
+    int rc = launch_rasterize(L, (hipStream_t)stream);
+    if (rc) return fail(rc, "batch rasterize launch failed: %s", hipGetErrorString(hipGetLastError()));
     return 0;
 }
 

@@ -62,6 +62,8 @@ struct ImageDesc {
     int tile_off;        // first global tile of this image
     int list_off;        // offset of this image's spill list
     int list_cap;        // capacity of that list
+    long long bin_off;   // first bin slot of this image's first tile
+    int bin_cap;         // bin capacity of this image's tiles
 };
 
 struct Batch {
@@ -72,6 +74,8 @@ struct Batch {
     int V, F, H, W, pos_batched;
     const int *ranges;         // device [n_img, 3] (start, count, spill-list offset) or nullptr
     int tiles_x, tiles_y;
+    int bin_cap;               // bin capacity of every tile
+    int local_ids;             // rast ids count from each image's first triangle (uneven batches)
 };
 
 __device__ __forceinline__ ImageDesc image_desc(const Batch &b, int img) {
@@ -98,6 +102,8 @@ __device__ __forceinline__ ImageDesc image_desc(const Batch &b, int img) {
     d.tiles_y = b.tiles_y;
     d.tile_off = img * b.tiles_x * b.tiles_y;
     d.list_cap = f1 - f0;
+    d.bin_cap = b.bin_cap;
+    d.bin_off = (long long)d.tile_off * b.bin_cap;
     return d;
 }
 

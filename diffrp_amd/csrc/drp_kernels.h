@@ -16,7 +16,7 @@ struct RasterLaunch {
     Batch b;
     const float *pos;
     const int *tri;
-    int tile_w, tile_h, bin_cap;
+    int tile_w, tile_h;
     int *bin_count;   // [n_tiles], immediately followed by list_count [n_img]
     int *list_count;
     int *bins;        // [n_tiles * bin_cap]

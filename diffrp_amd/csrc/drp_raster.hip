@@ -41,7 +41,7 @@ __device__ __forceinline__ unsigned long long lanemask_lt() {
 // ----------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_setup_bin(Batch b, const float *__restrict__ pos,
                                                    const int *__restrict__ tri, int tile_w, int tile_h,
-                                                   int bin_cap, int *__restrict__ bin_count,
+                                                   int *__restrict__ bin_count,
                                                    int *__restrict__ bins, int *__restrict__ list_count,
                                                    int *__restrict__ lists, int2 *__restrict__ list_box,
                                                    ItemBox *__restrict__ boxes) {
@@ -56,7 +56,7 @@ __global__ void __launch_bounds__(256) k_setup_bin(Batch b, const float *__restr
         if (load_tri(pos, tri, t, d, v)) setup_tri(v, d.H, d.W, s);
         if (boxes) {
             int2 pb = s.cls == kCull ? empty_box() : pack_box(s.c0, s.c1, s.r0, s.r1);
-            boxes[(long long)img * b.F + t] = ItemBox{pb.x, pb.y, 0};  // hit: set by k_fine
+            boxes[(long long)d.list_off + (t - d.f0)] = ItemBox{pb.x, pb.y, 0};  // hit: set by k_fine
         }
     }
     bool live = valid && s.cls != kCull;
@@ -89,7 +89,7 @@ __global__ void __launch_bounds__(256) k_setup_bin(Batch b, const float *__restr
             base = __shfl(base, leader);
             if (same) {
                 int slot = base + __popcll(m & lanemask_lt());
-                if (slot < bin_cap) bins[(long long)lt * bin_cap + slot] = t;
+                if (slot < d.bin_cap) bins[d.bin_off + (long long)(lt - d.tile_off) * d.bin_cap + slot] = t;
                 else spill = true;
             }
             pending &= ~m;
@@ -151,7 +151,7 @@ __device__ __forceinline__ float inv_ord_f32(uint32_t k) {
 
 template <int TW, int TH, bool PEEL, bool DB>
 __global__ void __launch_bounds__(TW *TH) k_fine(Batch b, const float *__restrict__ pos,
-                                                 const int *__restrict__ tri, int bin_cap,
+                                                 const int *__restrict__ tri,
                                                  const int *__restrict__ bin_count, const int *__restrict__ bins,
                                                  const int *__restrict__ list_count, const int *__restrict__ lists,
                                                  const int2 *__restrict__ list_box, int n_tiles, int n_tiles_pad,
@@ -183,7 +183,7 @@ __global__ void __launch_bounds__(TW *TH) k_fine(Batch b, const float *__restric
         unsigned long long pk = ~0ull;  // empty previous layer: nothing can follow
         if (inimg) {
             float4 pv = prev[pix];
-            if (pv.w > 0.f) pk = frag_key(pv.z, float_to_triidx(pv.w) - 1);
+            if (pv.w > 0.f) pk = frag_key(pv.z, float_to_triidx(pv.w) - 1 + (b.local_ids ? d.f0 : 0));
         }
         pkey[tid] = pk;
     }
@@ -192,7 +192,7 @@ __global__ void __launch_bounds__(TW *TH) k_fine(Batch b, const float *__restric
     int cnt = bin_count[tile];
     int lcnt = list_count[img];
     bool full_scan = lcnt > d.list_cap;
-    int nbin = full_scan ? 0 : min(cnt, bin_cap);
+    int nbin = full_scan ? 0 : min(cnt, d.bin_cap);
     int nlist = full_scan ? (d.f1 - d.f0) : lcnt;
     int total = nbin + nlist;
 
@@ -204,7 +204,7 @@ __global__ void __launch_bounds__(TW *TH) k_fine(Batch b, const float *__restric
             int t;
             bool maybe = true;
             if (idx < nbin) {
-                t = bins[(long long)tile * bin_cap + idx];
+                t = bins[d.bin_off + (long long)(tile - d.tile_off) * d.bin_cap + idx];
             } else if (full_scan) {
                 t = d.f0 + (idx - nbin);
             } else {
@@ -372,9 +372,9 @@ __global__ void __launch_bounds__(TW *TH) k_fine(Batch b, const float *__restric
     PixEval p;
     eval_pixel(v, px, py, p);
     float u = p.e[0] / p.s, vv = p.e[1] / p.s;
-    rast[pix] = make_float4(u, vv, zw, triidx_to_float(t + 1));
+    rast[pix] = make_float4(u, vv, zw, triidx_to_float(t + 1 - (b.local_ids ? d.f0 : 0)));
     if (DB) rast_db[pix] = bary_db(v, p, u, vv, d.H, d.W);
-    if (boxes) boxes[(long long)img * b.F + t].hit = 1;  // same value from every winning pixel
+    if (boxes) boxes[(long long)d.list_off + (t - d.f0)].hit = 1;  // same value from every winning pixel
 }
 
 // per-pixel fallback: atomics into grad_pos
@@ -448,7 +448,7 @@ __global__ void k_zero_i32(int *p, long long n) {
 template <int TW, int TH, bool PEEL, bool DB>
 static void launch_fine_v(const RasterLaunch &L, hipStream_t st) {
     int pad = (L.n_tiles + 7) & ~7;
-    hipLaunchKernelGGL((k_fine<TW, TH, PEEL, DB>), dim3(pad), dim3(TW * TH), 0, st, L.b, L.pos, L.tri, L.bin_cap,
+    hipLaunchKernelGGL((k_fine<TW, TH, PEEL, DB>), dim3(pad), dim3(TW * TH), 0, st, L.b, L.pos, L.tri,
                        L.bin_count, L.bins, L.list_count, L.lists, L.list_box, L.n_tiles, pad,
                        (const float4 *)L.prev, (float4 *)L.rast, (float4 *)L.rast_db, L.boxes);
 }
@@ -470,7 +470,7 @@ int launch_rasterize(const RasterLaunch &L, hipStream_t st) {
     hipLaunchKernelGGL(k_zero_i32, dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, st, L.bin_count, nz);
     if (L.max_tris > 0) {
         dim3 grid((L.max_tris + 255) / 256, L.b.n_img);
-        hipLaunchKernelGGL(k_setup_bin, grid, dim3(256), 0, st, L.b, L.pos, L.tri, L.tile_w, L.tile_h, L.bin_cap,
+        hipLaunchKernelGGL(k_setup_bin, grid, dim3(256), 0, st, L.b, L.pos, L.tri, L.tile_w, L.tile_h,
                            L.bin_count, L.bins, L.list_count, L.lists, L.list_box, L.boxes);
     }
     if (L.tile_w == 8 && L.tile_h == 8) launch_fine_t<8, 8>(L, st);

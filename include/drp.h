@@ -54,14 +54,41 @@ int drp_abi_version(void);
  *   boxes (nullable, int32 [n,F,3], 4-byte aligned): per (image, triangle) the
  *   packed pixel box (c0 | c1 << 16, r0 | r1 << 16), c0 > c1 when culled, and a
  *   hit flag, nonzero when the triangle won at least one pixel of the image;
- *   the triangle-centric backward passes read both.  Only triangles inside an
- *   image's range are written. */
+ *   the triangle-centric backward passes read both.  In range mode the layout
+ *   is [list_total,3], image i's range at its spill-list offset. */
 size_t drp_rasterize_workspace_size(int n_img, int H, int W, long long list_total, int tile_w, int tile_h,
                                     int bin_cap);
 int drp_rasterize_fwd(void *stream, const float *pos, int n_img, int V, int pos_batched, const int32_t *tri, int F,
                       const int32_t *ranges_dev, int H, int W, const float *prev, float *rast, float *rast_db,
                       void *workspace, size_t workspace_bytes, int tile_w, int tile_h, int bin_cap,
                       long long list_total, int max_range, int32_t *boxes);
+
+/* ---- uneven batches ------------------------------------------------------
+ * Many meshes of different triangle counts, each at its own resolution, in
+ * one rasterizer launch (the batched form of the DepthPeeler calls at
+ * diffrp/rendering/surface_deferred.py:259-265, one per mesh).  No nvdiffrast
+ * counterpart: nvdiffrast's range mode shares one resolution.
+ *   items (host int32 [n,5]) = (V_i, F_i, H_i, W_i, bin capacity of the
+ *   item's tiles, 0 = from its density); pos = the items' [V_i,4]
+ *   clip-space vertices concatenated, tri = their [F_i,3] triangles
+ *   concatenated with item-local vertex indices.
+ *   drp_batch_plan fills a host plan of drp_batch_plan_size bytes for a tile
+ *   shape; the caller copies it unchanged to 16-byte aligned device memory
+ *   (plan_dev) and passes both copies.
+ *   rast [n_pix,4]: item i's H_i x W_i image at the sum of the previous items'
+ *   pixel counts, bottom row first; ids are item-local (triangle - first
+ *   triangle of the item + 1), so each image is exactly what rasterizing the
+ *   item alone gives, and the per-item interpolate / antialias / backward
+ *   calls apply to its slices.  prev (nullable) is the previous layer in the
+ *   same layout (depth peeling).  boxes (nullable) int32 [n_tri,3] as for
+ *   drp_rasterize_fwd, indexed by the batch triangle. */
+size_t drp_batch_plan_size(const int32_t *items, int n_img, int tile_w, int tile_h);
+int drp_batch_plan(const int32_t *items, int n_img, int tile_w, int tile_h, void *plan, size_t plan_bytes);
+int drp_batch_plan_info(const void *plan, long long *n_pix, long long *n_tiles, long long *n_tri, long long *n_vert);
+size_t drp_rasterize_batch_workspace_size(const void *plan);
+int drp_rasterize_batch_fwd(void *stream, const float *pos, const int32_t *tri, const void *plan,
+                            const void *plan_dev, const float *prev, float *rast, float *rast_db, void *workspace,
+                            size_t workspace_bytes, int32_t *boxes);
 
 /* Backward of rasterize: autograd of `rast` as diffrp consumes it
  * (u, v through diffrp/rendering/interpolator.py:42 and :92).  Accumulates
