@@ -360,6 +360,110 @@ static inline const real *SFX(img_pos)(const real *pos, int pos_batched, int img
  * of (triangle + 1), in f64 it holds (triangle + 1) as a plain number.
  * prev (nullable): previous depth layer in the same format (peeling).
  * db (nullable): [n, H, W, 4] = (du/dX, du/dY, dv/dX, dv/dY), pixel units. */
+/* Winner's (u, v, z/w, id) and rast_db at pixel (r, c). */
+static void SFX(write_winner)(const real *P, int V, const int32_t *tri, int t, real zw, int r, int c, int H, int W,
+                              real *o, real *D) {
+    SFX(ora_setup) s;
+    SFX(setup_tri)(P, V, tri + 3 * (int64_t)t, H, W, &s);
+    SFX(ora_pix) p;
+    SFX(eval_pixel)(&s, r, c, H, W, &p);
+    real u = p.e[0] / p.s, v = p.e[1] / p.s;
+    o[0] = u;
+    o[1] = v;
+    o[2] = zw;
+#ifdef ORA_DOUBLE
+    o[3] = (real)(t + 1);
+#else
+    o[3] = ora_triidx_to_float(t + 1);
+#endif
+    if (D) {
+        real ex[3], ey[3];
+        ex[0] = FMA(s.w[2], p.Y[1], -(s.w[1] * p.Y[2]));
+        ex[1] = FMA(s.w[0], p.Y[2], -(s.w[2] * p.Y[0]));
+        ex[2] = FMA(s.w[1], p.Y[0], -(s.w[0] * p.Y[1]));
+        ey[0] = FMA(s.w[1], p.X[2], -(s.w[2] * p.X[1]));
+        ey[1] = FMA(s.w[2], p.X[0], -(s.w[0] * p.X[2]));
+        ey[2] = FMA(s.w[0], p.X[1], -(s.w[1] * p.X[0]));
+        real sx = (ex[0] + ex[1]) + ex[2], sy = (ey[0] + ey[1]) + ey[2];
+        real kx = (real)2 / (real)W, ky = (real)2 / (real)H;
+        D[0] = (FMA(-u, sx, ex[0]) / p.s) * kx;
+        D[1] = (FMA(-u, sy, ey[0]) / p.s) * ky;
+        D[2] = (FMA(-v, sx, ex[1]) / p.s) * kx;
+        D[3] = (FMA(-v, sy, ey[1]) / p.s) * ky;
+    }
+}
+
+#ifndef ORA_DOUBLE
+static inline float inv_ord_f32(uint32_t k) { return u2f((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k); }
+
+static inline void atomic_min_u64(uint64_t *p, uint64_t v) {
+    uint64_t cur = __atomic_load_n(p, __ATOMIC_RELAXED);
+    while (v < cur && !__atomic_compare_exchange_n(p, &cur, v, 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {
+    }
+}
+
+/* float: the depth test is the minimum of 64-bit keys (orderable(z/w) << 32 |
+ * triangle), so triangles run in parallel with an atomic minimum and the
+ * result does not depend on the order; then the winners, per pixel. */
+int SFX(ora_rasterize)(const real *pos, int n_img, int V, int pos_batched, const int32_t *tri, int F,
+                       const int32_t *ranges, int H, int W, const real *prev, real *rast, real *db) {
+    int64_t npx = (int64_t)H * W;
+    uint64_t *key = (uint64_t *)malloc(sizeof(uint64_t) * npx);
+    if (!key) return -1;
+    for (int img = 0; img < n_img; ++img) {
+        const real *P = SFX(img_pos)(pos, pos_batched, img, V);
+        int f0 = ranges ? ranges[2 * img] : 0;
+        int f1 = ranges ? ranges[2 * img] + ranges[2 * img + 1] : F;
+        if (f0 < 0) f0 = 0;
+        if (f1 > F) f1 = F;
+        const real *pv = prev ? prev + (int64_t)img * npx * 4 : NULL;
+        for (int64_t i = 0; i < npx; ++i) key[i] = ~0ull;
+#pragma omp parallel for schedule(dynamic, 4096)
+        for (int t = f0; t < f1; ++t) {
+            SFX(ora_setup) s;
+            SFX(setup_tri)(P, V, tri + 3 * (int64_t)t, H, W, &s);
+            if (!s.cls) continue;
+            for (int r = s.r0; r <= s.r1; ++r)
+                for (int c = s.c0; c <= s.c1; ++c) {
+                    SFX(ora_pix) p;
+                    SFX(eval_pixel)(&s, r, c, H, W, &p);
+                    if (!SFX(covers)(&s, r, c, &p)) continue;
+                    if (p.s == 0) continue;
+                    real zw = p.zn / p.wn;
+                    if (!(zw >= (real)-1 && zw <= (real)1)) continue;
+                    int64_t pi = (int64_t)r * W + c;
+                    uint64_t k = ((uint64_t)ord_f32(zw) << 32) | (uint32_t)t;
+                    if (pv) {
+                        float pid = pv[4 * pi + 3];
+                        if (pid <= 0) continue;
+                        uint64_t pkey = ((uint64_t)ord_f32(pv[4 * pi + 2]) << 32) | (uint32_t)(ora_float_to_triidx(pid) - 1);
+                        if (k <= pkey) continue;
+                    }
+                    atomic_min_u64(&key[pi], k);
+                }
+        }
+        real *R = rast + (int64_t)img * npx * 4;
+        real *D = db ? db + (int64_t)img * npx * 4 : NULL;
+#pragma omp parallel for schedule(static)
+        for (int r = 0; r < H; ++r)
+            for (int c = 0; c < W; ++c) {
+                int64_t pi = (int64_t)r * W + c;
+                real *o = R + 4 * pi;
+                if (key[pi] == ~0ull) {
+                    o[0] = o[1] = o[2] = o[3] = 0;
+                    if (D) D[4 * pi] = D[4 * pi + 1] = D[4 * pi + 2] = D[4 * pi + 3] = 0;
+                    continue;
+                }
+                int t = (int)(uint32_t)(key[pi] & 0xffffffffu);
+                SFX(write_winner)(P, V, tri, t, inv_ord_f32((uint32_t)(key[pi] >> 32)), r, c, H, W, o,
+                                  D ? D + 4 * pi : NULL);
+            }
+    }
+    free(key);
+    return 0;
+}
+#else
+/* double: a sequential minimum over (z/w, triangle) */
 int SFX(ora_rasterize)(const real *pos, int n_img, int V, int pos_batched, const int32_t *tri, int F,
                        const int32_t *ranges, int H, int W, const real *prev, real *rast, real *db) {
     int64_t npx = (int64_t)H * W;
@@ -387,7 +491,6 @@ int SFX(ora_rasterize)(const real *pos, int n_img, int V, int pos_batched, const
                     real zw = p.zn / p.wn;
                     if (!(zw >= (real)-1 && zw <= (real)1)) continue;
                     int64_t pi = (int64_t)r * W + c;
-#ifdef ORA_DOUBLE
                     if (pv) {
                         real pid = pv[4 * pi + 3], pz = pv[4 * pi + 2];
                         if (pid <= 0) continue;
@@ -395,16 +498,6 @@ int SFX(ora_rasterize)(const real *pos, int n_img, int V, int pos_batched, const
                         if (!(zw > pz || (zw == pz && t > pt))) continue;
                     }
                     if (bt[pi] < 0 || zw < bz[pi] || (zw == bz[pi] && t < bt[pi])) { bz[pi] = zw; bt[pi] = t; }
-#else
-                    uint64_t key = ((uint64_t)ord_f32(zw) << 32) | (uint32_t)t;
-                    if (pv) {
-                        float pid = pv[4 * pi + 3];
-                        if (pid <= 0) continue;
-                        uint64_t pkey = ((uint64_t)ord_f32(pv[4 * pi + 2]) << 32) | (uint32_t)(ora_float_to_triidx(pid) - 1);
-                        if (key <= pkey) continue;
-                    }
-                    if (bt[pi] < 0 || key < (((uint64_t)ord_f32(bz[pi]) << 32) | (uint32_t)bt[pi])) { bz[pi] = zw; bt[pi] = t; }
-#endif
                 }
         }
         real *R = rast + (int64_t)img * npx * 4;
@@ -418,41 +511,14 @@ int SFX(ora_rasterize)(const real *pos, int n_img, int V, int pos_batched, const
                     if (D) D[4 * pi] = D[4 * pi + 1] = D[4 * pi + 2] = D[4 * pi + 3] = 0;
                     continue;
                 }
-                int t = bt[pi];
-                SFX(ora_setup) s;
-                SFX(setup_tri)(P, V, tri + 3 * (int64_t)t, H, W, &s);
-                SFX(ora_pix) p;
-                SFX(eval_pixel)(&s, r, c, H, W, &p);
-                real u = p.e[0] / p.s, v = p.e[1] / p.s;
-                o[0] = u;
-                o[1] = v;
-                o[2] = bz[pi];
-#ifdef ORA_DOUBLE
-                o[3] = (real)(t + 1);
-#else
-                o[3] = ora_triidx_to_float(t + 1);
-#endif
-                if (D) {
-                    real ex[3], ey[3];
-                    ex[0] = FMA(s.w[2], p.Y[1], -(s.w[1] * p.Y[2]));
-                    ex[1] = FMA(s.w[0], p.Y[2], -(s.w[2] * p.Y[0]));
-                    ex[2] = FMA(s.w[1], p.Y[0], -(s.w[0] * p.Y[1]));
-                    ey[0] = FMA(s.w[1], p.X[2], -(s.w[2] * p.X[1]));
-                    ey[1] = FMA(s.w[2], p.X[0], -(s.w[0] * p.X[2]));
-                    ey[2] = FMA(s.w[0], p.X[1], -(s.w[1] * p.X[0]));
-                    real sx = (ex[0] + ex[1]) + ex[2], sy = (ey[0] + ey[1]) + ey[2];
-                    real kx = (real)2 / (real)W, ky = (real)2 / (real)H;
-                    D[4 * pi + 0] = (FMA(-u, sx, ex[0]) / p.s) * kx;
-                    D[4 * pi + 1] = (FMA(-u, sy, ey[0]) / p.s) * ky;
-                    D[4 * pi + 2] = (FMA(-v, sx, ex[1]) / p.s) * kx;
-                    D[4 * pi + 3] = (FMA(-v, sy, ey[1]) / p.s) * ky;
-                }
+                SFX(write_winner)(P, V, tri, bt[pi], bz[pi], r, c, H, W, o, D ? D + 4 * pi : NULL);
             }
     }
     free(bz);
     free(bt);
     return 0;
 }
+#endif
 
 static inline int32_t SFX(decode_id)(real x) {
 #ifdef ORA_DOUBLE
