@@ -195,6 +195,36 @@ def per_op_times(S, iters=10):
     return res
 
 
+# kernels each operator call launches (the unit of the per-operator timings)
+OP_KERNELS = {
+    'rasterize_fwd': ['k_zero_i32', 'k_setup_bin', 'k_fine'],
+    'interpolate_fwd': ['k_interp_fwd'],
+    'antialias_fwd': ['k_aa_project', 'k_aa_orient', 'k_aa_silhouette', 'k_aa_pairs', 'k_aa_gather'],
+    'antialias_bwd': ['k_aa_pairs_bwd', 'k_aa_gather_bwd'],
+    'interpolate_bwd': ['k_tile_reduce<1,C>', 'k_tile_gather<1,C>', 'k_gather_attr'],
+    'rasterize_bwd': ['k_tile_reduce<0>', 'k_tile_gather<0>', 'k_gather_pos'],
+}
+
+TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'profiles', 'traffic_64view_512.txt')
+
+
+def measured_traffic(op, S, subdiv):
+    """HBM bytes per call of ``op`` from the committed counter summary (FETCH_SIZE and
+    WRITE_SIZE passes, calibrated on a 256 MiB copy), when it covers this workload."""
+    try:
+        lines = open(TRAFFIC_FILE).read().splitlines()
+    except OSError:
+        return None
+    want = f'workload views={S.n} res={S.H}x{S.W} subdiv={subdiv} channels={S.C}'
+    if want not in lines:
+        return None
+    for ln in lines:
+        f = ln.split()
+        if len(f) == 4 and f[0] == 'op' and f[1] == op and f[2] == 'bytes':
+            return int(f[3])
+    return None
+
+
 def cpu_baseline(S, views):
     """The CPU oracle (scalar C restatement + torch-CPU loss) on a bounded sample of views."""
     import oracle
@@ -241,12 +271,21 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-ops', action='store_true')
     ap.add_argument('--no-tune', action='store_true')
+    ap.add_argument('--pmc-calibration', action='store_true',
+                    help='first copy 256 MiB device-to-device three times: the reference for counter scaling')
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
         print('bench.py measures one GPU: the rasterisation path has no multi-GPU mode in this build '
               '(it would shard by view); run with --gpus 1', file=sys.stderr)
         sys.exit(2)
     dev = torch.device('cuda:0')
+    if args.pmc_calibration:
+        a = torch.ones(64 * 2**20, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     S = Scene(dev, args.views, args.res, args.subdiv, args.channels)
     tune = None
     if not args.no_tune:
@@ -297,8 +336,9 @@ def main():
     roof = None
     if ops:
         dom = max(ops, key=lambda k: ops[k]['ms'])
-        roof = {'bound': 'hbm', 'kernel': dom, 'achieved': round(ops[dom]['gbs'], 1), 'peak': HBM_PEAK_GBS,
-                'unit': 'GB/s', 'frac': round(ops[dom]['gbs'] / HBM_PEAK_GBS, 4), 'traffic': None,
+        roof = {'bound': 'hbm', 'kernel': dom, 'kernels': OP_KERNELS[dom], 'achieved': round(ops[dom]['gbs'], 1),
+                'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': round(ops[dom]['gbs'] / HBM_PEAK_GBS, 4),
+                'traffic': measured_traffic(dom, S, args.subdiv),
                 'bytes_per_launch': ops[dom]['bytes'], 'ms_per_launch': round(ops[dom]['ms'], 4)}
     cpu = None if args.no_cpu else cpu_baseline(S, args.cpu_views)
 
