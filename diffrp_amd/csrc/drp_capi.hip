@@ -303,7 +303,8 @@ int drp_rasterize_batch_fwd(void *stream, const float *pos, const int32_t *tri, 
 
 size_t drp_rasterize_bwd_workspace_size(int n_img, int H, int W, int F) {
     if (n_img <= 0 || F <= 0 || H <= 0 || W <= 0) return 0;
-    return align256((size_t)n_img * (size_t)F * 3 * 16) + tile_records_bytes(n_img, H, W, 12);
+    return align256((size_t)n_img * (size_t)F * 3 * 16) + tile_records_bytes(n_img, H, W, 12) +
+           align256((size_t)F * 3 * 4);
 }
 
 int drp_rasterize_bwd(void *stream, const float *pos, int n_img, int V, int pos_batched, const int32_t *tri, int F,
@@ -326,6 +327,7 @@ int drp_rasterize_bwd(void *stream, const float *pos, int n_img, int V, int pos_
         L.boxes = (const ItemBox *)boxes;
         L.partial = (float *)workspace;
         L.records = (char *)workspace + align256((size_t)n_img * (size_t)F * 3 * 16);
+        L.corner_slot = (int *)((char *)L.records + tile_records_bytes(n_img, H, W, 12));
         L.inc_off = inc_off;
         L.inc = inc;
     }

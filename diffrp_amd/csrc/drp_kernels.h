@@ -40,8 +40,9 @@ struct RasterBwdLaunch {
     int pix_per_img;
     // triangle-centric path (all set) or per-pixel atomics (any null)
     const ItemBox *boxes;
-    float *partial;          // [n_img, F, 3] float4
+    float *partial;          // [n_img, 3F] float4, corners in vertex (CSR) order
     void *records;           // tile records (tile_records_bytes(n_img, H, W, 12))
+    int *corner_slot;        // [3F]: position of corner 3t+k in the incidence list
     const int *inc_off, *inc;
 };
 
@@ -105,6 +106,7 @@ struct TriBwdLaunch {
     float *grad_rast_out;    // attribute mode: grad_rast written by the tile pass
     const ItemBox *boxes;
     float *partial;
+    const int *corner_slot;  // positions mode: partial index of corner 3t+k (vertex order)
     int n_img, F, H, W;
     int views_per_block;     // images summed per partial (attribute mode); 1 for positions
     // per-tile records (16 x 16 tiles): count, triangle, sums

@@ -418,7 +418,7 @@ __global__ void __launch_bounds__(256) k_raster_bwd(Batch b, const float *__rest
 // (pos_batched: one output per image; else all images sum into one buffer)
 __global__ void __launch_bounds__(256) k_gather_pos(const float4 *__restrict__ partial, int n_img, int F, int V,
                                                     int pos_batched, const int *__restrict__ inc_off,
-                                                    const int *__restrict__ inc, float4 *__restrict__ grad_pos) {
+                                                    float4 *__restrict__ grad_pos) {
     int v = blockIdx.x * blockDim.x + threadIdx.x;
     int oimg = blockIdx.y;
     if (v >= V) return;
@@ -426,15 +426,21 @@ __global__ void __launch_bounds__(256) k_gather_pos(const float4 *__restrict__ p
     float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
     int ia = pos_batched ? oimg : 0, ib = pos_batched ? oimg + 1 : n_img;
     for (int img = ia; img < ib; ++img) {
-        const float4 *P = partial + (long long)img * F * 3;
+        const float4 *P = partial + (long long)img * F * 3;  // this vertex's corners are P[i0 .. i1)
         for (int i = i0; i < i1; ++i) {
-            float4 q = P[inc[i]];
+            float4 q = P[i];
             s.x += q.x; s.y += q.y; s.z += q.z; s.w += q.w;
         }
     }
     float4 *o = grad_pos + (long long)oimg * V + v;
     float4 cur = *o;
     *o = make_float4(cur.x + s.x, cur.y + s.y, cur.z + s.z, cur.w + s.w);
+}
+
+// slot[c] = position of corner c in the incidence list (inverse permutation)
+__global__ void k_corner_slots(const int *__restrict__ inc, long long n, int *__restrict__ slot) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) slot[inc[i]] = (int)i;
 }
 
 // ----------------------------------------------------------------------------
@@ -499,11 +505,15 @@ int launch_rasterize_bwd(const RasterBwdLaunch &L, hipStream_t st) {
         T.W = L.b.W;
         T.views_per_block = 1;
         tile_records_layout(T, L.records, L.b.n_img, L.b.H, L.b.W, 12);
+        long long nc = 3ll * L.b.F;
+        hipLaunchKernelGGL(k_corner_slots, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, st, L.inc, nc,
+                           L.corner_slot);
+        T.corner_slot = L.corner_slot;
         if (int rc = launch_tri_bwd_pos(T, st)) return rc;
         int n_out = L.b.pos_batched ? L.b.n_img : 1;
         dim3 gg((L.b.V + 255) / 256, n_out);
         hipLaunchKernelGGL(k_gather_pos, gg, dim3(256), 0, st, (const float4 *)L.partial, L.b.n_img, L.b.F, L.b.V,
-                           L.b.pos_batched, L.inc_off, L.inc, (float4 *)L.grad_pos);
+                           L.b.pos_batched, L.inc_off, (float4 *)L.grad_pos);
         return hipGetLastError() == hipSuccess ? 0 : -2;
     }
     dim3 grid((unsigned)((L.total_pix + 255) / 256));

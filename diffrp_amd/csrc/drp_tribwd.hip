@@ -15,8 +15,9 @@
 // for bit.
 //
 //   POS  (NV = 12): d rast (u, v, z/w [, db]) -> d clip (x, y, z, w) of the 3
-//        corners (double-precision chain, drp_grad.h); partial [n, F, 3] float4,
-//        reduced per vertex by k_gather_pos.
+//        corners (double-precision chain, drp_grad.h); partial [n, 3F] float4
+//        with the corners of each vertex adjacent (incidence order), summed per
+//        vertex by k_gather_pos.
 //   ATTR (NV = 3 C): (u, v, 1-u-v) * g per corner; the same pass writes
 //        grad_rast = (<g, a0-a2>, <g, a1-a2>, 0, 0); k_tile_gather sums the
 //        views of a group, partial [groups, F, 3, C], reduced by k_gather_attr.
@@ -216,10 +217,12 @@ __global__ void __launch_bounds__(256) k_tile_gather(TriBwdLaunch L) {
         if (b.hit) gather_records<NV>(L, img, t, b.cc, b.rr, acc);
     }
     if (MODE == 0) {
-        float4 *P = reinterpret_cast<float4 *>(L.partial) + ((long long)grp * L.F + t) * 3;
-        P[0] = make_float4(acc[0], acc[1], acc[2], acc[3]);
-        P[1] = make_float4(acc[4], acc[5], acc[6], acc[7]);
-        P[2] = make_float4(acc[8], acc[9], acc[10], acc[11]);
+        // corners go to their vertex's slots (CSR order), so the vertex gather reads contiguously
+        float4 *P = reinterpret_cast<float4 *>(L.partial) + (long long)grp * 3 * L.F;
+        const int *slot = L.corner_slot + 3 * (long long)t;
+        P[slot[0]] = make_float4(acc[0], acc[1], acc[2], acc[3]);
+        P[slot[1]] = make_float4(acc[4], acc[5], acc[6], acc[7]);
+        P[slot[2]] = make_float4(acc[8], acc[9], acc[10], acc[11]);
     } else {
         float *P = L.partial + ((long long)grp * L.F + t) * NV;
 #pragma unroll
