@@ -30,6 +30,31 @@ namespace drp {
 constexpr int kBT = 16;            // backward tile edge
 constexpr int kBTP = kBT * kBT;    // pixels (and records) per tile
 constexpr int kRowStride = kBTP + 1;
+constexpr unsigned long long kEmptyT = (1ull << 40) - 1;  // triangle field of an uncovered pixel's key
+
+// Bitonic sort of the wave's 64 keys across lanes (ascending by lane), no barriers.
+__device__ __forceinline__ unsigned long long wave_sort64(unsigned long long k) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int size = 2; size <= 64; size <<= 1) {
+#pragma unroll
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            unsigned long long o = __shfl_xor(k, stride, 64);
+            bool keep_min = ((lane & stride) == 0) == ((lane & size) == 0);
+            k = keep_min ? (o < k ? o : k) : (o > k ? o : k);
+        }
+    }
+    return k;
+}
+
+// number of entries of the sorted run a[0..64) below k (reads a[0..63] only)
+__device__ __forceinline__ int count_below64(const unsigned long long *a, unsigned long long k) {
+    int base = 0;
+#pragma unroll
+    for (int half = 32; half >= 1; half >>= 1)
+        if (a[base + half - 1] < k) base += half;
+    return base + (a[base] < k ? 1 : 0);
+}
 
 __device__ __forceinline__ int block_excl_scan256(int v, int *wsum, int &total) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -57,6 +82,7 @@ template <int MODE, int C, bool DB>
 __global__ void __launch_bounds__(256) k_tile_reduce(TriBwdLaunch L) {
     constexpr int NV = MODE == 0 ? 12 : 3 * C;
     __shared__ unsigned long long key[kBTP];
+    __shared__ unsigned long long run[kBTP];  // the waves' sorted runs
     __shared__ float contrib[NV * kRowStride];
     __shared__ int seg_start[kBTP + 1];
     __shared__ int wsum[4];
@@ -135,26 +161,24 @@ __global__ void __launch_bounds__(256) k_tile_reduce(TriBwdLaunch L) {
     }
 #pragma unroll
     for (int j = 0; j < NV; ++j) contrib[j * kRowStride + tid] = val[j];
-    key[tid] = t >= 0 ? (((unsigned long long)t << 8) | (unsigned)tid) : ~0ull;
-    __syncthreads();
-    // bitonic sort of the 256 keys, ascending (empty pixels last)
-    for (int k = 2; k <= kBTP; k <<= 1) {
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            int ixj = tid ^ j;
-            if (ixj > tid) {
-                unsigned long long a = key[tid], b = key[ixj];
-                bool up = (tid & k) == 0;
-                if ((a > b) == up) {
-                    key[tid] = b;
-                    key[ixj] = a;
-                }
-            }
-            __syncthreads();
-        }
+    // sort the 256 keys (triangle << 8 | pixel; uncovered pixels last, keys unique):
+    // each wave sorts its 64 across lanes, then every key's place is its lane
+    // plus the keys below it in the other three runs
+    {
+        unsigned long long k = wave_sort64(((t >= 0 ? (unsigned long long)t : kEmptyT) << 8) | (unsigned)tid);
+        const int wave = tid >> 6;
+        run[tid] = k;
+        __syncthreads();
+        int place = tid & 63;
+#pragma unroll
+        for (int w = 0; w < 4; ++w)
+            if (w != wave) place += count_below64(run + 64 * w, k);
+        key[place] = k;
+        __syncthreads();
     }
     // segments: runs of one triangle in sorted order
     unsigned long long sk = key[tid];
-    bool valid = sk != ~0ull;
+    bool valid = (sk >> 8) != kEmptyT;
     bool head = valid && (tid == 0 || (key[tid - 1] >> 8) != (sk >> 8));
     int nseg;
     int seg = block_excl_scan256(head ? 1 : 0, wsum, nseg);

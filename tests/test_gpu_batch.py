@@ -113,7 +113,8 @@ def test_batch_gradients_match_per_item(cuda):
     assert abs(lb - lp) <= 1e-6 * abs(lp)
     for i, (a, b) in enumerate(zip(gb, gp)):
         scale = max(np.abs(b).max(), 1e-30)
-        assert np.abs(a - b).max() <= 1e-5 * scale, f'item {i}'
+        err = np.abs(a - b).max()
+        assert err <= 1e-5 * scale, f'item {i}: max |diff| {err:.3e}, scale {scale:.3e}'
         assert np.abs(b).max() > 0
 
 
@@ -143,4 +144,5 @@ def test_uneven_batch_full_size(cuda):
     gp, lp = _loss_grads(pos, tri, res, attrs, targets, False, ctx)
     assert abs(lb - lp) <= 1e-6 * abs(lp)
     for i, (a, b) in enumerate(zip(gb, gp)):
-        assert np.abs(a - b).max() <= 1e-5 * max(np.abs(b).max(), 1e-30), f'item {i}'
+        err, scale = np.abs(a - b).max(), max(np.abs(b).max(), 1e-30)
+        assert err <= 1e-5 * scale, f'item {i}: max |diff| {err:.3e}, scale {scale:.3e}'
