@@ -4,11 +4,11 @@
 // triangle) pixel box, and wave-aggregated insertion into fixed-capacity
 // screen-tile bins (full bins and large triangles spill, with their box, to a
 // per-image list); (2) k_fine, one workgroup per tile: triangle records staged
-// in LDS, then fragment-parallel coverage -- lanes take (record, pixel of its
-// box inside the tile) candidates from a block prefix sum, a chunk of
-// consecutive candidates per lane -- with the depth test resolved by 64-bit
-// LDS atomicMin on (orderable(z/w) << 32 | triangle); (u, v) are recomputed for
-// the winner only.
+// in LDS, then coverage by rows -- lanes take (record, row of its box inside
+// the tile) items from a block prefix sum and find the row's exact covered
+// span; the covered pixels are queued and evaluated by all lanes -- with the
+// depth test resolved by 64-bit LDS atomicMin on (orderable(z/w) << 32 |
+// triangle); (u, v) are recomputed for the winner only.
 //
 // The setup pass also records each (image, triangle)'s pixel box, and k_fine
 // flags the triangles that won a pixel; the deterministic backward
@@ -22,7 +22,8 @@
 namespace drp {
 
 constexpr int kMaxBinTiles = 16;
-constexpr int kChunk = 4;  // fine raster: consecutive candidates per lane
+constexpr int kQueuePerLane = 2;  // fine raster: covered-pixel queue entries per lane and pass
+constexpr int kFineWavesPerEU = 5;  // occupancy target of k_fine (VGPR budget 512 / 5)
 
 __device__ __forceinline__ int2 pack_box(int c0, int c1, int r0, int r1) {
     return make_int2(c0 | (c1 << 16), r0 | (r1 << 16));
@@ -116,12 +117,57 @@ __global__ void __launch_bounds__(256) k_setup_bin(Batch b, const float *__restr
 // fine raster
 // ----------------------------------------------------------------------------
 struct FineRec {
-    long long E0[3];     // edge functions (with top-left bias) at the tile's first pixel centre
-    int A[3], B[3];      // per-sub-pixel increments in x and y
+    long long E0[3];  // edge functions (with top-left bias) at the tile's first pixel centre
+    int A[3], B[3];   // per-sub-pixel increments in x and y
     TriVerts v;
-    int t, cls;
-    int bx0, by0, bw, bh;  // candidate box inside the tile (tile-local)
+    int t;
+    int box;          // tile-local box columns x0, x1 and first row y0 (6 bits each), class << 18
 };
+__device__ __forceinline__ int rec_x0(int box) { return box & 63; }
+__device__ __forceinline__ int rec_y0(int box) { return (box >> 6) & 63; }
+__device__ __forceinline__ int rec_x1(int box) { return (box >> 12) & 63; }
+__device__ __forceinline__ int rec_cls(int box) { return box >> 18; }
+
+// Covered columns of tile row y of a direct-class record, intersected with
+// [xl, xr]: every edge needs E_e(x) = C_e + a_e x >= 0 (C_e = E0_e + 16 B_e y,
+// a_e = 16 A_e, 64-bit).  The crossing -C_e / a_e is estimated in floating
+// point (clamped to [xl - 1, xr + 1], where its error is far below one
+// column) and settled by one exact 64-bit step either way, so the span is
+// exactly the set of pixels whose three edge functions are non-negative.
+__device__ __forceinline__ void row_span(const FineRec &R, int y, int &xl, int &xr) {
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+        const long long a = (long long)R.A[e] * kSubpix;
+        const long long cc = R.E0[e] + (long long)R.B[e] * (kSubpix * y);
+        if (a == 0) {
+            if (cc < 0) xr = xl - 1;
+            continue;
+        }
+        double q = -(double)cc * (double)__builtin_amdgcn_rcpf((float)R.A[e] * (float)kSubpix);
+        q = fmin(fmax(q, (double)(xl - 1)), (double)(xr + 1));
+        if (a > 0) {  // covered for x >= crossing
+            int x0 = (int)ceil(q);
+            if (cc + a * (x0 - 1) >= 0) --x0;
+            else if (cc + a * x0 < 0) ++x0;
+            xl = max(xl, x0);
+        } else {      // covered for x <= crossing
+            int x0 = (int)floor(q);
+            if (cc + a * (x0 + 1) >= 0) ++x0;
+            else if (cc + a * x0 < 0) --x0;
+            xr = min(xr, x0);
+        }
+    }
+}
+
+// last k in [0, N) with a[k] <= i (a non-decreasing, a[0] = 0 <= i)
+template <int N>
+__device__ __forceinline__ int search_last_le(const int *a, int i) {
+    int k = 0;
+#pragma unroll
+    for (int half = N / 2; half >= 1; half >>= 1)
+        if (a[k + half] <= i) k += half;
+    return k;
+}
 
 template <int NT>
 __device__ __forceinline__ int block_excl_scan(int v, int *wsum, int &total) {
@@ -150,7 +196,7 @@ __device__ __forceinline__ float inv_ord_f32(uint32_t k) {
 }
 
 template <int TW, int TH, bool PEEL, bool DB>
-__global__ void __launch_bounds__(TW *TH) k_fine(Batch b, const float *__restrict__ pos,
+__global__ void __launch_bounds__(TW *TH, kFineWavesPerEU) k_fine(Batch b, const float *__restrict__ pos,
                                                  const int *__restrict__ tri,
                                                  const int *__restrict__ bin_count, const int *__restrict__ bins,
                                                  const int *__restrict__ list_count, const int *__restrict__ lists,
@@ -161,11 +207,11 @@ __global__ void __launch_bounds__(TW *TH) k_fine(Batch b, const float *__restric
     static_assert(NT % 64 == 0, "tile must be a whole number of waves");
     __shared__ FineRec rec[NT];
     __shared__ int cstart[NT + 1];
-    __shared__ int queue[NT * kChunk];  // covered candidates: record | x << 8 | y << 16 (tile-local)
-    __shared__ int qcount[2];
+    constexpr int kQueue = NT * kQueuePerLane;
+    __shared__ int queue[kQueue];  // covered pixels: record | x << 8 | y << 16 (tile-local)
     __shared__ unsigned long long zbuf[NT];
     __shared__ unsigned long long pkey[PEEL ? NT : 1];
-    __shared__ int wsum[NT / 64];
+    __shared__ int wsum[3 * (NT / 64)];  // scan partials: records; row spans (two, alternating)
     int tile = xcd_remap(blockIdx.x, n_tiles_pad);
     if (tile >= n_tiles) return;
     int img = tile_image(b, tile);
@@ -178,7 +224,6 @@ __global__ void __launch_bounds__(TW *TH) k_fine(Batch b, const float *__restric
     bool inimg = c < d.W && r < d.H;
     long long pix = d.pix_off + (long long)r * d.W + c;
     zbuf[tid] = ~0ull;
-    if (tid < 2) qcount[tid] = 0;
     if (PEEL) {
         unsigned long long pk = ~0ull;  // empty previous layer: nothing can follow
         if (inimg) {
@@ -196,7 +241,7 @@ __global__ void __launch_bounds__(TW *TH) k_fine(Batch b, const float *__restric
     int nlist = full_scan ? (d.f1 - d.f0) : lcnt;
     int total = nbin + nlist;
 
-    int par = 0;  // queue counter in use; alternates every sub-round, reset one sub-round ahead
+    int par = 0;  // scan partials of the row spans in use; alternates every sub-round
     for (int base = 0; base < total; base += NT) {
         int idx = base + tid;
         int count = 0;
@@ -224,13 +269,9 @@ __global__ void __launch_bounds__(TW *TH) k_fine(Batch b, const float *__restric
                     if (x0 <= x1 && y0 <= y1) {
                         FineRec &R = rec[tid];
                         R.t = t;
-                        R.cls = s.cls;
+                        R.box = x0 | (y0 << 6) | (x1 << 12) | (s.cls << 18);
                         R.v = v;
-                        R.bx0 = x0;
-                        R.by0 = y0;
-                        R.bw = x1 - x0 + 1;
-                        R.bh = y1 - y0 + 1;
-                        count = R.bw * R.bh;
+                        count = y1 - y0 + 1;  // row items
                         if (s.cls == kDirect) {
                             long long Px = (long long)kSubpix * tc0 + kHalfSub, Py = (long long)kSubpix * tr0 + kHalfSub;
 #pragma unroll
@@ -247,113 +288,55 @@ __global__ void __launch_bounds__(TW *TH) k_fine(Batch b, const float *__restric
                 }
             }
         }
-        int ntot;
-        int ex = block_excl_scan<NT>(count, wsum, ntot);
+        int nrows;
+        int ex = block_excl_scan<NT>(count, wsum, nrows);
         cstart[tid] = ex;
         if (tid == NT - 1) cstart[NT] = ex + count;
         __syncthreads();
-        // Candidates in sub-rounds of NT * kChunk.  Test: each lane takes
-        // kChunk consecutive candidates (one search and one division per
-        // chunk, the record's fields in registers, E += 16 A along a row) and
-        // queues the covered ones (wave-aggregated append).  Evaluate: the
-        // queued pixels are spread over all lanes, so the barycentric and depth
-        // work runs without divergence.  The depth test is a minimum, so the
-        // queue order does not matter.
-        for (int sb = 0; sb < ntot; sb += NT * kChunk, par ^= 1) {
-            int ents[kChunk];
-            int nent = 0;
-            const int i0 = sb + tid * kChunk;
-            if (i0 < ntot) {
-                // record k: the last slot whose start is <= i0 (empty slots share starts)
-                int k = 0, hi = NT - 1;
-                while (k < hi) {
-                    int mid = (k + hi + 1) >> 1;
-                    if (cstart[mid] <= i0) k = mid;
-                    else hi = mid - 1;
-                }
-                int local = i0 - cstart[k], cnt = cstart[k + 1] - cstart[k];
-                int bw = rec[k].bw;
-                int dy = div_box(local, bw), dx = local - dy * bw;
-                int bx0 = 0, by0 = 0, cls = 0, a16[3] = {0, 0, 0};
-                long long E[3] = {0, 0, 0};
-                bool fresh = true;
-                const int iend = min(i0 + kChunk, ntot);
-#pragma unroll
-                for (int s = 0; s < kChunk; ++s) {
-                    if (i0 + s >= iend) break;
-                    while (local >= cnt) {  // next record with candidates
-                        ++k;
-                        local = 0;
-                        dx = dy = 0;
-                        cnt = cstart[k + 1] - cstart[k];
-                        bw = rec[k].bw;
-                        fresh = true;
-                    }
-                    if (fresh) {  // new record or new row: reload fields, recompute E
-                        const FineRec &R = rec[k];
-                        bx0 = R.bx0;
-                        by0 = R.by0;
-                        cls = R.cls;
-                        if (cls == kDirect) {
-                            int ox = (bx0 + dx) * kSubpix, oy = (by0 + dy) * kSubpix;
-#pragma unroll
-                            for (int e = 0; e < 3; ++e) {
-                                a16[e] = R.A[e] * kSubpix;
-                                E[e] = R.E0[e] + (long long)R.A[e] * ox + (long long)R.B[e] * oy;
-                            }
+        // Row items (record, tile row) in sub-rounds of NT: each lane finds the
+        // exact covered span of its row; the covered pixels are queued (block
+        // prefix sum, at most kQueue per pass) and evaluated by all lanes, so
+        // the barycentric and depth work runs without divergence.  The depth
+        // test is a minimum, so the queue order does not matter.
+        for (int sb = 0; sb < nrows; sb += NT, par ^= 1) {
+            const int ri = sb + tid;
+            int len = 0, k = 0, y = 0, xl = 0;
+            if (ri < nrows) {
+                k = search_last_le<NT>(cstart, ri);  // empty records share their successor's start
+                const FineRec &R = rec[k];
+                const int box = R.box;
+                y = rec_y0(box) + (ri - cstart[k]);
+                xl = rec_x0(box);
+                int xr = rec_x1(box);
+                if (rec_cls(box) == kDirect) row_span(R, y, xl, xr);
+                len = max(xr - xl + 1, 0);
+            }
+            int npix;
+            const int off = block_excl_scan<NT>(len, wsum + (NT / 64) * (1 + par), npix);
+            for (int q0 = 0; q0 < npix; q0 += kQueue) {
+                const int lo = max(off, q0), hi = min(off + len, q0 + kQueue);
+                for (int i = lo; i < hi; ++i) queue[i - q0] = k | ((xl + i - off) << 8) | (y << 16);
+                __syncthreads();
+                const int nq = min(kQueue, npix - q0);
+                for (int qi = tid; qi < nq; qi += NT) {
+                    int en = queue[qi];
+                    const FineRec &R = rec[en & 255];
+                    int lx = (en >> 8) & 255, ly = en >> 16;
+                    TriVerts v = R.v;
+                    float px = pix_ndc(tc0 + lx, d.W), py = pix_ndc(tr0 + ly, d.H);
+                    PixEval p;
+                    eval_pixel(v, px, py, p);
+                    if ((rec_cls(R.box) != kHomog || homog_inside(p)) && p.s != 0.f) {
+                        float zw = p.zn / p.wn;
+                        if (zw >= -1.f && zw <= 1.f) {
+                            unsigned long long key = frag_key(zw, R.t);
+                            int q = ly * TW + lx;
+                            if (!PEEL || key > pkey[q]) atomicMin(&zbuf[q], key);
                         }
-                        fresh = false;
-                    }
-                    if (cls != kDirect || (E[0] | E[1] | E[2]) >= 0)
-                        ents[nent++] = k | ((bx0 + dx) << 8) | ((by0 + dy) << 16);
-                    ++local;
-                    if (++dx == bw) {
-                        dx = 0;
-                        ++dy;
-                        fresh = true;
-                    } else {
-#pragma unroll
-                        for (int e = 0; e < 3; ++e) E[e] += a16[e];
                     }
                 }
+                __syncthreads();
             }
-            // wave-aggregated append
-            {
-                const int lane = tid & 63;
-                int x = nent;
-#pragma unroll
-                for (int o = 1; o < 64; o <<= 1) {
-                    int y = __shfl_up(x, o, 64);
-                    if (lane >= o) x += y;
-                }
-                int wtot = __shfl(x, 63, 64), base = 0;
-                if (lane == 63 && wtot > 0) base = atomicAdd(&qcount[par], wtot);
-                base = __shfl(base, 63, 64) + x - nent;
-#pragma unroll
-                for (int s = 0; s < kChunk; ++s)
-                    if (s < nent) queue[base + s] = ents[s];
-            }
-            __syncthreads();
-            const int nq = qcount[par];
-            if (tid == 0) qcount[par ^ 1] = 0;  // the next sub-round's counter
-            for (int qi = tid; qi < nq; qi += NT) {
-                int en = queue[qi];
-                const FineRec &R = rec[en & 255];
-                int lx = (en >> 8) & 255, ly = en >> 16;
-                TriVerts v = R.v;
-                float px = pix_ndc(tc0 + lx, d.W), py = pix_ndc(tr0 + ly, d.H);
-                PixEval p;
-                eval_pixel(v, px, py, p);
-                if ((R.cls != kHomog || homog_inside(p)) && p.s != 0.f) {
-                    float zw = p.zn / p.wn;
-                    if (zw >= -1.f && zw <= 1.f) {
-                        unsigned long long key = frag_key(zw, R.t);
-                        int q = ly * TW + lx;
-                        if (!PEEL || key > pkey[q]) atomicMin(&zbuf[q], key);
-                    }
-                }
-            }
-            __syncthreads();
         }
         __syncthreads();
     }
