@@ -146,10 +146,11 @@ __global__ void k_aa_orient(const float *__restrict__ pos, int V, const int *__r
     orient[g] = o;
 }
 
-// silh[img][t] = 1 when t has an edge that can be a silhouette in that image:
-// no neighbour, or a neighbour of another screen orientation.  A pair whose
-// candidate triangle has none cannot blend through it, so the pair evaluation
-// skips the edge search for it (same outcome, far fewer gathers).
+// silh[img][t]: bit e (e = 0, 1, 2) when edge e -> e+1 of t can be a silhouette
+// in that image (no neighbour, or a neighbour of another screen orientation),
+// bit 3 when t's screen orientation is negative; 0 for an edge-on triangle.
+// A pair whose candidate triangle has no such edge cannot blend through it, and
+// the pair search reads the edge test and the orientation from this one byte.
 __global__ void k_aa_silhouette(const int *__restrict__ opp, const signed char *__restrict__ orient, int F,
                                 int n_proj, unsigned char *__restrict__ silh) {
     long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -162,8 +163,9 @@ __global__ void k_aa_silhouette(const int *__restrict__ opp, const signed char *
 #pragma unroll
         for (int e = 0; e < 3; ++e) {
             int n = opp[3 * (long long)t + e];
-            if (!(n >= 0 && n < F && O[n] == o)) f = 1;
+            if (!(n >= 0 && n < F && O[n] == o)) f |= (unsigned char)(1 << e);
         }
+        if (f && o < 0) f |= 8;
     }
     silh[g] = f;
 }
@@ -185,15 +187,18 @@ struct AAImg {
     int V, F, H, W;
 };
 
-__device__ __forceinline__ PairRes eval_pair(const AAImg &I, const int *__restrict__ tri,
-                                             const int *__restrict__ opp, int ra, int ca, int rb, int cb) {
+__device__ __forceinline__ int rast_id(float4 p, int F) {  // triangle + 1, 0 for none
+    int id = p.w > 0.f ? float_to_triidx(p.w) : 0;
+    return id > F ? 0 : id;
+}
+
+// Pair (a, b) from the two pixels' rast values and their triangles' silhouette
+// bytes (the caller loads them, so the loads of neighbouring pairs overlap).
+__device__ __forceinline__ PairRes eval_pair(const AAImg &I, const int *__restrict__ tri, float4 pa, float4 pb,
+                                             int sa, int sb, int ra, int ca, int rb, int cb) {
     PairRes res;
     res.recv = -1;
-    float4 pa = I.rast[(long long)ra * I.W + ca], pb = I.rast[(long long)rb * I.W + cb];
-    int ida = pa.w > 0.f ? float_to_triidx(pa.w) : 0;
-    int idb = pb.w > 0.f ? float_to_triidx(pb.w) : 0;
-    if (ida > I.F) ida = 0;
-    if (idb > I.F) idb = 0;
+    int ida = rast_id(pa, I.F), idb = rast_id(pb, I.F);
     if (ida == idb) return res;
     int a_first = 1;
     if (ida == 0) a_first = 0;
@@ -203,7 +208,8 @@ __device__ __forceinline__ PairRes eval_pair(const AAImg &I, const int *__restri
         int id = use_a ? ida : idb;
         if (id == 0) continue;
         int t = id - 1;
-        if (!I.silh[t]) continue;
+        const int m = use_a ? sa : sb;
+        if (!(m & 7)) continue;
         const int *t3 = tri + 3 * (long long)t;
         int vi[3] = {t3[0], t3[1], t3[2]};
         if ((unsigned)vi[0] >= (unsigned)I.V || (unsigned)vi[1] >= (unsigned)I.V || (unsigned)vi[2] >= (unsigned)I.V)
@@ -221,8 +227,7 @@ __device__ __forceinline__ PairRes eval_pair(const AAImg &I, const int *__restri
             Y[k] = q.y;
         }
         if (!front) continue;
-        int o = I.orient[t];
-        if (o == 0) continue;
+        const int o = (m & 8) ? -1 : 1;
         double fx = (double)(use_a ? ca : cb) + 0.5, fy = (double)(use_a ? ra : rb) + 0.5;
         double tx = (double)(use_a ? cb : ca) + 0.5, ty = (double)(use_a ? rb : ra) + 0.5;
         int best = -1;
@@ -238,9 +243,7 @@ __device__ __forceinline__ PairRes eval_pair(const AAImg &I, const int *__restri
             double te = ff / (ff - ft);
             if (best < 0 || te < bt) { best = k; bt = te; }
         }
-        if (best < 0) continue;
-        int n = opp[3 * (long long)t + best];
-        if (n >= 0 && n < I.F && I.orient[n] == o) continue;
+        if (best < 0 || !((m >> best) & 1)) continue;  // no crossing, or the edge has a same-facing neighbour
         int clamped = 0;
         if (bt < 0.0) { bt = 0.0; clamped = 1; }
         if (bt > 1.0) { bt = 1.0; clamped = 1; }
@@ -292,26 +295,34 @@ __device__ __forceinline__ PairRes unpack_decision(int d) {
     return p;
 }
 
+__device__ __forceinline__ int silh_of(const AAImg &I, int id) { return id ? I.silh[id - 1] : 0; }
+
 // weights[pix] = (signed weight of pair (pix, right), of pair (pix, above));
-// decisions[pix] (when kept) the two pairs' decisions for the backward
+// decisions[pix] (when kept) the two pairs' decisions for the backward.
+// Blocks of 64 x 4 pixels (grid: columns, rows, images); the three rast values
+// and silhouette bytes of a pixel's two pairs are loaded before either pair is
+// evaluated.
 __global__ void __launch_bounds__(256) k_aa_pairs(AALaunch L, float2 *__restrict__ weights,
                                                   int2 *__restrict__ decisions) {
-    long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    long long npx = (long long)L.H * L.W;
-    if (g >= (long long)L.n_img * npx) return;
-    int img = (int)(g / npx);
-    long long q = g - img * npx;
-    int r = (int)(q / L.W), c = (int)(q % L.W);
-    AAImg I = aa_img(L, img);
+    const int c = blockIdx.x * 64 + (threadIdx.x & 63), r = blockIdx.y * 4 + (threadIdx.x >> 6), img = blockIdx.z;
+    if (c >= L.W || r >= L.H) return;
+    const AAImg I = aa_img(L, img);
+    const long long q = (long long)r * L.W + c, g = (long long)img * L.H * L.W + q;
+    const bool has_r = c + 1 < L.W, has_u = r + 1 < L.H;
+    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 p0 = I.rast[q], pr = has_r ? I.rast[q + 1] : z4, pu = has_u ? I.rast[q + L.W] : z4;
+    const int i0 = rast_id(p0, I.F), ir = rast_id(pr, I.F), iu = rast_id(pu, I.F);
+    const int s0 = silh_of(I, i0), sr = (has_r && ir != i0) ? silh_of(I, ir) : 0,
+              su = (has_u && iu != i0) ? silh_of(I, iu) : 0;
     float2 w = make_float2(0.f, 0.f);
     int2 dec = make_int2(0, 0);
-    if (c + 1 < L.W) {
-        PairRes p = eval_pair(I, L.tri, L.opp, r, c, r, c + 1);
+    if (has_r) {
+        PairRes p = eval_pair(I, L.tri, p0, pr, s0, sr, r, c, r, c + 1);
         w.x = signed_w(p);
         dec.x = pack_decision(p);
     }
-    if (r + 1 < L.H) {
-        PairRes p = eval_pair(I, L.tri, L.opp, r, c, r + 1, c);
+    if (has_u) {
+        PairRes p = eval_pair(I, L.tri, p0, pu, s0, su, r, c, r + 1, c);
         w.y = signed_w(p);
         dec.y = pack_decision(p);
     }
@@ -515,7 +526,9 @@ __global__ void __launch_bounds__(256) k_aa_pairs_bwd(AALaunch L, float2 *__rest
             for (int dir = 0; dir < 2; ++dir) {
                 int rb = dir ? r + 1 : r, cb = dir ? c : c + 1;
                 if (rb >= L.H || cb >= L.W) continue;
-                PairRes pr = eval_pair(I, L.tri, L.opp, r, c, rb, cb);
+                const float4 pa = I.rast[(long long)r * L.W + c], pb = I.rast[(long long)rb * L.W + cb];
+                const int ia = rast_id(pa, I.F), ib = rast_id(pb, I.F);
+                PairRes pr = eval_pair(I, L.tri, pa, pb, silh_of(I, ia), ia != ib ? silh_of(I, ib) : 0, r, c, rb, cb);
                 float sw = signed_w(pr);
                 if (dir) w.y = sw; else w.x = sw;
                 if (pr.recv < 0 || pr.clamped || !want_pos) continue;
@@ -619,7 +632,7 @@ int launch_antialias(const AALaunch &L, hipStream_t st) {
     long long total = (long long)L.n_img * L.H * L.W;
     if (total == 0) return 0;
     launch_orient(L, st);
-    dim3 grid((unsigned)((total + 255) / 256)), block(256);
+    dim3 grid((unsigned)((L.W + 63) / 64), (unsigned)((L.H + 3) / 4), (unsigned)L.n_img), block(256);
     hipLaunchKernelGGL(k_aa_pairs, grid, block, 0, st, L, L.weights, L.decisions);
     grid = gather_grid(L, total);
     DRP_AA_DISPATCH(k_aa_gather, L, (const float2 *)L.weights)

@@ -525,6 +525,7 @@ static int aa_common(AALaunch &L, void *stream, const float *color, int n_img, i
                      const float *rast, const float *pos, int V, int pos_batched, const int32_t *tri, int F,
                      const int32_t *opp, void *workspace, size_t workspace_bytes) {
     if (int e = check_dims(n_img, H, W)) return e;
+    if (n_img > 65535) return fail(-1, "antialias takes at most 65535 images per call (got %d)", n_img);
     if (C < 1) return fail(-1, "color channel count must be >= 1");
     if (!color || !rast || !pos || !tri || !opp) return fail(-1, "null pointer argument");
     if (!aligned16(pos) || !aligned16(rast)) return fail(-1, "pos/rast must be 16-byte aligned");

@@ -145,7 +145,8 @@ int drp_topology_build(void *stream, const int32_t *tri, int F, int32_t *opp, vo
                        size_t workspace_bytes);
 
 /* Replaces dr.antialias(color, rast, pos, tri) (diffrp/rendering/
- * surface_deferred.py:775, via nvdr_antialias :758-775).  color/out [n,H,W,C]. */
+ * surface_deferred.py:775, via nvdr_antialias :758-775).  color/out [n,H,W,C],
+ * n <= 65535 per call. */
 size_t drp_antialias_workspace_size(int n_img, int H, int W, int V, int F, int pos_batched);
 int drp_antialias_fwd(void *stream, const float *color, int n_img, int H, int W, int C, const float *rast,
                       const float *pos, int V, int pos_batched, const int32_t *tri, int F, const int32_t *opp,
