@@ -402,9 +402,10 @@ __global__ void __launch_bounds__(256) k_aa_gather(AALaunch L, const float2 *__r
 // Per-block accumulation of vertex gradients before the global atomics.
 // Device-scope float atomics on a multi-XCD GPU are performed beyond the
 // per-XCD L2 caches, so every one is a memory transaction; neighbouring
-// silhouette pairs share vertices, and summing them per 16 x 16 tile first
+// silhouette pairs share vertices, and summing them per 32 x 32 tile first
 // leaves one atomic per distinct vertex and component.
-constexpr int kAAHash = 1024;
+constexpr int kAAHashBits = 9;
+constexpr int kAAHash = 1 << kAAHashBits;  // table slots per tile (a 32 x 32 tile touches ~50-100 vertices)
 
 struct VertexAcc {
     int key[kAAHash];
@@ -412,7 +413,7 @@ struct VertexAcc {
 };
 
 __device__ __forceinline__ void vacc_add(VertexAcc &A, float *GP, int v, float gx, float gy, float gw) {
-    unsigned h = ((unsigned)v * 2654435761u) >> 22;  // top 10 bits
+    unsigned h = ((unsigned)v * 2654435761u) >> (32 - kAAHashBits);  // top bits
     for (int probe = 0; probe < 32; ++probe) {
         int k = atomicCAS(&A.key[h], -1, v);
         if (k == -1 || k == v) {
@@ -433,11 +434,27 @@ __device__ __forceinline__ void vacc_add(VertexAcc &A, float *GP, int v, float g
 // gradient and the colour difference) times dw/dt times dt/d(edge vertices).
 // The pair's decisions are the forward's; the derivative is taken in double:
 // t = fa / (fa - fb) is ill-conditioned for edges nearly parallel to the pair.
-__device__ __forceinline__ void pair_pos_grad(const AALaunch &L, const AAImg &I, const float4 *P, float *GP,
-                                              VertexAcc &A, long long g, int r, int c, int dir, const PairRes &pr) {
-    int rb = dir ? r + 1 : r, cb = dir ? c : c + 1;
-    long long ga = g, gb = dir ? g + L.W : g + 1;
-    long long gr = pr.recv == 0 ? ga : gb, gs = pr.recv == 0 ? gb : ga;
+// The edge's vertex loads do not wait for dL/dw, and the result is returned
+// rather than added, so the loads of a pixel's two pairs overlap.
+struct PairGrad {
+    int v[2];
+    float gx[2], gy[2], gw[2];
+    bool live;
+};
+
+__device__ __forceinline__ PairGrad pair_pos_grad(const AALaunch &L, const AAImg &I, const float4 *P, long long g,
+                                                  int r, int c, int dir, const PairRes &pr) {
+    PairGrad out;
+    const int rb = dir ? r + 1 : r, cb = dir ? c : c + 1;
+    const long long ga = g, gb = dir ? g + L.W : g + 1;
+    const long long gr = pr.recv == 0 ? ga : gb, gs = pr.recv == 0 ? gb : ga;
+    const int *t3 = L.tri + 3 * (long long)pr.tri;
+    const int i = pr.edge, j = pr.edge == 2 ? 0 : pr.edge + 1;
+    out.v[0] = t3[i];
+    out.v[1] = t3[j];
+    const double2 q0 = I.proj[out.v[0]], q1 = I.proj[out.v[1]];
+    const float4 pv[2] = {P[out.v[0]], P[out.v[1]]};
+    const double so = I.orient[pr.tri] < 0 ? -1.0 : 1.0;
     double dLdw = 0.0;
     if ((L.C & 3) == 0) {
         const float4 *g4 = reinterpret_cast<const float4 *>(L.grad_out + gr * L.C);
@@ -452,22 +469,17 @@ __device__ __forceinline__ void pair_pos_grad(const AALaunch &L, const AAImg &I,
         for (int ch = 0; ch < L.C; ++ch)
             dLdw += (double)L.grad_out[gr * L.C + ch] * (L.color[gs * L.C + ch] - L.color[gr * L.C + ch]);
     }
-    if (dLdw == 0.0) return;
-    bool recv_is_from = (pr.recv == 0) == (pr.from_a != 0);
-    double dLdt = recv_is_from ? -dLdw : dLdw;
-    const int *t3 = L.tri + 3 * (long long)pr.tri;
-    int i = pr.edge, j = pr.edge == 2 ? 0 : pr.edge + 1;
-    int vv[2] = {t3[i], t3[j]};
-    double2 q0 = I.proj[vv[0]], q1 = I.proj[vv[1]];
-    double fxp = (double)(pr.from_a ? c : cb) + 0.5, fyp = (double)(pr.from_a ? r : rb) + 0.5;
-    double txp = (double)(pr.from_a ? cb : c) + 0.5, typ = (double)(pr.from_a ? rb : r) + 0.5;
-    double so = I.orient[pr.tri] < 0 ? -1.0 : 1.0;
-    double ex = q1.x - q0.x, ey = q1.y - q0.y;
-    double fa = so * (ex * (fyp - q0.y) - ey * (fxp - q0.x));
-    double fb = so * (ex * (typ - q0.y) - ey * (txp - q0.x));
-    double den = fa - fb;
-    double dt_dfa = -fb / (den * den), dt_dfb = fa / (den * den);
-    double k = dLdt * so * (double)L.boost;
+    out.live = dLdw != 0.0;
+    const bool recv_is_from = (pr.recv == 0) == (pr.from_a != 0);
+    const double dLdt = recv_is_from ? -dLdw : dLdw;
+    const double fxp = (double)(pr.from_a ? c : cb) + 0.5, fyp = (double)(pr.from_a ? r : rb) + 0.5;
+    const double txp = (double)(pr.from_a ? cb : c) + 0.5, typ = (double)(pr.from_a ? rb : r) + 0.5;
+    const double ex = q1.x - q0.x, ey = q1.y - q0.y;
+    const double fa = so * (ex * (fyp - q0.y) - ey * (fxp - q0.x));
+    const double fb = so * (ex * (typ - q0.y) - ey * (txp - q0.x));
+    const double den = fa - fb;
+    const double dt_dfa = -fb / (den * den), dt_dfb = fa / (den * den);
+    const double k = dLdt * so * (double)L.boost;
     double gx2[2], gy2[2];
     gx2[0] = k * (dt_dfa * (q1.y - fyp) + dt_dfb * (q1.y - typ));
     gy2[0] = k * (dt_dfa * (fxp - q1.x) + dt_dfb * (txp - q1.x));
@@ -476,21 +488,38 @@ __device__ __forceinline__ void pair_pos_grad(const AALaunch &L, const AAImg &I,
     const double hw = (double)L.W * 0.5, hh = (double)L.H * 0.5;
 #pragma unroll
     for (int m = 0; m < 2; ++m) {
-        float4 pv = P[vv[m]];
-        double iw = 1.0 / (double)pv.w;
+        double iw = 1.0 / (double)pv[m].w;
         double gx = gx2[m] * hw * iw, gy = gy2[m] * hh * iw;
-        vacc_add(A, GP, vv[m], (float)gx, (float)gy, (float)(-(gx * pv.x + gy * pv.y) * iw));
+        out.gx[m] = (float)gx;
+        out.gy[m] = (float)gy;
+        out.gw[m] = (float)(-(gx * pv[m].x + gy * pv[m].y) * iw);
     }
+    return out;
 }
 
-// Backward pairs, one block per 16 x 16 tile of one image.  With the
-// forward's decisions, read them; otherwise evaluate each pair and store its
-// weight.  Silhouette pairs add the position gradient of their edge's two
-// vertices to the tile's table, flushed with one atomic per vertex component.
+__device__ __forceinline__ void pair_grad_add(VertexAcc &A, float *GP, const PairGrad &pg) {
+    if (!pg.live) return;
+#pragma unroll
+    for (int m = 0; m < 2; ++m) vacc_add(A, GP, pg.v[m], pg.gx[m], pg.gy[m], pg.gw[m]);
+}
+
+// Backward pairs, one block (256 threads) per 32 x 32 tile of one image.  With
+// the forward's decisions, read them; otherwise evaluate each pair and store
+// its weight.  The tile's blending pairs (a fifth of all pairs on a folded
+// mesh) are queued in LDS and their position gradients computed by
+// consecutive lanes, so the double-precision chain runs on full waves; the
+// gradients of the edge's two vertices are summed in the tile's table,
+// flushed with one atomic per vertex component.
+constexpr int kAAT = 32;                 // backward tile edge
+constexpr int kAAPix = kAAT * kAAT;      // pixels per block (4 per thread)
+constexpr int kAAQueue = 2 * kAAPix;     // at most two pairs per pixel
+
 __global__ void __launch_bounds__(256) k_aa_pairs_bwd(AALaunch L, float2 *__restrict__ weights,
                                                       const int2 *__restrict__ decisions, int tiles_x,
                                                       int tiles_per_img, int n_blocks_pad) {
     __shared__ VertexAcc A;
+    __shared__ int q_tri[kAAQueue], q_meta[kAAQueue];  // triangle; pixel | dir << 10 | decision bits << 11
+    __shared__ int q_n;
     const int tid = threadIdx.x;
     const int bid = xcd_remap(blockIdx.x, n_blocks_pad);
     const int img = bid / tiles_per_img, lt = bid - img * tiles_per_img;
@@ -502,42 +531,87 @@ __global__ void __launch_bounds__(256) k_aa_pairs_bwd(AALaunch L, float2 *__rest
         A.gy[e] = 0.f;
         A.gw[e] = 0.f;
     }
+    if (tid == 0) q_n = 0;
     __syncthreads();
-    const int r = (lt / tiles_x) * 16 + (tid >> 4), c = (lt % tiles_x) * 16 + (tid & 15);
+    const int r0 = (lt / tiles_x) * kAAT, c0 = (lt % tiles_x) * kAAT;
     const long long npx = (long long)L.H * L.W;
-    float *GP = nullptr;
-    if (live_block && r < L.H && c < L.W) {
-        long long g = (long long)img * npx + (long long)r * L.W + c;
-        AAImg I = aa_img(L, img);
-        const float4 *P = reinterpret_cast<const float4 *>(L.pos) + (L.pos_batched ? (long long)img * L.V : 0);
-        GP = want_pos ? L.grad_pos + (L.pos_batched ? (long long)img * L.V * 4 : 0) : nullptr;
-        if (decisions) {
-            int2 d = want_pos ? decisions[g] : make_int2(0, 0);
-            if (d.x != 0) {
-                PairRes pr = unpack_decision(d.x);
-                if (!pr.clamped) pair_pos_grad(L, I, P, GP, A, g, r, c, 0, pr);
+    const AAImg I = aa_img(L, img < L.n_img ? img : 0);
+    for (int k = 0; k < kAAPix / 256; ++k) {
+        const int px = tid + 256 * k, r = r0 + (px >> 5), c = c0 + (px & 31);
+        // the pixel's blending, unclamped pairs: slot a, slot b (direction: 0 right, 1 above)
+        PairRes qa, qb;
+        int da = 0, db = 0, nq = 0;
+        qa.recv = qb.recv = -1;
+        if (live_block && r < L.H && c < L.W) {
+            long long g = (long long)img * npx + (long long)r * L.W + c;
+            if (decisions) {
+                int2 d = want_pos ? decisions[g] : make_int2(0, 0);
+                const PairRes p0 = unpack_decision(d.x), p1 = unpack_decision(d.y);
+                const bool h0 = d.x != 0 && !p0.clamped, h1 = d.y != 0 && !p1.clamped;
+                qa = h0 ? p0 : p1;
+                da = h0 ? 0 : 1;
+                qb = p1;
+                db = 1;
+                nq = (int)h0 + (int)h1;
+            } else {
+                float2 w = make_float2(0.f, 0.f);
+                for (int dir = 0; dir < 2; ++dir) {
+                    int rb = dir ? r + 1 : r, cb = dir ? c : c + 1;
+                    if (rb >= L.H || cb >= L.W) continue;
+                    const float4 pa = I.rast[(long long)r * L.W + c], pb = I.rast[(long long)rb * L.W + cb];
+                    const int ia = rast_id(pa, I.F), ib = rast_id(pb, I.F);
+                    PairRes pr = eval_pair(I, L.tri, pa, pb, silh_of(I, ia), ia != ib ? silh_of(I, ib) : 0, r, c,
+                                           rb, cb);
+                    float sw = signed_w(pr);
+                    if (dir) w.y = sw; else w.x = sw;
+                    if (pr.recv < 0 || pr.clamped || !want_pos) continue;
+                    if (nq == 0) { qa = pr; da = dir; }
+                    else { qb = pr; db = dir; }
+                    ++nq;
+                }
+                weights[g] = w;
             }
-            if (d.y != 0) {
-                PairRes pr = unpack_decision(d.y);
-                if (!pr.clamped) pair_pos_grad(L, I, P, GP, A, g, r, c, 1, pr);
-            }
-        } else {
-            float2 w = make_float2(0.f, 0.f);
-            for (int dir = 0; dir < 2; ++dir) {
-                int rb = dir ? r + 1 : r, cb = dir ? c : c + 1;
-                if (rb >= L.H || cb >= L.W) continue;
-                const float4 pa = I.rast[(long long)r * L.W + c], pb = I.rast[(long long)rb * L.W + cb];
-                const int ia = rast_id(pa, I.F), ib = rast_id(pb, I.F);
-                PairRes pr = eval_pair(I, L.tri, pa, pb, silh_of(I, ia), ia != ib ? silh_of(I, ib) : 0, r, c, rb, cb);
-                float sw = signed_w(pr);
-                if (dir) w.y = sw; else w.x = sw;
-                if (pr.recv < 0 || pr.clamped || !want_pos) continue;
-                pair_pos_grad(L, I, P, GP, A, g, r, c, dir, pr);
-            }
-            weights[g] = w;
+        }
+        if (!want_pos || !live_block) continue;  // uniform across the block
+        // wave-aggregated append of the pixel's queued pairs
+        const int lane = tid & 63;
+        int x = nq;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            int y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        int wtot = __shfl(x, 63, 64), base = 0;
+        if (lane == 63 && wtot > 0) base = atomicAdd(&q_n, wtot);
+        base = __shfl(base, 63, 64) + x - nq;
+        if (nq > 0) {
+            q_tri[base] = qa.tri;
+            q_meta[base] = px | (da << 10) | ((qa.recv << 4 | qa.clamped << 3 | qa.from_a << 2 | qa.edge) << 11);
+        }
+        if (nq > 1) {
+            q_tri[base + 1] = qb.tri;
+            q_meta[base + 1] = px | (db << 10) | ((qb.recv << 4 | qb.clamped << 3 | qb.from_a << 2 | qb.edge) << 11);
         }
     }
     if (!want_pos || !live_block) return;  // uniform across the block
+    __syncthreads();
+    {
+        const int n = q_n;
+        const float4 *P = reinterpret_cast<const float4 *>(L.pos) + (L.pos_batched ? (long long)img * L.V : 0);
+        float *GP = L.grad_pos + (L.pos_batched ? (long long)img * L.V * 4 : 0);
+        for (int qi = tid; qi < n; qi += 256) {
+            const int m = q_meta[qi], px = m & (kAAPix - 1), dir = (m >> 10) & 1, bits = m >> 11;
+            PairRes pr;
+            pr.tri = q_tri[qi];
+            pr.recv = (bits >> 4) & 1;
+            pr.clamped = (bits >> 3) & 1;
+            pr.from_a = (bits >> 2) & 1;
+            pr.edge = bits & 3;
+            const int pr_r = r0 + (px >> 5), pr_c = c0 + (px & 31);
+            const long long g = (long long)img * npx + (long long)pr_r * L.W + pr_c;
+            pair_grad_add(A, GP, pair_pos_grad(L, I, P, g, pr_r, pr_c, dir, pr));
+        }
+    }
     __syncthreads();
     float *G = L.grad_pos + (L.pos_batched ? (long long)img * L.V * 4 : 0);
     for (int e = tid; e < kAAHash; e += 256) {
@@ -644,7 +718,7 @@ int launch_antialias_bwd(const AALaunch &L, hipStream_t st) {
     if (total == 0) return 0;
     // with the forward's state, projections, orientations, weights and decisions are in place
     if (!L.reuse) launch_orient(L, st);
-    int tiles_x = (L.W + 15) / 16, tiles_per_img = tiles_x * ((L.H + 15) / 16);
+    int tiles_x = (L.W + kAAT - 1) / kAAT, tiles_per_img = tiles_x * ((L.H + kAAT - 1) / kAAT);
     long long nb = (long long)tiles_per_img * L.n_img, nb_pad = (nb + 7) & ~7ll;
     dim3 block(256);
     hipLaunchKernelGGL(k_aa_pairs_bwd, dim3((unsigned)nb_pad), block, 0, st, L, L.weights,
