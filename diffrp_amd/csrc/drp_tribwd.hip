@@ -200,29 +200,76 @@ __global__ void __launch_bounds__(256) k_tile_reduce(TriBwdLaunch L) {
     }
 }
 
-// add the records of triangle t in the tiles of its box, in tile order
+// Add the records of triangle t over images [v0, v1): for each image where t
+// won a pixel, the tiles of its pixel box, in (image, tile) order.  The
+// searches (records sorted by triangle) run kSearch at a time in lockstep, so
+// their dependent loads overlap instead of queueing one after another.
+constexpr int kSearch = 4;
+
 template <int NV>
-__device__ __forceinline__ void gather_records(const TriBwdLaunch &L, int img, int t, int cc, int rr, float *acc) {
-    int c0 = cc & 0xffff, c1 = (unsigned)cc >> 16, r0 = rr & 0xffff, r1 = (unsigned)rr >> 16;
-    if (c0 > c1 || r0 > r1) return;
-    for (int ty = r0 / kBT; ty <= r1 / kBT; ++ty)
-        for (int tx = c0 / kBT; tx <= c1 / kBT; ++tx) {
-            int tile = img * L.tiles_per_img + ty * L.tiles_x + tx;
-            long long rbase = (long long)tile * kBTP;
-            int lo = 0, hi = L.rec_count[tile] - 1;
-            while (lo <= hi) {  // records sorted by triangle
-                int mid = (lo + hi) >> 1;
-                int tm = L.rec_t[rbase + mid];
-                if (tm < t) lo = mid + 1;
-                else if (tm > t) hi = mid - 1;
-                else {
-                    const float *q = L.rec_v + (rbase + mid) * NV;
+__device__ __forceinline__ void gather_records(const TriBwdLaunch &L, int t, int v0, int v1, float *acc) {
+    int img = v0 - 1, tx = 0, tx0 = 0, tx1 = -1, ty = 1, ty1 = 0;  // empty range: the first fill reads a box
+    for (;;) {
+        long long rb[kSearch];
+        int lo[kSearch], hi[kSearch], hit[kSearch];
+        int n = 0;
+        bool done = false;
 #pragma unroll
-                    for (int j = 0; j < NV; ++j) acc[j] += q[j];
+        for (int k = 0; k < kSearch; ++k) {
+            rb[k] = 0;
+            lo[k] = 0;
+            hi[k] = -1;
+            hit[k] = -1;
+            if (done) continue;
+            if (ty > ty1) {  // next image where t won a pixel
+                done = true;
+                while (++img < v1) {
+                    ItemBox b = L.boxes[(long long)img * L.F + t];
+                    int c0 = b.cc & 0xffff, c1 = (unsigned)b.cc >> 16, r0 = b.rr & 0xffff, r1 = (unsigned)b.rr >> 16;
+                    if (!b.hit || c0 > c1 || r0 > r1) continue;
+                    tx0 = tx = c0 / kBT;
+                    tx1 = c1 / kBT;
+                    ty = r0 / kBT;
+                    ty1 = r1 / kBT;
+                    done = false;
                     break;
                 }
+                if (done) continue;
+            }
+            const int tile = img * L.tiles_per_img + ty * L.tiles_x + tx;
+            rb[k] = (long long)tile * kBTP;
+            hi[k] = tile;  // replaced by the record count below
+            ++n;
+            if (++tx > tx1) {
+                tx = tx0;
+                ++ty;
             }
         }
+#pragma unroll
+        for (int k = 0; k < kSearch; ++k)
+            if (k < n) hi[k] = L.rec_count[hi[k]] - 1;
+        for (bool more = true; more;) {
+            more = false;
+#pragma unroll
+            for (int k = 0; k < kSearch; ++k) {
+                if (hit[k] >= 0 || lo[k] > hi[k]) continue;
+                const int mid = (lo[k] + hi[k]) >> 1;
+                const int tm = L.rec_t[rb[k] + mid];
+                if (tm < t) lo[k] = mid + 1;
+                else if (tm > t) hi[k] = mid - 1;
+                else hit[k] = mid;
+                more = more || (hit[k] < 0 && lo[k] <= hi[k]);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kSearch; ++k) {
+            if (hit[k] < 0) continue;
+            const float *q = L.rec_v + (rb[k] + hit[k]) * NV;
+#pragma unroll
+            for (int j = 0; j < NV; ++j) acc[j] += q[j];
+        }
+        if (n < kSearch) return;
+    }
 }
 
 template <int MODE, int C>
@@ -236,10 +283,7 @@ __global__ void __launch_bounds__(256) k_tile_gather(TriBwdLaunch L) {
     for (int j = 0; j < NV; ++j) acc[j] = 0.f;
     const int VB = L.views_per_block;
     const int v0 = grp * VB, v1 = min(v0 + VB, L.n_img);
-    for (int img = v0; img < v1; ++img) {
-        ItemBox b = L.boxes[(long long)img * L.F + t];
-        if (b.hit) gather_records<NV>(L, img, t, b.cc, b.rr, acc);
-    }
+    gather_records<NV>(L, t, v0, v1, acc);
     if (MODE == 0) {
         // corners go to their vertex's slots (CSR order), so the vertex gather reads contiguously
         float4 *P = reinterpret_cast<float4 *>(L.partial) + (long long)grp * 3 * L.F;
