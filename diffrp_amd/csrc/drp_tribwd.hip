@@ -30,16 +30,25 @@ namespace drp {
 constexpr int kBT = 16;            // backward tile edge
 constexpr int kBTP = kBT * kBT;    // pixels (and records) per tile
 constexpr int kRowStride = kBTP + 1;
-constexpr unsigned long long kEmptyT = (1ull << 40) - 1;  // triangle field of an uncovered pixel's key
+// Sort keys: triangle << 8 | pixel, 32-bit while triangle ids fit 24 bits,
+// else 64-bit; an uncovered pixel's triangle field is all ones.
+template <typename K> struct SortKey;
+template <> struct SortKey<unsigned> {
+    static constexpr unsigned kEmptyT = (1u << 24) - 1;
+};
+template <> struct SortKey<unsigned long long> {
+    static constexpr unsigned long long kEmptyT = (1ull << 40) - 1;
+};
 
 // Bitonic sort of the wave's 64 keys across lanes (ascending by lane), no barriers.
-__device__ __forceinline__ unsigned long long wave_sort64(unsigned long long k) {
+template <typename K>
+__device__ __forceinline__ K wave_sort64(K k) {
     const int lane = threadIdx.x & 63;
 #pragma unroll
     for (int size = 2; size <= 64; size <<= 1) {
 #pragma unroll
         for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            unsigned long long o = __shfl_xor(k, stride, 64);
+            K o = __shfl_xor(k, stride, 64);
             bool keep_min = ((lane & stride) == 0) == ((lane & size) == 0);
             k = keep_min ? (o < k ? o : k) : (o > k ? o : k);
         }
@@ -48,7 +57,8 @@ __device__ __forceinline__ unsigned long long wave_sort64(unsigned long long k) 
 }
 
 // number of entries of the sorted run a[0..64) below k (reads a[0..63] only)
-__device__ __forceinline__ int count_below64(const unsigned long long *a, unsigned long long k) {
+template <typename K>
+__device__ __forceinline__ int count_below64(const K *a, K k) {
     int base = 0;
 #pragma unroll
     for (int half = 32; half >= 1; half >>= 1)
@@ -78,11 +88,12 @@ __device__ __forceinline__ int block_excl_scan256(int v, int *wsum, int &total) 
 }
 
 // MODE 0: POS (NV = 12), MODE 1: ATTR (NV = 3 C)
-template <int MODE, int C, bool DB>
+template <int MODE, int C, bool DB, typename K>
 __global__ void __launch_bounds__(256) k_tile_reduce(TriBwdLaunch L) {
     constexpr int NV = MODE == 0 ? 12 : 3 * C;
-    __shared__ unsigned long long key[kBTP];
-    __shared__ unsigned long long run[kBTP];  // the waves' sorted runs
+    constexpr K kEmptyT = SortKey<K>::kEmptyT;
+    __shared__ K key[kBTP];
+    __shared__ K run[kBTP];  // the waves' sorted runs
     __shared__ float contrib[NV * kRowStride];
     __shared__ int seg_start[kBTP + 1];
     __shared__ int wsum[4];
@@ -165,19 +176,19 @@ __global__ void __launch_bounds__(256) k_tile_reduce(TriBwdLaunch L) {
     // each wave sorts its 64 across lanes, then every key's place is its lane
     // plus the keys below it in the other three runs
     {
-        unsigned long long k = wave_sort64(((t >= 0 ? (unsigned long long)t : kEmptyT) << 8) | (unsigned)tid);
+        K k = wave_sort64<K>(((t >= 0 ? (K)t : kEmptyT) << 8) | (K)tid);
         const int wave = tid >> 6;
         run[tid] = k;
         __syncthreads();
         int place = tid & 63;
 #pragma unroll
         for (int w = 0; w < 4; ++w)
-            if (w != wave) place += count_below64(run + 64 * w, k);
+            if (w != wave) place += count_below64<K>(run + 64 * w, k);
         key[place] = k;
         __syncthreads();
     }
     // segments: runs of one triangle in sorted order
-    unsigned long long sk = key[tid];
+    K sk = key[tid];
     bool valid = (sk >> 8) != kEmptyT;
     bool head = valid && (tid == 0 || (key[tid - 1] >> 8) != (sk >> 8));
     int nseg;
@@ -305,8 +316,14 @@ static void tile_grid(const TriBwdLaunch &L, dim3 &grid) {
 int launch_tri_bwd_pos(const TriBwdLaunch &L, hipStream_t st) {
     dim3 grid;
     tile_grid(L, grid);
-    if (L.grad_db) hipLaunchKernelGGL((k_tile_reduce<0, 1, true>), grid, dim3(kBTP), 0, st, L);
-    else hipLaunchKernelGGL((k_tile_reduce<0, 1, false>), grid, dim3(kBTP), 0, st, L);
+    const bool k32 = L.F < (int)SortKey<unsigned>::kEmptyT;
+    if (L.grad_db) {
+        if (k32) hipLaunchKernelGGL((k_tile_reduce<0, 1, true, unsigned>), grid, dim3(kBTP), 0, st, L);
+        else hipLaunchKernelGGL((k_tile_reduce<0, 1, true, unsigned long long>), grid, dim3(kBTP), 0, st, L);
+    } else {
+        if (k32) hipLaunchKernelGGL((k_tile_reduce<0, 1, false, unsigned>), grid, dim3(kBTP), 0, st, L);
+        else hipLaunchKernelGGL((k_tile_reduce<0, 1, false, unsigned long long>), grid, dim3(kBTP), 0, st, L);
+    }
     hipLaunchKernelGGL((k_tile_gather<0, 1>), dim3((L.F + 255) / 256, L.n_img), dim3(256), 0, st, L);
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
@@ -315,7 +332,10 @@ template <int C>
 static void launch_attr_c(const TriBwdLaunch &L, hipStream_t st) {
     dim3 grid;
     tile_grid(L, grid);
-    hipLaunchKernelGGL((k_tile_reduce<1, C, false>), grid, dim3(kBTP), 0, st, L);
+    if (L.F < (int)SortKey<unsigned>::kEmptyT)
+        hipLaunchKernelGGL((k_tile_reduce<1, C, false, unsigned>), grid, dim3(kBTP), 0, st, L);
+    else
+        hipLaunchKernelGGL((k_tile_reduce<1, C, false, unsigned long long>), grid, dim3(kBTP), 0, st, L);
     int groups = (L.n_img + L.views_per_block - 1) / L.views_per_block;
     hipLaunchKernelGGL((k_tile_gather<1, C>), dim3((L.F + 255) / 256, groups), dim3(256), 0, st, L);
 }
