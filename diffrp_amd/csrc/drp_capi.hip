@@ -461,7 +461,7 @@ int drp_edge_grad_bwd(void *stream, const float *grad_out, const float *rgb, int
     return 0;
 }
 
-size_t drp_incidence_workspace_size(int V) { return V > 0 ? incidence_ws_bytes(V) : 0; }
+size_t drp_incidence_workspace_size(int V, int F) { return V > 0 && F >= 0 ? incidence_ws_bytes(V, F) : 0; }
 
 int drp_incidence_build(void *stream, const int32_t *tri, int F, int V, int32_t *inc_off, int32_t *inc,
                         void *workspace, size_t workspace_bytes) {
@@ -469,8 +469,9 @@ int drp_incidence_build(void *stream, const int32_t *tri, int F, int V, int32_t 
     if (F > 0x2aaaaaaa) return fail(-1, "too many triangles");
     if (V == 0) return 0;
     if ((F > 0 && !tri) || !inc_off || (F > 0 && !inc)) return fail(-1, "null pointer argument");
-    if (!workspace || workspace_bytes < incidence_ws_bytes(V))
-        return fail(-1, "workspace too small: need %zu bytes, got %zu", incidence_ws_bytes(V), workspace_bytes);
+    const size_t need = incidence_ws_bytes(V, F);
+    if (!workspace || workspace_bytes < need)
+        return fail(-1, "workspace too small: need %zu bytes, got %zu", need, workspace_bytes);
     int rc = launch_incidence(tri, F, V, inc_off, inc, workspace, (hipStream_t)stream);
     if (rc) return fail(rc, "incidence launch failed: %s", hipGetErrorString(hipGetLastError()));
     return 0;

@@ -135,7 +135,7 @@ int launch_interpolate_bwd(const InterpLaunch &L, hipStream_t st);
 int launch_topology(const TopoLaunch &L, hipStream_t st);
 int launch_antialias(const AALaunch &L, hipStream_t st);
 int launch_antialias_bwd(const AALaunch &L, hipStream_t st);
-size_t incidence_ws_bytes(int V);
+size_t incidence_ws_bytes(int V, int F);
 int launch_incidence(const int *tri, int F, int V, int *inc_off, int *inc, void *ws, hipStream_t st);
 
 }  // namespace drp

@@ -2,7 +2,11 @@
 // vertex gathers of the triangle-centric backward passes.
 //   inc_off [V+1]: exclusive prefix of corner counts per vertex
 //   inc [3F]     : corner ids (3 t + k) grouped by vertex, ascending within a vertex
-// Corners whose index is outside [0, V) are left out.
+// Corners whose index is outside [0, V) are left out.  The order within a
+// vertex comes from one radix sort of (vertex << 32 | corner) keys, so it is
+// fixed whatever the vertex valence (a fan centre with 10^5 corners included).
+#include <rocprim/device/device_radix_sort.hpp>
+
 #include "drp_common.h"
 #include "drp_kernels.h"
 
@@ -82,55 +86,68 @@ __global__ void k_scan_add(int *__restrict__ out, long long n, const int *__rest
     }
 }
 
-__global__ void k_inc_fill(const int *__restrict__ tri, int F, int V, const int *__restrict__ off,
-                           int *__restrict__ cursor, int *__restrict__ inc) {
+__global__ void k_inc_keys(const int *__restrict__ tri, long long ne, int V, unsigned long long *__restrict__ keys) {
     long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= 3ll * F) return;
+    if (g >= ne) return;
     int v = tri[g];
-    if ((unsigned)v >= (unsigned)V) return;
-    int s = atomicAdd(&cursor[v], 1);
-    inc[off[v] + s] = (int)g;
+    unsigned long long vv = (unsigned)v < (unsigned)V ? (unsigned)v : (unsigned)V;  // V: left out, sorted last
+    keys[g] = (vv << 32) | (unsigned long long)g;
 }
 
-// order each vertex's corners ascending so gathers sum in a fixed order
-__global__ void k_inc_sort(const int *__restrict__ off, int V, int *__restrict__ inc) {
-    int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= V) return;
-    int a = off[v], b = off[v + 1];
-    for (int i = a + 1; i < b; ++i) {
-        int x = inc[i];
-        int j = i - 1;
-        while (j >= a && inc[j] > x) {
-            inc[j + 1] = inc[j];
-            --j;
-        }
-        inc[j + 1] = x;
-    }
+__global__ void k_inc_from_keys(const unsigned long long *__restrict__ keys, long long ne, int V,
+                                int *__restrict__ inc) {
+    long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ne) return;
+    unsigned long long k = keys[g];
+    if ((k >> 32) < (unsigned long long)V) inc[g] = (int)(k & 0xffffffffu);
 }
 
-size_t incidence_ws_bytes(int V) {
+static unsigned key_end_bit(int V) {
+    unsigned b = 0;
+    while ((1ull << b) <= (unsigned long long)V) ++b;  // bits of the sentinel V
+    return 32 + b;
+}
+
+static size_t sort_temp_bytes(int V, long long ne) {
+    size_t s = 0;
+    if (ne <= 0) return 0;
+    if (rocprim::radix_sort_keys(nullptr, s, (unsigned long long *)nullptr, (unsigned long long *)nullptr,
+                                 (size_t)ne, 0, key_end_bit(V), (hipStream_t)0) != hipSuccess)
+        return 0;
+    return s;
+}
+
+static size_t a256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+size_t incidence_ws_bytes(int V, int F) {
     long long nb = ((long long)V + 1023) / 1024 + 1;
-    return (((size_t)V * 4 + 255) & ~(size_t)255) * 2 + (((size_t)nb * 4 + 255) & ~(size_t)255);
+    long long ne = 3ll * F;
+    return a256((size_t)V * 4) + a256((size_t)nb * 4) + 2 * a256((size_t)ne * 8) + a256(sort_temp_bytes(V, ne));
 }
 
 int launch_incidence(const int *tri, int F, int V, int *inc_off, int *inc, void *ws, hipStream_t st) {
     if (V <= 0) return 0;
     char *b = (char *)ws;
-    size_t a = ((size_t)V * 4 + 255) & ~(size_t)255;
+    const long long ne = 3ll * F;
     int *cnt = (int *)b;
-    int *cursor = (int *)(b + a);
-    int *bsum = (int *)(b + 2 * a);
+    int *bsum = (int *)(b + a256((size_t)V * 4));
     long long nb = ((long long)V + 1023) / 1024;
-    hipLaunchKernelGGL(k_inc_zero, dim3((unsigned)((2 * (long long)a / 4 + 255) / 256)), dim3(256), 0, st, cnt,
-                       (long long)(2 * a / 4));
-    long long ne = 3ll * F;
+    char *kb = b + a256((size_t)V * 4) + a256((size_t)(nb + 1) * 4);
+    unsigned long long *keys = (unsigned long long *)kb, *sorted = (unsigned long long *)(kb + a256((size_t)ne * 8));
+    void *temp = kb + 2 * a256((size_t)ne * 8);
+    hipLaunchKernelGGL(k_inc_zero, dim3((unsigned)(((long long)V + 255) / 256)), dim3(256), 0, st, cnt, (long long)V);
     if (ne) hipLaunchKernelGGL(k_inc_count, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, st, tri, F, V, cnt);
     hipLaunchKernelGGL(k_scan_block, dim3((unsigned)nb), dim3(1024), 0, st, cnt, inc_off, (long long)V, bsum);
     hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, st, bsum, (int)nb);
     hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nb), dim3(1024), 0, st, inc_off, (long long)V, bsum, cnt);
-    if (ne) hipLaunchKernelGGL(k_inc_fill, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, st, tri, F, V, inc_off,
-                               cursor, inc);
-    hipLaunchKernelGGL(k_inc_sort, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, st, inc_off, V, inc);
+    if (ne) {
+        hipLaunchKernelGGL(k_inc_keys, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, st, tri, ne, V, keys);
+        size_t temp_bytes = sort_temp_bytes(V, ne);
+        if (rocprim::radix_sort_keys(temp, temp_bytes, keys, sorted, (size_t)ne, 0, key_end_bit(V), st) !=
+            hipSuccess)
+            return -2;
+        hipLaunchKernelGGL(k_inc_from_keys, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, st, sorted, ne, V, inc);
+    }
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

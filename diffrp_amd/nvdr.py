@@ -203,7 +203,7 @@ def incidence(tri: torch.Tensor, V: int):
     off = torch.empty(V + 1, dtype=torch.int32, device=tri.device)
     inc = torch.empty(max(3 * F, 1), dtype=torch.int32, device=tri.device)
     L = _lib.lib()
-    ws_bytes = L.drp_incidence_workspace_size(V)
+    ws_bytes = L.drp_incidence_workspace_size(V, F)
     ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=tri.device)
     check(L.drp_incidence_build(stream_of(tri), ptr(tri), F, V, ptr(off), ptr(inc), ptr(ws), ws_bytes), 'incidence')
     _inc_cache[key] = (weakref.ref(tri, lambda _r, k=key: _inc_cache.pop(k, None)), tri._version, (off, inc))

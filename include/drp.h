@@ -107,8 +107,10 @@ int drp_rasterize_bwd(void *stream, const float *pos, int n_img, int V, int pos_
 /* Vertex -> incident corner table of tri (CSR): inc_off [V+1], inc [3F] with
  * corner ids 3t+k grouped by vertex in ascending order.  Built once per mesh;
  * used by the deterministic backward gathers.  No reference counterpart: it is
- * the gather index of the backward passes that replace scatter atomics. */
-size_t drp_incidence_workspace_size(int V);
+ * the gather index of the backward passes that replace scatter atomics.  The
+ * workspace holds the corner sort keys (16 bytes per corner) and the sort's
+ * temporary storage. */
+size_t drp_incidence_workspace_size(int V, int F);
 int drp_incidence_build(void *stream, const int32_t *tri, int F, int V, int32_t *inc_off, int32_t *inc,
                         void *workspace, size_t workspace_bytes);
 

@@ -64,7 +64,7 @@ def test_workspace_queries(L):
         64, 512, 512, 81920, 8, 0, 1)
     assert L.drp_rasterize_bwd_workspace_size(64, 0, 512, 81920) == 0
     assert L.drp_topology_workspace_size(81920) > 0
-    assert L.drp_incidence_workspace_size(40962) > 0
+    assert L.drp_incidence_workspace_size(40962, 81920) > 0
     assert L.drp_antialias_workspace_size(1, 64, 64, 3, 100, 50) > 0
 
 

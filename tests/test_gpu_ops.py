@@ -269,3 +269,38 @@ def test_antialias_backward_with_and_without_forward_state(cuda):
     assert torch.equal(res[0][0], res[1][0])
     assert res[0][1].abs().max() > 0
     _close_rel(res[1][1].numpy(), res[0][1].numpy(), 1e-6, 'grad pos with forward state')
+
+
+def test_backward_with_triangle_ids_beyond_24_bits(cuda):
+    """2^24 culled triangles (over 1024 extra vertices behind the eye) in front of
+    the mesh: the real triangles' ids exceed 2^24 (the id codec's integer-bits
+    range) and the deterministic backward sorts 64-bit keys.  Forward and
+    gradients equal the unpadded mesh's bit for bit."""
+    pos, tri, _ = noisy_clip(4, 0.01, 96, views=2)
+    n_pad, n_dummy = 1 << 24, 1024
+    V = pos.shape[1]
+    dummy = torch.tensor([0.0, 0.0, 0.0, -1.0]).expand(2, n_dummy, 4)
+    pos_p = torch.cat([pos, dummy], 1).contiguous()
+    d = V + torch.arange(n_pad, dtype=torch.int32) % n_dummy
+    tri_p = torch.cat([torch.stack([d, d, d], 1), tri.int()]).contiguous()
+    g = torch.Generator().manual_seed(13)
+    attr = torch.rand(V, 4, generator=g)
+    go = torch.randn(2, 96, 96, 4, generator=g).cuda()
+
+    def run(p_all, t_all):
+        p = p_all.cuda().requires_grad_(True)
+        a = torch.cat([attr, torch.zeros(p_all.shape[1] - V, 4)]).cuda().requires_grad_(True)
+        rast, _ = _rast(p, t_all.cuda(), (96, 96))
+        out, _ = nvdr.interpolate(a, rast, t_all.cuda())
+        (out * go).sum().backward()
+        return rast.detach(), p.grad[:, :V].cpu(), a.grad[:V].cpu()
+
+    r0, gp0, ga0 = run(pos, tri)
+    r1, gp1, ga1 = run(pos_p, tri_p)
+    id0 = oracle.float_to_triidx(r0[..., 3].cpu().numpy())
+    id1 = oracle.float_to_triidx(r1[..., 3].cpu().numpy())
+    assert np.array_equal(id1, np.where(id0 > 0, id0 + n_pad, 0))
+    assert (id1 > (1 << 24)).any()
+    assert torch.equal(r0[..., :3], r1[..., :3])
+    assert torch.equal(gp0, gp1) and gp0.abs().max() > 0
+    assert torch.equal(ga0, ga1) and ga0.abs().max() > 0
