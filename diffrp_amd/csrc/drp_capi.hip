@@ -461,6 +461,17 @@ int drp_edge_grad_bwd(void *stream, const float *grad_out, const float *rgb, int
     return 0;
 }
 
+int drp_count_covered(void *stream, const float *rast, int n_img, int H, int W, int32_t *counts) {
+    if (int e = check_dims(n_img, H, W)) return e;
+    if (n_img > 65535) return fail(-1, "at most 65535 images per call (got %d)", n_img);
+    if (n_img == 0) return 0;
+    if (!rast || !counts) return fail(-1, "null pointer argument");
+    if (!aligned16(rast)) return fail(-1, "rast must be 16-byte aligned");
+    int rc = launch_count_covered(rast, n_img, (long long)H * W, counts, (hipStream_t)stream);
+    if (rc) return fail(rc, "count launch failed: %s", hipGetErrorString(hipGetLastError()));
+    return 0;
+}
+
 size_t drp_incidence_workspace_size(int V, int F) { return V > 0 && F >= 0 ? incidence_ws_bytes(V, F) : 0; }
 
 int drp_incidence_build(void *stream, const int32_t *tri, int F, int V, int32_t *inc_off, int32_t *inc,

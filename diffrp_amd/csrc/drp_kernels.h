@@ -136,6 +136,7 @@ int launch_topology(const TopoLaunch &L, hipStream_t st);
 int launch_antialias(const AALaunch &L, hipStream_t st);
 int launch_antialias_bwd(const AALaunch &L, hipStream_t st);
 size_t incidence_ws_bytes(int V, int F);
+int launch_count_covered(const float *rast, int n_img, long long npx, int *counts, hipStream_t st);
 int launch_incidence(const int *tri, int F, int V, int *inc_off, int *inc, void *ws, hipStream_t st);
 
 }  // namespace drp

@@ -434,6 +434,30 @@ __global__ void k_zero_i32(int *p, long long n) {
     if (i < n) p[i] = 0;
 }
 
+// covered pixels (id channel > 0) per image: grid (blocks per image, images),
+// 4 pixels per thread and pass, one atomic per wave
+__global__ void __launch_bounds__(256) k_count_covered(const float4 *__restrict__ rast, long long npx,
+                                                       int *__restrict__ counts) {
+    const int img = blockIdx.y;
+    const float4 *R = rast + (long long)img * npx;
+    int n = 0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < npx; i += (long long)gridDim.x * blockDim.x)
+        n += R[i].w > 0.f ? 1 : 0;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) n += __shfl_xor(n, o, 64);
+    if ((threadIdx.x & 63) == 0 && n) atomicAdd(&counts[img], n);
+}
+
+int launch_count_covered(const float *rast, int n_img, long long npx, int *counts, hipStream_t st) {
+    hipLaunchKernelGGL(k_zero_i32, dim3((unsigned)((n_img + 255) / 256)), dim3(256), 0, st, counts, (long long)n_img);
+    long long blocks = (npx + 1023) / 1024;
+    if (blocks > 256) blocks = 256;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_count_covered, dim3((unsigned)blocks, (unsigned)n_img), dim3(256), 0, st,
+                       reinterpret_cast<const float4 *>(rast), npx, counts);
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
 template <int TW, int TH, bool PEEL, bool DB>
 static void launch_fine_v(const RasterLaunch &L, hipStream_t st) {
     int pad = (L.n_tiles + 7) & ~7;

@@ -23,7 +23,7 @@ from typing import Callable, Dict, List, Optional, Sequence, Tuple, Union
 import torch
 import torch.nn.functional as F
 
-from . import nvdr
+from . import layered, nvdr
 from .camera import Camera
 from .geometry import compute_face_normals, compute_vertex_normals, make_face_soup
 
@@ -434,14 +434,8 @@ class SurfaceDeferredRenderSession:
         vao = self.vertex_array_object()
         h, w = self.camera.resolution()
         ranges = torch.tensor([[0, len(vao.tris)]], dtype=torch.int32)
-        layers = []
-        with nvdr.DepthPeeler(self.ctx, self.clip_space(), vao.tris, (h, w), ranges, grad_db=False) as dp:
-            for i in range(self.options.max_layers):
-                rast, _ = dp.rasterize_next_layer()
-                if i < self.options.max_layers - 1 and not bool((rast[..., 3] > 0).any()):
-                    break
-                layers.append(rast)
-        return layers
+        # the emptiness test runs on the device, read back one layer late
+        return layered.peel_layers(self.ctx, self.clip_space(), vao.tris, (h, w), self.options.max_layers, ranges)
 
     # -- materials --------------------------------------------------------------
     @_memo

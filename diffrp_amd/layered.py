@@ -13,12 +13,61 @@ runs under ``torch.utils.checkpoint``: autograd keeps only the layer's inputs
 backward pass, so the per-layer G-buffers (4 C bytes per pixel) and shaded
 colours are never held for the whole graph.
 """
-from typing import Callable, Tuple
+from typing import Callable, List, Optional, Tuple
 
 import torch
 import torch.utils.checkpoint as ckpt
 
-from . import nvdr
+from . import _lib, nvdr
+from ._lib import check, ptr, stream_of
+
+
+def count_covered(rast: torch.Tensor) -> torch.Tensor:
+    """Covered pixels per image of rast [n, H, W, 4] as a device int32 [n]
+    (no host synchronisation)."""
+    if rast.dim() != 4 or rast.shape[-1] != 4 or rast.dtype != torch.float32 or not rast.is_contiguous():
+        raise ValueError('rast must be a contiguous float32 [n, H, W, 4] tensor')
+    n, H, W = rast.shape[0], rast.shape[1], rast.shape[2]
+    counts = torch.empty(n, dtype=torch.int32, device=rast.device)
+    check(_lib.lib().drp_count_covered(stream_of(rast), ptr(rast), n, H, W, ptr(counts)), 'count_covered')
+    return counts
+
+
+def peel_layers(glctx: 'nvdr.RasterizeCudaContext', pos: torch.Tensor, tri: torch.Tensor,
+                resolution: Tuple[int, int], max_layers: int, ranges: Optional[torch.Tensor] = None,
+                grad_db: bool = False) -> List[torch.Tensor]:
+    """The layers of diffrp's peeling loop (``surface_deferred.py:258-266``):
+    layer i is kept unless it is empty and i < max_layers - 1; the loop stops at
+    the first such empty layer.  Emptiness is counted on the device
+    (``count_covered``) and copied to pinned host memory; the host reads layer
+    i's count only after layer i + 1 has been queued, so the GPU never waits for
+    the host (the one speculative layer behind an empty one is dropped)."""
+    layers: List[torch.Tensor] = []
+    pending = None  # (rast, pinned counts or None, event)
+    with nvdr.DepthPeeler(glctx, pos, tri, resolution, ranges, grad_db=grad_db) as dp:
+        for i in range(max_layers):
+            rast, _ = dp.rasterize_next_layer()
+            host, ev = None, None
+            if i < max_layers - 1:
+                cnt = count_covered(rast)
+                host = torch.empty(cnt.shape, dtype=cnt.dtype, pin_memory=True)
+                host.copy_(cnt, non_blocking=True)
+                ev = torch.cuda.Event()
+                ev.record()
+            if pending is not None:
+                prast, phost, pev = pending
+                pev.synchronize()
+                if int(phost.sum()) == 0:
+                    return layers
+                layers.append(prast)
+            pending = (rast, host, ev)
+    prast, phost, pev = pending
+    if phost is not None:
+        pev.synchronize()
+        if int(phost.sum()) == 0:
+            return layers
+    layers.append(prast)
+    return layers
 
 
 def _layer(ctx, pos, tri, attr, resolution, prev, shade):

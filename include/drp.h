@@ -104,6 +104,13 @@ int drp_rasterize_bwd(void *stream, const float *pos, int n_img, int V, int pos_
                       const float *grad_db, float *grad_pos, const int32_t *boxes, const int32_t *inc_off,
                       const int32_t *inc, void *workspace, size_t workspace_bytes);
 
+/* Covered pixels (id channel > 0) of each of n_img rast images into counts[n]
+ * (int32, device; no host synchronisation).  Replaces the host-side emptiness
+ * test of the depth-peeling loop, (rast.a <= 0).all() at diffrp/rendering/
+ * surface_deferred.py:263: the caller reads the counts one layer late, while
+ * the next layer is being rasterised. */
+int drp_count_covered(void *stream, const float *rast, int n_img, int H, int W, int32_t *counts);
+
 /* Vertex -> incident corner table of tri (CSR): inc_off [V+1], inc [3F] with
  * corner ids 3t+k grouped by vertex in ascending order.  Built once per mesh;
  * used by the deterministic backward gathers.  No reference counterpart: it is
