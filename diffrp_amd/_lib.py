@@ -39,6 +39,9 @@ _SIGS = {
     'drp_rasterize_batch_workspace_size': (c_size, [c_void_p]),
     'drp_rasterize_batch_fwd': (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                         c_void_p, c_void_p, c_size, c_void_p]),
+    'drp_stencil_partition_workspace_size': (c_size, [c_int, c_int, c_int]),
+    'drp_stencil_partition': (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_int, c_int, c_void_p, c_void_p,
+                                      c_void_p, c_void_p, c_size]),
     'drp_count_covered': (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p]),
     'drp_incidence_workspace_size': (c_size, [c_int, c_int]),
     'drp_incidence_build': (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_size]),

@@ -472,6 +472,26 @@ int drp_count_covered(void *stream, const float *rast, int n_img, int H, int W, 
     return 0;
 }
 
+size_t drp_stencil_partition_workspace_size(int H, int W, int n_owner) {
+    if (H <= 0 || W <= 0 || n_owner < 0) return 0;
+    return stencil_partition_ws_bytes((long long)H * W, n_owner);
+}
+
+int drp_stencil_partition(void *stream, const float *rast, int H, int W, const int32_t *stencils, int F, int n_owner,
+                          int32_t *order, int32_t *starts, int32_t *ends, void *workspace, size_t workspace_bytes) {
+    if (int e = check_dims(1, H, W)) return e;
+    if (F < 0 || n_owner < 0 || n_owner >= (1 << 30)) return fail(-1, "bad F or owner count");
+    if (!rast || !stencils || !order || !starts || !ends) return fail(-1, "null pointer argument");
+    if (!aligned16(rast)) return fail(-1, "rast must be 16-byte aligned");
+    const size_t need = stencil_partition_ws_bytes((long long)H * W, n_owner);
+    if (!workspace || workspace_bytes < need)
+        return fail(-1, "workspace too small: need %zu bytes, got %zu", need, workspace_bytes);
+    int rc = launch_stencil_partition(rast, (long long)H * W, stencils, F, n_owner, order, starts, ends, workspace,
+                                      (hipStream_t)stream);
+    if (rc) return fail(rc, "stencil partition launch failed: %s", hipGetErrorString(hipGetLastError()));
+    return 0;
+}
+
 size_t drp_incidence_workspace_size(int V, int F) { return V > 0 && F >= 0 ? incidence_ws_bytes(V, F) : 0; }
 
 int drp_incidence_build(void *stream, const int32_t *tri, int F, int V, int32_t *inc_off, int32_t *inc,

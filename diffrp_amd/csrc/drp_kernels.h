@@ -136,6 +136,9 @@ int launch_topology(const TopoLaunch &L, hipStream_t st);
 int launch_antialias(const AALaunch &L, hipStream_t st);
 int launch_antialias_bwd(const AALaunch &L, hipStream_t st);
 size_t incidence_ws_bytes(int V, int F);
+size_t stencil_partition_ws_bytes(long long npx, int n_owner);
+int launch_stencil_partition(const float *rast, long long npx, const int *stencils, int F, int n_owner, int *order,
+                             int *starts, int *ends, void *ws, hipStream_t st);
 int launch_count_covered(const float *rast, int n_img, long long npx, int *counts, hipStream_t st);
 int launch_incidence(const int *tri, int F, int V, int *inc_off, int *inc, void *ws, hipStream_t st);
 

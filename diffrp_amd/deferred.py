@@ -104,17 +104,67 @@ class FullScreenInterpolator:
         return nvdr.interpolate(values.contiguous(), self.vi_data, self.tris)[0]
 
 
-class MaskedSparseInterpolator:
-    """Only the pixels where ``mask`` holds, as a flat batch [P, C]."""
+_FRAG_ROW = 8192  # fragments per row of the 2-D image the gathered fragments are laid out as
 
-    def __init__(self, rast: torch.Tensor, tris: torch.Tensor, mask: torch.Tensor):
+
+class MaskedSparseInterpolator:
+    """Only the pixels where ``mask`` holds, as a flat batch [P, C]
+    (``interpolator.py:87-97``).  ``indices`` are nonzero()'s row-major index
+    tuple, ``vi_data`` the gathered fragments [P, 4]; ``interpolate`` runs the
+    HIP interpolation kernel over the fragments as one 1 x P image (the same
+    (a0 - a2) u + ((a1 - a2) v + a2) blend as diffrp's polyfill, without
+    materialising the [P, 3, C] corner rows)."""
+
+    def __init__(self, rast: torch.Tensor, tris: torch.Tensor, mask: Optional[torch.Tensor] = None,
+                 pixels: Optional[torch.Tensor] = None):
         self.tris = tris
-        self.indices = mask.nonzero(as_tuple=True)
-        self.vi_data = rast[self.indices]
-        self.tri_idx = tris[float_to_triidx(self.vi_data[..., 3]) - 1].long()
+        if pixels is None:
+            self.indices = mask.nonzero(as_tuple=True)
+            self.vi_data = rast[self.indices]
+        else:  # flat row-major pixel ids of a [1, H, W, 4] layer (from stencil_partition)
+            W = rast.shape[2]
+            p = pixels.long()
+            self.indices = (torch.zeros_like(p), p // W, p % W)
+            self.vi_data = rast.reshape(-1, 4).index_select(0, p)
+        self._tri_idx = None
+
+    @property
+    def tri_idx(self) -> torch.Tensor:
+        if self._tri_idx is None:
+            self._tri_idx = self.tris[float_to_triidx(self.vi_data[..., 3]) - 1].long()
+        return self._tri_idx
 
     def interpolate(self, values: torch.Tensor) -> torch.Tensor:
-        return polyfill_interpolate(values, self.vi_data, self.tri_idx)
+        P, C = self.vi_data.shape[0], values.shape[-1]
+        if P == 0:
+            return values.new_zeros(0, C)
+        frag = self.vi_data.contiguous()
+        rows = -(-P // _FRAG_ROW)
+        if rows > 1:  # an image row holds at most _FRAG_ROW fragments; the padding is empty (id 0)
+            frag = torch.cat([frag, frag.new_zeros(rows * _FRAG_ROW - P, 4)])
+        frag = frag.view(1, rows, -1, 4)
+        out, _ = nvdr.interpolate(values.float().contiguous(), frag, self.tris)
+        return out.view(-1, C)[:P]
+
+
+def stencil_partition(rast: torch.Tensor, stencils: torch.Tensor, n_owner: int):
+    """Pixels of one layer [1, H, W, 4] grouped by owner = stencils[id]
+    (``surface_deferred.py:283-284``): returns (order [H*W] int32 on the device,
+    [(start, end)] per owner 0..n_owner on the host), with one host read for
+    the whole layer.  Within an owner the pixels are in row-major order."""
+    from . import _lib
+    from ._lib import check, ptr, stream_of
+    H, W = rast.shape[1], rast.shape[2]
+    rast = rast.contiguous()
+    st = stencils.int().contiguous()
+    L = _lib.lib()
+    order = torch.empty(H * W, dtype=torch.int32, device=rast.device)
+    se = torch.empty(2, n_owner + 1, dtype=torch.int32, device=rast.device)
+    ws_bytes = L.drp_stencil_partition_workspace_size(H, W, n_owner)
+    ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=rast.device)
+    check(L.drp_stencil_partition(stream_of(rast), ptr(rast), H, W, ptr(st), st.numel() - 1, n_owner, ptr(order),
+                                  ptr(se[0]), ptr(se[1]), ptr(ws), ws_bytes), 'stencil partition')
+    return order, se.cpu().t().tolist()
 
 
 # ----------------------------------------------------------------------------
@@ -443,16 +493,18 @@ class SurfaceDeferredRenderSession:
         vao = self.vertex_array_object()
         V, P = self.camera_V(), self.camera_P()
         result = []
+        n_obj = len(self.scene.objects)
         for rast in self.rasterize():
             if self.options.interpolator_impl == 'stencil_masked':
-                owner = vao.stencils[float_to_triidx(rast[..., 3])]
+                order, runs = stencil_partition(rast, vao.stencils, n_obj)
             per_obj = []
             for i, obj in enumerate(self.scene.objects):
                 su = SurfaceUniform(obj.M, V, P)
                 if self.options.interpolator_impl == 'full_screen':
                     si = SurfaceInput(su, vao, FullScreenInterpolator(rast, vao.tris))
                 elif self.options.interpolator_impl == 'stencil_masked':
-                    si = SurfaceInput(su, vao, MaskedSparseInterpolator(rast, vao.tris, owner == i + 1))
+                    s, e = runs[i + 1]
+                    si = SurfaceInput(su, vao, MaskedSparseInterpolator(rast, vao.tris, pixels=order[s:e]))
                     if len(si.interpolator.indices[0]) == 0:
                         per_obj.append((si, SurfaceOutputStandard()))
                         continue

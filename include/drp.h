@@ -104,6 +104,19 @@ int drp_rasterize_bwd(void *stream, const float *pos, int n_img, int V, int pos_
                       const float *grad_db, float *grad_pos, const int32_t *boxes, const int32_t *inc_off,
                       const int32_t *inc, void *workspace, size_t workspace_bytes);
 
+/* Stencil partition of one layer [H,W,4] for the stencil-masked G-buffer path
+ * (replaces the per-object owner masks and mask.nonzero() calls of
+ * diffrp/rendering/surface_deferred.py:283-290 and interpolator.py:87-97):
+ * owner(p) = stencils[triangle id + 1 of p] (0 = background, stencils has F+1
+ * entries, values in [0, n_owner]); order [H*W] lists the pixels by owner and,
+ * within an owner, in row-major order (nonzero()'s order); owner k's pixels
+ * are order[starts[k] .. ends[k]) (starts/ends [n_owner+1], 0/0 when absent).
+ * One read of starts/ends per layer replaces one host synchronisation per
+ * object. */
+size_t drp_stencil_partition_workspace_size(int H, int W, int n_owner);
+int drp_stencil_partition(void *stream, const float *rast, int H, int W, const int32_t *stencils, int F, int n_owner,
+                          int32_t *order, int32_t *starts, int32_t *ends, void *workspace, size_t workspace_bytes);
+
 /* Covered pixels (id channel > 0) of each of n_img rast images into counts[n]
  * (int32, device; no host synchronisation).  Replaces the host-side emptiness
  * test of the depth-peeling loop, (rast.a <= 0).all() at diffrp/rendering/
