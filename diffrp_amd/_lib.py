@@ -39,6 +39,13 @@ _SIGS = {
     'drp_rasterize_batch_workspace_size': (c_size, [c_void_p]),
     'drp_rasterize_batch_fwd': (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                         c_void_p, c_void_p, c_size, c_void_p]),
+    'drp_sample2d_fwd': (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p, c_ll, c_int, c_int, c_void_p]),
+    'drp_sample2d_bwd': (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p, c_ll, c_int, c_int, c_void_p,
+                                 c_void_p, c_void_p]),
+    'drp_sample3d_fwd': (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_ll, c_int, c_int,
+                                 c_void_p]),
+    'drp_sample3d_bwd': (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_ll, c_int, c_int,
+                                 c_void_p, c_void_p, c_void_p]),
     'drp_stencil_partition_workspace_size': (c_size, [c_int, c_int, c_int]),
     'drp_stencil_partition': (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_int, c_int, c_void_p, c_void_p,
                                       c_void_p, c_void_p, c_size]),

@@ -472,6 +472,58 @@ int drp_count_covered(void *stream, const float *rast, int n_img, int H, int W, 
     return 0;
 }
 
+static int tex_check(int D, int H, int W, int C, long long n, int wrap, int mode, bool cyclic_ok) {
+    if (D < 1 || H < 1 || W < 1 || C < 1) return fail(-1, "texture dimensions must be >= 1");
+    if ((long long)D * H * W * C >= (1ll << 40)) return fail(-1, "texture too large");
+    if (n < 0) return fail(-1, "negative sample count");
+    if (wrap < 0 || wrap > 3 || (wrap == 2 && !cyclic_ok))
+        return fail(-1, "wrap must be border (0), reflection (1), %slatlong (3) (got %d)", cyclic_ok ? "cyclic (2), " : "",
+                    wrap);
+    if (mode != 0 && mode != 1) return fail(-1, "mode must be bilinear (0) or nearest (1) (got %d)", mode);
+    return 0;
+}
+
+int drp_sample2d_fwd(void *stream, const float *tex, int H, int W, int C, const float *uv, long long n, int wrap,
+                     int mode, float *out) {
+    if (int e = tex_check(1, H, W, C, n, wrap, mode, true)) return e;
+    if (n == 0) return 0;
+    if (!tex || !uv || !out) return fail(-1, "null pointer argument");
+    int rc = launch_sample2d(tex, H, W, C, uv, n, wrap, mode, out, (hipStream_t)stream);
+    if (rc) return fail(rc, "sample2d launch failed: %s", hipGetErrorString(hipGetLastError()));
+    return 0;
+}
+
+int drp_sample2d_bwd(void *stream, const float *tex, int H, int W, int C, const float *uv, long long n, int wrap,
+                     int mode, const float *grad_out, float *grad_tex, float *grad_uv) {
+    if (int e = tex_check(1, H, W, C, n, wrap, mode, true)) return e;
+    if (n == 0) return 0;
+    if (!tex || !uv || !grad_out) return fail(-1, "null pointer argument");
+    int rc = launch_sample2d_bwd(tex, H, W, C, uv, n, wrap, mode, grad_out, grad_tex, grad_uv, (hipStream_t)stream);
+    if (rc) return fail(rc, "sample2d backward launch failed: %s", hipGetErrorString(hipGetLastError()));
+    return 0;
+}
+
+int drp_sample3d_fwd(void *stream, const float *tex, int D, int H, int W, int C, const float *uvw, long long n,
+                     int wrap, int mode, float *out) {
+    if (int e = tex_check(D, H, W, C, n, wrap, mode, false)) return e;
+    if (n == 0) return 0;
+    if (!tex || !uvw || !out) return fail(-1, "null pointer argument");
+    int rc = launch_sample3d(tex, D, H, W, C, uvw, n, wrap, mode, out, (hipStream_t)stream);
+    if (rc) return fail(rc, "sample3d launch failed: %s", hipGetErrorString(hipGetLastError()));
+    return 0;
+}
+
+int drp_sample3d_bwd(void *stream, const float *tex, int D, int H, int W, int C, const float *uvw, long long n,
+                     int wrap, int mode, const float *grad_out, float *grad_tex, float *grad_uvw) {
+    if (int e = tex_check(D, H, W, C, n, wrap, mode, false)) return e;
+    if (n == 0) return 0;
+    if (!tex || !uvw || !grad_out) return fail(-1, "null pointer argument");
+    int rc = launch_sample3d_bwd(tex, D, H, W, C, uvw, n, wrap, mode, grad_out, grad_tex, grad_uvw,
+                                 (hipStream_t)stream);
+    if (rc) return fail(rc, "sample3d backward launch failed: %s", hipGetErrorString(hipGetLastError()));
+    return 0;
+}
+
 size_t drp_stencil_partition_workspace_size(int H, int W, int n_owner) {
     if (H <= 0 || W <= 0 || n_owner < 0) return 0;
     return stencil_partition_ws_bytes((long long)H * W, n_owner);

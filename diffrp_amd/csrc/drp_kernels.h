@@ -137,6 +137,14 @@ int launch_antialias(const AALaunch &L, hipStream_t st);
 int launch_antialias_bwd(const AALaunch &L, hipStream_t st);
 size_t incidence_ws_bytes(int V, int F);
 size_t stencil_partition_ws_bytes(long long npx, int n_owner);
+int launch_sample2d(const float *tex, int H, int W, int C, const float *uv, long long n, int wrap, int mode,
+                    float *out, hipStream_t st);
+int launch_sample2d_bwd(const float *tex, int H, int W, int C, const float *uv, long long n, int wrap, int mode,
+                        const float *gout, float *gtex, float *guv, hipStream_t st);
+int launch_sample3d(const float *tex, int D, int H, int W, int C, const float *uvw, long long n, int wrap, int mode,
+                    float *out, hipStream_t st);
+int launch_sample3d_bwd(const float *tex, int D, int H, int W, int C, const float *uvw, long long n, int wrap,
+                        int mode, const float *gout, float *gtex, float *guvw, hipStream_t st);
 int launch_stencil_partition(const float *rast, long long npx, const int *stencils, int F, int n_owner, int *order,
                              int *starts, int *ends, void *ws, hipStream_t st);
 int launch_count_covered(const float *rast, int n_img, long long npx, int *counts, hipStream_t st);

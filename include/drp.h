@@ -104,6 +104,24 @@ int drp_rasterize_bwd(void *stream, const float *pos, int n_img, int V, int pos_
                       const float *grad_db, float *grad_pos, const int32_t *boxes, const int32_t *inc_off,
                       const int32_t *inc, void *workspace, size_t workspace_bytes);
 
+/* ---- texture sampling ------------------------------------------------------
+ * Replace diffrp's sample2d / sample3d (diffrp/utils/shader_ops.py:198-310,
+ * torch grid_sample underneath; called by GLTFSampler, gltf_material.py:15-22,
+ * and image-based lighting).  Texture channels-last: [H,W,C] / [D,H,W,C];
+ * coordinates uv [n,2] / uvw [n,3] with (0,0) the bottom-left corner (last
+ * row).  wrap: 0 border, 1 reflection, 2 cyclic (2-D only), 3 latlong
+ * (reflection, align_corners); mode: 0 bilinear, 1 nearest (no coordinate
+ * gradient).  out [n,C].  Backward: grad_tex accumulated (float atomics, may
+ * be NULL), grad_uv / grad_uvw written (may be NULL). */
+int drp_sample2d_fwd(void *stream, const float *tex, int H, int W, int C, const float *uv, long long n, int wrap,
+                     int mode, float *out);
+int drp_sample2d_bwd(void *stream, const float *tex, int H, int W, int C, const float *uv, long long n, int wrap,
+                     int mode, const float *grad_out, float *grad_tex, float *grad_uv);
+int drp_sample3d_fwd(void *stream, const float *tex, int D, int H, int W, int C, const float *uvw, long long n,
+                     int wrap, int mode, float *out);
+int drp_sample3d_bwd(void *stream, const float *tex, int D, int H, int W, int C, const float *uvw, long long n,
+                     int wrap, int mode, const float *grad_out, float *grad_tex, float *grad_uvw);
+
 /* Stencil partition of one layer [H,W,4] for the stencil-masked G-buffer path
  * (replaces the per-object owner masks and mask.nonzero() calls of
  * diffrp/rendering/surface_deferred.py:283-290 and interpolator.py:87-97):

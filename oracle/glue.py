@@ -165,6 +165,40 @@ def edge_gradient_backward(grad_output, frag_xy, flat_tris_xy, rast, rgb):
     return torch.cat([gx * (W / 2), gy * (H / 2)], -1), grad_output.clone()
 
 
+def sample2d(texture2d, texcoords, wrap: str = "border", mode: str = "bilinear"):
+    """diffrp/utils/shader_ops.py:197-252: grid_sample on the (C, H, W) view of an
+    (H, W, C) texture at x = 2u - 1, y = -(2v - 1); 'latlong' = reflection with
+    align_corners; 'cyclic' = the 2 x 2 tiled texture at (uv mod 1) / 2 moved
+    into [1/4, 3/4), reflection padding."""
+    shape = texcoords.shape
+    tc = texcoords.reshape(1, 1, -1, shape[-1])
+    align = False
+    if wrap == "latlong":
+        align, wrap = True, "reflection"
+    if wrap == "cyclic":
+        texture2d = texture2d.tile(2, 2, 1)
+        tc = (tc % 1.0) * 0.5
+        tc = torch.where(tc >= 0.25, tc, tc + 0.5)
+        wrap = "reflection"
+    tc = (tc * 2 - 1) * torch.tensor([1.0, -1.0], dtype=tc.dtype)
+    out = F.grid_sample(texture2d[None].permute(0, 3, 1, 2), tc, padding_mode=wrap, mode=mode, align_corners=align)
+    return out.reshape(texture2d.shape[-1], -1).T.reshape(*shape[:-1], texture2d.shape[-1])
+
+
+def sample3d(texture3d, texcoords, wrap: str = "border", mode: str = "bilinear"):
+    """diffrp/utils/shader_ops.py:255-310: grid_sample on the (C, D, H, W) view
+    of a (D, H, W, C) texture at (2u - 1, -(2v - 1), 2w - 1)."""
+    shape = texcoords.shape
+    tc = texcoords.reshape(1, 1, 1, -1, shape[-1])
+    tc = (tc * 2 - 1) * torch.tensor([1.0, -1.0, 1.0], dtype=tc.dtype)
+    align = False
+    if wrap == "latlong":
+        align, wrap = True, "reflection"
+    out = F.grid_sample(texture3d[None].permute(0, 4, 1, 2, 3), tc, padding_mode=wrap, mode=mode,
+                        align_corners=align)
+    return out.reshape(texture3d.shape[-1], -1).T.reshape(*shape[:-1], texture3d.shape[-1])
+
+
 def to_uint8(x: torch.Tensor) -> np.ndarray:
     """diffrp/utils/exchange.py:7-18: (saturate(x) * 255).byte()  (truncation)."""
     return (torch.clamp(x.float(), 0.0, 1.0) * 255).byte().numpy()
