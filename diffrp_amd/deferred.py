@@ -7,8 +7,8 @@ Mirrors the surface of diffrp's ``SurfaceDeferredRenderSession`` and its inputs
 ``scene/objects.py:11-98``, ``scene/scene.py:10-31``) so that scenes, materials
 and G-buffer queries written against diffrp run unchanged on top of
 ``diffrp_amd.nvdr``.  Shading stays PyTorch (as in diffrp); rasterisation,
-interpolation and antialiasing are the HIP kernels.  Image-based lighting and
-texture sampling are not part of this module.
+interpolation and antialiasing are the HIP kernels.  Texture sampling for
+materials is ``diffrp_amd.texture``; image-based lighting is not built.
 
 Conventions kept from diffrp: G-buffers are produced bottom row first and the
 composed images are flipped to top row first; background takes the G-buffer
