@@ -77,6 +77,15 @@ def polyfill_interpolate(values: torch.Tensor, frag: torch.Tensor, tri_idx: torc
     return out
 
 
+def linear_to_srgb(rgb: torch.Tensor) -> torch.Tensor:
+    """Linear -> sRGB (``diffrp/utils/colors.py:33-42``): 12.92 x below 0.0031308,
+    else 1.055 x^(1/2.4) - 0.055 (as torch.add(-0.055, ., alpha=1.055));
+    clamped at 1e-5 from below when differentiated."""
+    if rgb.requires_grad:
+        rgb = torch.clamp_min(rgb, 1e-5)
+    return torch.where(rgb < 0.0031308, 12.92 * rgb, torch.add(-0.055, rgb ** (1 / 2.4), alpha=1.055))
+
+
 def _memo(fn):
     """Per-instance memo keyed by the method's qualified name (one value per session)."""
     key = fn.__qualname__
@@ -646,6 +655,12 @@ class SurfaceDeferredRenderSession:
     @_memo
     def albedo(self):
         return self.compose_layers(self.albedo_layered())
+
+    @_memo
+    def albedo_srgb(self):
+        """Albedo with RGB in sRGB (``surface_deferred.py:457-467``)."""
+        a = self.albedo()
+        return torch.cat([linear_to_srgb(a[..., :3]), a[..., 3:]], -1)
 
     @_memo
     def emission(self):
