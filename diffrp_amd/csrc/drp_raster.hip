@@ -159,15 +159,6 @@ __device__ __forceinline__ void row_span(const FineRec &R, int y, int &xl, int &
     }
 }
 
-// last k in [0, N) with a[k] <= i (a non-decreasing, a[0] = 0 <= i)
-template <int N>
-__device__ __forceinline__ int search_last_le(const int *a, int i) {
-    int k = 0;
-#pragma unroll
-    for (int half = N / 2; half >= 1; half >>= 1)
-        if (a[k + half] <= i) k += half;
-    return k;
-}
 
 template <int NT>
 __device__ __forceinline__ int block_excl_scan(int v, int *wsum, int &total) {
@@ -206,7 +197,7 @@ __global__ void __launch_bounds__(TW *TH, kFineWavesPerEU) k_fine(Batch b, const
     constexpr int NT = TW * TH;
     static_assert(NT % 64 == 0, "tile must be a whole number of waves");
     __shared__ FineRec rec[NT];
-    __shared__ int cstart[NT + 1];
+    __shared__ unsigned short rowmap[NT];  // row item of a sub-round -> record << 5 | row within the record
     constexpr int kQueue = NT * kQueuePerLane;
     __shared__ int queue[kQueue];  // covered pixels: record | x << 8 | y << 16 (tile-local)
     __shared__ unsigned long long zbuf[NT];
@@ -289,23 +280,25 @@ __global__ void __launch_bounds__(TW *TH, kFineWavesPerEU) k_fine(Batch b, const
             }
         }
         int nrows;
-        int ex = block_excl_scan<NT>(count, wsum, nrows);
-        cstart[tid] = ex;
-        if (tid == NT - 1) cstart[NT] = ex + count;
-        __syncthreads();
+        const int ex = block_excl_scan<NT>(count, wsum, nrows);  // this lane's record owns rows [ex, ex + count)
         // Row items (record, tile row) in sub-rounds of NT: each lane finds the
         // exact covered span of its row; the covered pixels are queued (block
         // prefix sum, at most kQueue per pass) and evaluated by all lanes, so
         // the barycentric and depth work runs without divergence.  The depth
         // test is a minimum, so the queue order does not matter.
         for (int sb = 0; sb < nrows; sb += NT, par ^= 1) {
+            // each record marks its rows of this sub-round's window (at most 32 rows per record)
+            for (int rr = max(ex, sb), re = min(ex + count, sb + NT); rr < re; ++rr)
+                rowmap[rr - sb] = (unsigned short)((tid << 5) | (rr - ex));
+            __syncthreads();
             const int ri = sb + tid;
             int len = 0, k = 0, y = 0, xl = 0;
             if (ri < nrows) {
-                k = search_last_le<NT>(cstart, ri);  // empty records share their successor's start
+                const int m = rowmap[tid];
+                k = m >> 5;
                 const FineRec &R = rec[k];
                 const int box = R.box;
-                y = rec_y0(box) + (ri - cstart[k]);
+                y = rec_y0(box) + (m & 31);
                 xl = rec_x0(box);
                 int xr = rec_x1(box);
                 if (rec_cls(box) == kDirect) row_span(R, y, xl, xr);
