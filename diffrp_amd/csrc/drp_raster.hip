@@ -23,7 +23,7 @@ namespace drp {
 
 constexpr int kMaxBinTiles = 16;
 constexpr int kQueuePerLane = 2;  // fine raster: covered-pixel queue entries per lane and pass
-constexpr int kFineWavesPerEU = 5;  // occupancy target of k_fine (VGPR budget 512 / 5)
+constexpr int kFineWavesPerEU = 6;  // occupancy target of k_fine (VGPR budget 512 / 6)
 
 __device__ __forceinline__ int2 pack_box(int c0, int c1, int r0, int r1) {
     return make_int2(c0 | (c1 << 16), r0 | (r1 << 16));
@@ -117,33 +117,36 @@ __global__ void __launch_bounds__(256) k_setup_bin(Batch b, const float *__restr
 // fine raster
 // ----------------------------------------------------------------------------
 struct FineRec {
-    long long E0[3];  // edge functions (with top-left bias) at the tile's first pixel centre
-    int A[3], B[3];   // per-sub-pixel increments in x and y
+    int X[3], Y[3];   // snapped vertices minus the tile's first pixel centre (sub-pixels; direct class)
     TriVerts v;
     int t;
-    int box;          // tile-local box columns x0, x1 and first row y0 (6 bits each), class << 18
+    int box;          // tile-local box columns x0, x1 and first row y0 (6 bits each), class << 18, top-left flags << 20
 };
 __device__ __forceinline__ int rec_x0(int box) { return box & 63; }
 __device__ __forceinline__ int rec_y0(int box) { return (box >> 6) & 63; }
 __device__ __forceinline__ int rec_x1(int box) { return (box >> 12) & 63; }
-__device__ __forceinline__ int rec_cls(int box) { return box >> 18; }
+__device__ __forceinline__ int rec_cls(int box) { return (box >> 18) & 3; }
 
 // Covered columns of tile row y of a direct-class record, intersected with
-// [xl, xr]: every edge needs E_e(x) = C_e + a_e x >= 0 (C_e = E0_e + 16 B_e y,
-// a_e = 16 A_e, 64-bit).  The crossing -C_e / a_e is estimated in floating
+// [xl, xr]: every edge e -> j needs E_e(x) = C_e + a_e x >= 0 with
+// C_e = dx (16 y - Y_e) + dy X_e - [not top-left], a_e = -16 dy (dx, dy the
+// edge vector, vertices relative to the tile's first pixel centre; 64-bit).  The crossing -C_e / a_e is estimated in floating
 // point (clamped to [xl - 1, xr + 1], where its error is far below one
 // column) and settled by one exact 64-bit step either way, so the span is
 // exactly the set of pixels whose three edge functions are non-negative.
-__device__ __forceinline__ void row_span(const FineRec &R, int y, int &xl, int &xr) {
+__device__ __forceinline__ void row_span(const FineRec &R, int box, int y, int &xl, int &xr) {
 #pragma unroll
     for (int e = 0; e < 3; ++e) {
-        const long long a = (long long)R.A[e] * kSubpix;
-        const long long cc = R.E0[e] + (long long)R.B[e] * (kSubpix * y);
+        const int j = e == 2 ? 0 : e + 1;
+        const int dx = R.X[j] - R.X[e], dy = R.Y[j] - R.Y[e];
+        const long long a = -(long long)dy * kSubpix;
+        const long long cc = (long long)dx * (kSubpix * y - R.Y[e]) + (long long)dy * R.X[e] -
+                             (((box >> (20 + e)) & 1) ? 0 : 1);
         if (a == 0) {
             if (cc < 0) xr = xl - 1;
             continue;
         }
-        double q = -(double)cc * (double)__builtin_amdgcn_rcpf((float)R.A[e] * (float)kSubpix);
+        double q = -(double)cc * (double)__builtin_amdgcn_rcpf((float)(-dy) * (float)kSubpix);
         q = fmin(fmax(q, (double)(xl - 1)), (double)(xr + 1));
         if (a > 0) {  // covered for x >= crossing
             int x0 = (int)ceil(q);
@@ -260,21 +263,19 @@ __global__ void __launch_bounds__(TW *TH, kFineWavesPerEU) k_fine(Batch b, const
                     if (x0 <= x1 && y0 <= y1) {
                         FineRec &R = rec[tid];
                         R.t = t;
-                        R.box = x0 | (y0 << 6) | (x1 << 12) | (s.cls << 18);
+                        int box = x0 | (y0 << 6) | (x1 << 12) | (s.cls << 18);
                         R.v = v;
                         count = y1 - y0 + 1;  // row items
                         if (s.cls == kDirect) {
-                            long long Px = (long long)kSubpix * tc0 + kHalfSub, Py = (long long)kSubpix * tr0 + kHalfSub;
+                            const int Px = kSubpix * tc0 + kHalfSub, Py = kSubpix * tr0 + kHalfSub;
 #pragma unroll
                             for (int e = 0; e < 3; ++e) {
-                                int j = e == 2 ? 0 : e + 1;
-                                int dx = s.X[j] - s.X[e], dy = s.Y[j] - s.Y[e];
-                                R.E0[e] = (long long)dx * (Py - s.Y[e]) - (long long)dy * (Px - s.X[e]) -
-                                          (s.tl[e] ? 0 : 1);
-                                R.A[e] = -dy;
-                                R.B[e] = dx;
+                                R.X[e] = s.X[e] - Px;
+                                R.Y[e] = s.Y[e] - Py;
+                                box |= (s.tl[e] ? 1 : 0) << (20 + e);
                             }
                         }
+                        R.box = box;
                     }
                 }
             }
@@ -301,7 +302,7 @@ __global__ void __launch_bounds__(TW *TH, kFineWavesPerEU) k_fine(Batch b, const
                 y = rec_y0(box) + (m & 31);
                 xl = rec_x0(box);
                 int xr = rec_x1(box);
-                if (rec_cls(box) == kDirect) row_span(R, y, xl, xr);
+                if (rec_cls(box) == kDirect) row_span(R, box, y, xl, xr);
                 len = max(xr - xl + 1, 0);
             }
             int npix;
