@@ -93,9 +93,11 @@ __global__ void __launch_bounds__(256) k_tile_reduce(TriBwdLaunch L) {
     constexpr int NV = MODE == 0 ? 12 : 3 * C;
     constexpr K kEmptyT = SortKey<K>::kEmptyT;
     __shared__ K key[kBTP];
-    __shared__ K run[kBTP];  // the waves' sorted runs
+    __shared__ union {
+        K run[kBTP];             // the waves' sorted runs (merge)
+        int seg_start[kBTP + 1]; // segment starts (after the merge)
+    } U;
     __shared__ float contrib[NV * kRowStride];
-    __shared__ int seg_start[kBTP + 1];
     __shared__ int wsum[4];
     const int tid = threadIdx.x;
     const int tile = xcd_remap(blockIdx.x, L.n_tiles_pad);
@@ -178,12 +180,12 @@ __global__ void __launch_bounds__(256) k_tile_reduce(TriBwdLaunch L) {
     {
         K k = wave_sort64<K>(((t >= 0 ? (K)t : kEmptyT) << 8) | (K)tid);
         const int wave = tid >> 6;
-        run[tid] = k;
+        U.run[tid] = k;
         __syncthreads();
         int place = tid & 63;
 #pragma unroll
         for (int w = 0; w < 4; ++w)
-            if (w != wave) place += count_below64<K>(run + 64 * w, k);
+            if (w != wave) place += count_below64<K>(U.run + 64 * w, k);
         key[place] = k;
         __syncthreads();
     }
@@ -194,8 +196,8 @@ __global__ void __launch_bounds__(256) k_tile_reduce(TriBwdLaunch L) {
     int nseg;
     int seg = block_excl_scan256(head ? 1 : 0, wsum, nseg);
     int nvalid = __syncthreads_count(valid);
-    if (head) seg_start[seg] = tid;
-    if (tid == 0) seg_start[nseg] = nvalid;
+    if (head) U.seg_start[seg] = tid;
+    if (tid == 0) U.seg_start[nseg] = nvalid;
     __syncthreads();
     const long long rbase = (long long)tile * kBTP;
     if (head) L.rec_t[rbase + seg] = (int)(sk >> 8);
@@ -203,7 +205,7 @@ __global__ void __launch_bounds__(256) k_tile_reduce(TriBwdLaunch L) {
     // one thread per (segment, component): consecutive threads -> consecutive record floats
     for (int w = tid; w < nseg * NV; w += kBTP) {
         int sg = w / NV, j = w - sg * NV;
-        int p0 = seg_start[sg], p1 = seg_start[sg + 1];
+        int p0 = U.seg_start[sg], p1 = U.seg_start[sg + 1];
         const float *row = contrib + j * kRowStride;
         float s = row[(int)(key[p0] & 255)];
         for (int p = p0 + 1; p < p1; ++p) s += row[(int)(key[p] & 255)];
