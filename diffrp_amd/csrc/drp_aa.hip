@@ -346,12 +346,16 @@ __device__ __forceinline__ void ld(const float *p, float (&x)[CH]) {
         for (int i = 0; i < CH; ++i) x[i] = p[i];
     }
 }
+// streaming (non-temporal) stores: the output is not read again by this pass
 template <int CH>
 __device__ __forceinline__ void st(float *p, const float (&x)[CH]) {
-    if (CH == 4) *reinterpret_cast<float4 *>(p) = make_float4(x[0], x[1], x[2], x[3]);
-    else
+    if (CH == 4) {
+        typedef float v4 __attribute__((ext_vector_type(4)));
+        v4 v = {x[0], x[1], x[2], x[3]};
+        __builtin_nontemporal_store(v, reinterpret_cast<v4 *>(p));
+    } else
 #pragma unroll
-        for (int i = 0; i < CH; ++i) p[i] = x[i];
+        for (int i = 0; i < CH; ++i) __builtin_nontemporal_store(x[i], p + i);
 }
 
 // one thread per (pixel, channel chunk): consecutive threads touch consecutive bytes

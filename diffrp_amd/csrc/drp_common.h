@@ -275,6 +275,22 @@ __device__ __forceinline__ float4 bary_db(const TriVerts &v, const PixEval &p, f
 // (gfx950 dispatches workgroups round-robin over its 8 XCDs).
 // floor(a / b) for 0 <= a < b * 65536, 1 <= b <= 65536: float estimate, one fix-up step
 // (the estimate's error is below 1/64, so it is off by at most one).
+// Bitonic sort of the wave's 64 keys across lanes (ascending by lane), no barriers.
+template <typename K>
+__device__ __forceinline__ K wave_sort64(K k) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int size = 2; size <= 64; size <<= 1) {
+#pragma unroll
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            K o = __shfl_xor(k, stride, 64);
+            bool keep_min = ((lane & stride) == 0) == ((lane & size) == 0);
+            k = keep_min ? (o < k ? o : k) : (o > k ? o : k);
+        }
+    }
+    return k;
+}
+
 __device__ __forceinline__ int div_box(int a, int b) {
     int q = (int)((float)a * __builtin_amdgcn_rcpf((float)b));
     int r = a - q * b;

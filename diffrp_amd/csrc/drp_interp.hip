@@ -60,7 +60,9 @@ __global__ void __launch_bounds__(256) k_interp_fwd(const float *__restrict__ at
             y.y = fma_(x0.y - x2.y, p.u, fma_(x1.y - x2.y, p.v, x2.y));
             y.z = fma_(x0.z - x2.z, p.u, fma_(x1.z - x2.z, p.v, x2.z));
             y.w = fma_(x0.w - x2.w, p.u, fma_(x1.w - x2.w, p.v, x2.w));
-            *reinterpret_cast<float4 *>(o + ch) = y;
+            typedef float v4 __attribute__((ext_vector_type(4)));
+            v4 yy = {y.x, y.y, y.z, y.w};
+            __builtin_nontemporal_store(yy, reinterpret_cast<v4 *>(o + ch));
         }
     } else {
         for (int ch = 0; ch < NC; ++ch) {

@@ -70,31 +70,44 @@ __global__ void __launch_bounds__(256) k_setup_bin(Batch b, const float *__restr
     int mine = (live && !spill) ? ntiles : 0;
     int ncols = tx1 - tx0 + 1;
     const int lane = threadIdx.x & 63;
-    // round k: every lane inserts its k-th tile; lanes hitting the same tile share one atomic
+    // Round k: every lane inserts its k-th tile.  The wave sorts its (tile,
+    // lane) keys; the first lane of each run of one tile reserves the run's
+    // slots with one atomic -- all runs at once -- and every sorted lane writes
+    // the triangle of the lane its key came from.  A full bin hands the flag
+    // back to that lane, whose triangle then goes to the spill list.
+    constexpr unsigned kNoTile = (1u << 26) - 1;  // tile field of a lane with nothing to insert
     for (int k = 0;; ++k) {
-        bool active = k < mine;
+        const bool active = k < mine;
         if (!__any(active)) break;
-        int my_tile = -1;
+        unsigned key = (kNoTile << 6) | (unsigned)lane;  // unique: the sorted lanes are a permutation
         if (active) {
             int ty = ty0 + k / ncols, tx = tx0 + k % ncols;
-            my_tile = d.tile_off + ty * d.tiles_x + tx;
+            key = ((unsigned)(ty * d.tiles_x + tx) << 6) | (unsigned)lane;
         }
-        unsigned long long pending = __ballot(active);
-        while (pending) {
-            int leader = __ffsll((unsigned long long)pending) - 1;
-            int lt = __shfl(my_tile, leader);
-            bool same = active && my_tile == lt;
-            unsigned long long m = __ballot(same);
-            int base = 0;
-            if (lane == leader) base = atomicAdd(&bin_count[lt], __popcll(m));
-            base = __shfl(base, leader);
-            if (same) {
-                int slot = base + __popcll(m & lanemask_lt());
-                if (slot < d.bin_cap) bins[d.bin_off + (long long)(lt - d.tile_off) * d.bin_cap + slot] = t;
-                else spill = true;
-            }
-            pending &= ~m;
+        const unsigned sk = wave_sort64<unsigned>(key);
+        const unsigned tl = sk >> 6;
+        const int src = (int)(sk & 63);
+        const bool valid = tl != kNoTile;
+        const unsigned prev = __shfl_up(sk, 1, 64);
+        const bool head = valid && (lane == 0 || (prev >> 6) != tl);
+        // wave-wide ballots first: inside a conditional they would see only its lanes
+        const unsigned long long heads = __ballot(head);
+        const int nvalid = __popcll(__ballot(valid));
+        const unsigned long long upto = (2ull << lane) - 1;  // lanes <= this one
+        const int h = valid ? 63 - __clzll(heads & upto) : 0;
+        const unsigned long long after = heads & ~upto;
+        const int e = after ? __ffsll(after) - 1 : nvalid;  // end of this lane's run
+        int base = 0;
+        if (head) base = atomicAdd(&bin_count[d.tile_off + tl], e - lane);
+        base = __shfl(base, h, 64);
+        const int tsrc = __shfl(t, src, 64);
+        bool full = false;
+        if (valid) {
+            const int slot = base + (lane - h);
+            if (slot < d.bin_cap) bins[d.bin_off + (long long)tl * d.bin_cap + slot] = tsrc;
+            else full = true;
         }
+        if (__builtin_amdgcn_ds_permute(src << 2, full ? 1 : 0) && active) spill = true;
     }
     // spill list (per image): wave-aggregated as well
     unsigned long long sm = __ballot(spill);
@@ -254,6 +267,7 @@ __global__ void __launch_bounds__(TW *TH, kFineWavesPerEU) k_fine(Batch b, const
                 maybe = bc1 >= tc0 && bc0 < tc0 + TW && br1 >= tr0 && br0 < tr0 + TH;
             }
             TriVerts v;
+            maybe = maybe && t >= d.f0 && t < d.f1;  // every entry names a triangle of this image
             if (maybe && load_tri(pos, tri, t, d, v)) {
                 TriSetup s;
                 setup_tri(v, d.H, d.W, s);

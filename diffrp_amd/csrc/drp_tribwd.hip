@@ -40,21 +40,6 @@ template <> struct SortKey<unsigned long long> {
     static constexpr unsigned long long kEmptyT = (1ull << 40) - 1;
 };
 
-// Bitonic sort of the wave's 64 keys across lanes (ascending by lane), no barriers.
-template <typename K>
-__device__ __forceinline__ K wave_sort64(K k) {
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int size = 2; size <= 64; size <<= 1) {
-#pragma unroll
-        for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            K o = __shfl_xor(k, stride, 64);
-            bool keep_min = ((lane & stride) == 0) == ((lane & size) == 0);
-            k = keep_min ? (o < k ? o : k) : (o > k ? o : k);
-        }
-    }
-    return k;
-}
 
 // number of entries of the sorted run a[0..64) below k (reads a[0..63] only)
 template <typename K>
