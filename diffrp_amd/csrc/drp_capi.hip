@@ -313,7 +313,9 @@ int drp_rasterize_bwd(void *stream, const float *pos, int n_img, int V, int pos_
                       const int32_t *inc, void *workspace, size_t workspace_bytes) {
     if (int e = check_dims(n_img, H, W)) return e;
     if (n_img == 0) return 0;
-    if (!pos || !tri || !rast || !grad_rast || !grad_pos) return fail(-1, "null pointer argument");
+    if ((V > 0 && !pos) || (F > 0 && !tri) || !rast || !grad_rast || (V > 0 && !grad_pos))
+        return fail(-1, "null pointer argument");
+    if (F == 0 || V == 0) return 0;  // no triangle: nothing reaches grad_pos
     if (!aligned16(pos) || !aligned16(rast) || !aligned16(grad_rast) || (grad_db && !aligned16(grad_db)) ||
         !aligned16(grad_pos))
         return fail(-1, "pos/rast/grad buffers must be 16-byte aligned");
@@ -359,7 +361,7 @@ int drp_interpolate_fwd(void *stream, const float *attr, int n_img, int V, int a
     if (int e = check_dims(n_img, H, W)) return e;
     if (n_img == 0) return 0;
     if (C < 1) return fail(-1, "attribute channel count must be >= 1");
-    if (!attr || !rast || !tri || !out) return fail(-1, "null pointer argument");
+    if ((V > 0 && !attr) || !rast || (F > 0 && !tri) || !out) return fail(-1, "null pointer argument");
     if (n_diff > 0 && (!rast_db || !diff_idx || !out_da)) return fail(-1, "diff_attrs needs rast_db and out_da");
     if (!aligned16(rast) || (rast_db && !aligned16(rast_db))) return fail(-1, "rast/rast_db must be 16-byte aligned");
     InterpLaunch L{};
@@ -405,7 +407,8 @@ int drp_interpolate_bwd(void *stream, const float *attr, int n_img, int V, int a
     if (int e = check_dims(n_img, H, W)) return e;
     if (n_img == 0) return 0;
     if (C < 1) return fail(-1, "attribute channel count must be >= 1");
-    if (!attr || !rast || !tri || !grad_out || !grad_attr || !grad_rast) return fail(-1, "null pointer argument");
+    if ((V > 0 && (!attr || !grad_attr)) || !rast || (F > 0 && !tri) || !grad_out || !grad_rast)
+        return fail(-1, "null pointer argument");
     if (n_diff > 0 && (!rast_db || !diff_idx)) return fail(-1, "diff_attrs needs rast_db");
     if (!aligned16(rast) || !aligned16(grad_rast) || (grad_db && !aligned16(grad_db)) ||
         (rast_db && !aligned16(rast_db)))
@@ -432,7 +435,7 @@ int drp_interpolate_bwd(void *stream, const float *attr, int n_img, int V, int a
     L.H = H;
     L.W = W;
     L.views_per_group = attr_batched ? 1 : views_per_block(views_per_group);
-    bool tri_path = boxes && inc_off && inc && n_diff == 0 && L.cv > 0;
+    bool tri_path = F > 0 && boxes && inc_off && inc && n_diff == 0 && L.cv > 0;
     if (tri_path) {
         size_t need = drp_interpolate_bwd_workspace_size(n_img, H, W, F, C, attr_batched, L.views_per_group);
         if (!workspace || workspace_bytes < need)
@@ -611,7 +614,7 @@ static int aa_common(AALaunch &L, void *stream, const float *color, int n_img, i
     if (int e = check_dims(n_img, H, W)) return e;
     if (n_img > 65535) return fail(-1, "antialias takes at most 65535 images per call (got %d)", n_img);
     if (C < 1) return fail(-1, "color channel count must be >= 1");
-    if (!color || !rast || !pos || !tri || !opp) return fail(-1, "null pointer argument");
+    if (!color || !rast || (V > 0 && !pos) || (F > 0 && (!tri || !opp))) return fail(-1, "null pointer argument");
     if (!aligned16(pos) || !aligned16(rast)) return fail(-1, "pos/rast must be 16-byte aligned");
     if (C % 4 == 0 && !aligned16(color)) return fail(-1, "color must be 16-byte aligned when C %% 4 == 0");
     size_t need = drp_antialias_workspace_size(n_img, H, W, V, F, pos_batched);
