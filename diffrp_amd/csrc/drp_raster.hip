@@ -428,10 +428,16 @@ __global__ void __launch_bounds__(256) k_gather_pos(const float4 *__restrict__ p
     *o = make_float4(cur.x + s.x, cur.y + s.y, cur.z + s.z, cur.w + s.w);
 }
 
-// slot[c] = position of corner c in the incidence list (inverse permutation)
-__global__ void k_corner_slots(const int *__restrict__ inc, long long n, int *__restrict__ slot) {
+// slot[c] = position of corner c in the incidence list (inverse permutation);
+// -1 for corners whose vertex index is outside [0, V) (not in the list)
+__global__ void k_corner_slots_clear(long long n, int *__restrict__ slot) {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) slot[inc[i]] = (int)i;
+    if (i < n) slot[i] = -1;
+}
+__global__ void k_corner_slots(const int *__restrict__ inc, const int *__restrict__ inc_off, int V,
+                               int *__restrict__ slot) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < inc_off[V]) slot[inc[i]] = (int)i;
 }
 
 // ----------------------------------------------------------------------------
@@ -521,8 +527,10 @@ int launch_rasterize_bwd(const RasterBwdLaunch &L, hipStream_t st) {
         T.views_per_block = 1;
         tile_records_layout(T, L.records, L.b.n_img, L.b.H, L.b.W, 12);
         long long nc = 3ll * L.b.F;
-        hipLaunchKernelGGL(k_corner_slots, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, st, L.inc, nc,
+        hipLaunchKernelGGL(k_corner_slots_clear, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, st, nc,
                            L.corner_slot);
+        hipLaunchKernelGGL(k_corner_slots, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, st, L.inc, L.inc_off,
+                           L.b.V, L.corner_slot);
         T.corner_slot = L.corner_slot;
         if (int rc = launch_tri_bwd_pos(T, st)) return rc;
         int n_out = L.b.pos_batched ? L.b.n_img : 1;

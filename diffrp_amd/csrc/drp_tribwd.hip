@@ -286,9 +286,9 @@ __global__ void __launch_bounds__(256) k_tile_gather(TriBwdLaunch L) {
         // corners go to their vertex's slots (CSR order), so the vertex gather reads contiguously
         float4 *P = reinterpret_cast<float4 *>(L.partial) + (long long)grp * 3 * L.F;
         const int *slot = L.corner_slot + 3 * (long long)t;
-        P[slot[0]] = make_float4(acc[0], acc[1], acc[2], acc[3]);
-        P[slot[1]] = make_float4(acc[4], acc[5], acc[6], acc[7]);
-        P[slot[2]] = make_float4(acc[8], acc[9], acc[10], acc[11]);
+        if (slot[0] >= 0) P[slot[0]] = make_float4(acc[0], acc[1], acc[2], acc[3]);  // -1: vertex out of range
+        if (slot[1] >= 0) P[slot[1]] = make_float4(acc[4], acc[5], acc[6], acc[7]);
+        if (slot[2] >= 0) P[slot[2]] = make_float4(acc[8], acc[9], acc[10], acc[11]);
     } else {
         float *P = L.partial + ((long long)grp * L.F + t) * NV;
 #pragma unroll

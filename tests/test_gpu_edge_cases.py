@@ -88,3 +88,29 @@ def test_triangles_far_larger_than_the_screen(cuda):
         o = oracle.rasterize_bwd(pos.numpy(), tri.numpy(), r.detach().cpu().numpy(), gr.numpy())
         scale = max(np.abs(o).max(), 1e-30)
         assert np.abs(p.grad.cpu().numpy() - o).max() <= 1e-4 * scale, res
+
+
+def test_triangles_with_out_of_range_vertices_are_skipped(cuda):
+    """Triangles naming a vertex outside [0, V) draw nothing and receive no
+    gradient; the rest of the mesh renders and differentiates exactly as
+    without them."""
+    pos, tri, _ = noisy_clip(4, 0.01, 96, views=2)
+    V = pos.shape[1]
+    bad = torch.tensor([[0, 1, V], [V + 5, 2, 3], [-1, 4, 5], [6, 7, 1 << 30]], dtype=torch.int32)
+    out = []
+    for t in (tri.int(), torch.cat([tri.int(), bad])):
+        g = torch.Generator().manual_seed(14)
+        attr = torch.rand(V, 4, generator=g)
+        go = torch.randn(2, 96, 96, 4, generator=g).cuda()
+        p = pos.cuda().requires_grad_(True)
+        a = attr.cuda().requires_grad_(True)
+        tc = t.cuda().contiguous()
+        rast, _ = nvdr.rasterize(nvdr.RasterizeCudaContext(), p, tc, (96, 96), grad_db=False)
+        col, _ = nvdr.interpolate(a, rast, tc)
+        y = nvdr.antialias(col, rast, p, tc)
+        (y * go).sum().backward()
+        out.append((rast.detach(), p.grad.clone(), a.grad.clone()))
+    (r0, gp0, ga0), (r1, gp1, ga1) = out
+    assert torch.equal(r0.view(torch.int32), r1.view(torch.int32))
+    assert torch.equal(ga0, ga1)
+    assert (gp0 - gp1).abs().max() <= 1e-6 * gp0.abs().max()  # antialias: float atomics
