@@ -482,7 +482,8 @@ static int tex_check(int D, int H, int W, int C, long long n, int wrap, int mode
     if (wrap < 0 || wrap > 3 || (wrap == 2 && !cyclic_ok))
         return fail(-1, "wrap must be border (0), reflection (1), %slatlong (3) (got %d)", cyclic_ok ? "cyclic (2), " : "",
                     wrap);
-    if (mode != 0 && mode != 1) return fail(-1, "mode must be bilinear (0) or nearest (1) (got %d)", mode);
+    if (mode < 0 || mode > (cyclic_ok ? 2 : 1))  // the 2-D entry points also take bicubic
+        return fail(-1, "mode must be bilinear (0), nearest (1)%s (got %d)", cyclic_ok ? " or bicubic (2)" : "", mode);
     return 0;
 }
 

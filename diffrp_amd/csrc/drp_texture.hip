@@ -17,10 +17,12 @@
 namespace drp {
 
 enum : int { kWrapBorder = 0, kWrapReflection = 1, kWrapCyclic = 2, kWrapLatlong = 3 };
-enum : int { kModeBilinear = 0, kModeNearest = 1 };
+enum : int { kModeBilinear = 0, kModeNearest = 1, kModeBicubic = 2 };
 
 // grid_sample's source index for one axis with its derivative d(index)/d(grid)
-__device__ __forceinline__ float source_index(float g, int size, bool align, bool reflect, float &dg) {
+__device__ __forceinline__ float pad_coord(float x, int size, bool align, bool reflect, float &dg);
+
+__device__ __forceinline__ float source_index(float g, int size, bool align, bool reflect, bool raw, float &dg) {
     float x;
     if (align) {
         x = (g + 1.f) * 0.5f * (float)(size - 1);
@@ -29,6 +31,11 @@ __device__ __forceinline__ float source_index(float g, int size, bool align, boo
         x = ((g + 1.f) * (float)size - 1.f) * 0.5f;
         dg = (float)size * 0.5f;
     }
+    return raw ? x : pad_coord(x, size, align, reflect, dg);  // bicubic pads each tap instead
+}
+
+// border / reflection padding of a source coordinate, with its derivative factor
+__device__ __forceinline__ float pad_coord(float x, int size, bool align, bool reflect, float &dg) {
     if (reflect) {
         const float lo = align ? 0.f : -0.5f;  // reflection interval [lo, lo + span]
         const float span = align ? (float)(size - 1) : (float)size;
@@ -70,7 +77,7 @@ struct TexAxis {
 };
 
 // texture coordinate -> source index of one axis; flip: y axis (1 - 2v)
-__device__ __forceinline__ TexAxis make_axis(float u, int size, int wrap, bool flip) {
+__device__ __forceinline__ TexAxis make_axis(float u, int size, int wrap, bool flip, bool raw = false) {
     TexAxis a;
     a.period = size;
     float du = 1.f;
@@ -86,13 +93,41 @@ __device__ __forceinline__ TexAxis make_axis(float u, int size, int wrap, bool f
     float dg;
     const bool align = wrap == kWrapLatlong;
     const bool reflect = wrap != kWrapBorder;
-    a.pos = source_index(g, size, align, reflect, dg);
+    a.pos = source_index(g, size, align, reflect, raw, dg);
     a.size = size;
     a.dpos = dg * (flip ? -2.f : 2.f) * du;
     return a;
 }
 
 __device__ __forceinline__ int texel(const TexAxis &a, int i) { return a.size == a.period ? i : i % a.period; }
+
+// bicubic (Keys, A = -0.75, as grid_sample): weights of the taps x0 - 1 .. x0 + 2 at
+// fraction t, the weights' derivatives with respect to -t, and each tap's padded texel
+__device__ __forceinline__ void cubic_weights(float t, float w[4], float dw[4]) {
+    const float A = -0.75f;
+    float x = t + 1.f;
+    w[0] = ((A * x - 5.f * A) * x + 8.f * A) * x - 4.f * A;
+    x = t;
+    w[1] = ((A + 2.f) * x - (A + 3.f)) * x * x + 1.f;
+    x = 1.f - t;
+    w[2] = ((A + 2.f) * x - (A + 3.f)) * x * x + 1.f;
+    x = 2.f - t;
+    w[3] = ((A * x - 5.f * A) * x + 8.f * A) * x - 4.f * A;
+    x = -1.f - t;
+    dw[0] = (-3.f * A * x - 10.f * A) * x - 8.f * A;
+    x = -t;
+    dw[1] = (-3.f * (A + 2.f) * x - 2.f * (A + 3.f)) * x;
+    x = 1.f - t;
+    dw[2] = (3.f * (A + 2.f) * x - 2.f * (A + 3.f)) * x;
+    x = 2.f - t;
+    dw[3] = (3.f * A * x - 10.f * A) * x + 8.f * A;
+}
+
+__device__ __forceinline__ int cubic_tap(const TexAxis &a, int i, int wrap) {
+    float dg = 0.f;
+    const float x = pad_coord((float)i, a.size, wrap == kWrapLatlong, wrap != kWrapBorder, dg);
+    return texel(a, (int)x);
+}
 
 template <int CV>
 __global__ void __launch_bounds__(256) k_sample2d_fwd(const float *__restrict__ tex, int H, int W, int C,
@@ -101,8 +136,32 @@ __global__ void __launch_bounds__(256) k_sample2d_fwd(const float *__restrict__ 
     const int Cn = CV > 0 ? CV : C;
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const TexAxis ax = make_axis(uv[2 * i], W, wrap, false), ay = make_axis(uv[2 * i + 1], H, wrap, true);
+    const bool raw = mode == kModeBicubic;
+    const TexAxis ax = make_axis(uv[2 * i], W, wrap, false, raw), ay = make_axis(uv[2 * i + 1], H, wrap, true, raw);
     float *o = out + i * Cn;
+    if (mode == kModeBicubic) {
+        const float fx0 = floorf(ax.pos), fy0 = floorf(ay.pos);
+        float wx[4], wy[4], dwx[4], dwy[4];
+        cubic_weights(ax.pos - fx0, wx, dwx);
+        cubic_weights(ay.pos - fy0, wy, dwy);
+        int tx[4], ty[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            tx[k] = cubic_tap(ax, (int)fx0 - 1 + k, wrap);
+            ty[k] = cubic_tap(ay, (int)fy0 - 1 + k, wrap);
+        }
+        for (int c = 0; c < Cn; ++c) {
+            float rows[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const float *row = tex + (long long)ty[k] * W * Cn + c;
+                rows[k] = row[tx[0] * Cn] * wx[0] + row[tx[1] * Cn] * wx[1] + row[tx[2] * Cn] * wx[2] +
+                          row[tx[3] * Cn] * wx[3];
+            }
+            o[c] = rows[0] * wy[0] + rows[1] * wy[1] + rows[2] * wy[2] + rows[3] * wy[3];
+        }
+        return;
+    }
     if (mode == kModeNearest) {
         const int x = (int)rintf(ax.pos), y = (int)rintf(ay.pos);
         const bool in = x >= 0 && x < ax.size && y >= 0 && y < ay.size;
@@ -138,8 +197,40 @@ __global__ void __launch_bounds__(256) k_sample2d_bwd(const float *__restrict__ 
     const int Cn = CV > 0 ? CV : C;
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const TexAxis ax = make_axis(uv[2 * i], W, wrap, false), ay = make_axis(uv[2 * i + 1], H, wrap, true);
+    const bool raw = mode == kModeBicubic;
+    const TexAxis ax = make_axis(uv[2 * i], W, wrap, false, raw), ay = make_axis(uv[2 * i + 1], H, wrap, true, raw);
     const float *g = gout + i * Cn;
+    if (mode == kModeBicubic) {
+        const float fx0 = floorf(ax.pos), fy0 = floorf(ay.pos);
+        float wx[4], wy[4], dwx[4], dwy[4];
+        cubic_weights(ax.pos - fx0, wx, dwx);
+        cubic_weights(ay.pos - fy0, wy, dwy);
+        int tx[4], ty[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            tx[k] = cubic_tap(ax, (int)fx0 - 1 + k, wrap);
+            ty[k] = cubic_tap(ay, (int)fy0 - 1 + k, wrap);
+        }
+        float gx = 0.f, gy = 0.f;
+        for (int c = 0; c < Cn; ++c) {
+            const float gc = g[c];
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    const long long off = ((long long)ty[a] * W + tx[b]) * Cn + c;
+                    const float v = tex[off];
+                    gx -= v * dwx[b] * wy[a] * gc;  // dw holds -d w / d t
+                    gy -= v * wx[b] * dwy[a] * gc;
+                    if (gtex) atomicAdd(gtex + off, wy[a] * wx[b] * gc);
+                }
+        }
+        if (guv) {
+            guv[2 * i] = gx * ax.dpos;
+            guv[2 * i + 1] = gy * ay.dpos;
+        }
+        return;
+    }
     if (mode == kModeNearest) {
         const int x = (int)rintf(ax.pos), y = (int)rintf(ay.pos);
         if (gtex && x >= 0 && x < ax.size && y >= 0 && y < ay.size) {

@@ -6,9 +6,9 @@ Texture sampling on the HIP kernels: diffrp's ``sample2d`` / ``sample3d``
 Same signatures and conventions as diffrp: textures are channels-last
 ([H, W, C] / [D, H, W, C]), coordinates (..., 2) / (..., 3) with (0, 0) the
 bottom-left corner; wrap 'border', 'reflection', 'cyclic' (2-D) or 'latlong';
-mode 'bilinear' or 'nearest'.  Differentiable w.r.t. the texture and the
-coordinates (bilinear; nearest has no coordinate gradient, as grid_sample).
-'bicubic' is not provided (no caller on the path uses it).
+mode 'bilinear', 'nearest' or (2-D) 'bicubic' -- grid_sample's modes.
+Differentiable w.r.t. the texture and the coordinates (nearest has no
+coordinate gradient, as in grid_sample).
 """
 from dataclasses import dataclass
 
@@ -18,14 +18,14 @@ from . import _lib
 from ._lib import check, ptr, require_device, stream_of
 
 _WRAP = {'border': 0, 'reflection': 1, 'cyclic': 2, 'latlong': 3}
-_MODE = {'bilinear': 0, 'nearest': 1}
+_MODE = {'bilinear': 0, 'nearest': 1, 'bicubic': 2}
 
 
 def _codes(wrap, mode, dims):
     if wrap not in _WRAP or (dims == 3 and wrap == 'cyclic'):
         raise ValueError(f'unsupported wrap {wrap!r} for sample{dims}d')
-    if mode not in _MODE:
-        raise ValueError(f'unsupported mode {mode!r} (bilinear or nearest)')
+    if mode not in _MODE or (dims == 3 and mode == 'bicubic'):
+        raise ValueError(f'unsupported mode {mode!r} for sample{dims}d')
     return _WRAP[wrap], _MODE[mode]
 
 

@@ -111,7 +111,7 @@ int drp_rasterize_bwd(void *stream, const float *pos, int n_img, int V, int pos_
  * coordinates uv [n,2] / uvw [n,3] with (0,0) the bottom-left corner (last
  * row).  wrap: 0 border, 1 reflection, 2 cyclic (2-D only), 3 latlong
  * (reflection, align_corners); mode: 0 bilinear, 1 nearest (no coordinate
- * gradient).  out [n,C].  Backward: grad_tex accumulated (float atomics, may
+ * gradient), 2 bicubic (2-D only; Keys A = -0.75, padding per tap).  out [n,C].  Backward: grad_tex accumulated (float atomics, may
  * be NULL), grad_uv / grad_uvw written (may be NULL). */
 int drp_sample2d_fwd(void *stream, const float *tex, int H, int W, int C, const float *uv, long long n, int wrap,
                      int mode, float *out);

@@ -70,6 +70,23 @@ def test_sample2d_nearest_matches_grid_sample(cuda, wrap):
     _close(t.grad, t_ref.grad, 1e-5, 'grad texture')
 
 
+@pytest.mark.parametrize('wrap', ['border', 'reflection', 'cyclic', 'latlong'])
+def test_sample2d_bicubic_matches_grid_sample(cuda, wrap):
+    g = torch.Generator().manual_seed(4)
+    tex = torch.rand(23, 31, 3, generator=g).double()
+    uv = _coords(3000, 2, -2.0 if wrap == 'cyclic' else -0.3, 3.0 if wrap == 'cyclic' else 1.3, 10).float().double()
+    go = torch.randn(3000, 3, generator=g, dtype=torch.float64)
+    t_ref, uv_ref = tex.clone().requires_grad_(True), uv.clone().requires_grad_(True)
+    ref = glue.sample2d(t_ref, uv_ref, wrap, 'bicubic')
+    (ref * go).sum().backward()
+    t, u = tex.float().cuda().requires_grad_(True), uv.float().cuda().requires_grad_(True)
+    out = texture.sample2d(t, u, wrap, 'bicubic')
+    (out * go.float().cuda()).sum().backward()
+    _close(out, ref, 1e-5, 'value')
+    _close(t.grad, t_ref.grad, 1e-4, 'grad texture')
+    _close(u.grad, uv_ref.grad, 1e-4, 'grad uv')
+
+
 @pytest.mark.parametrize('wrap', ['border', 'reflection', 'latlong'])
 def test_sample3d_matches_grid_sample(cuda, wrap):
     g = torch.Generator().manual_seed(5)
@@ -94,7 +111,7 @@ def test_texel_centres_and_orientation(cuda):
     # texel (y, x) sits at u = (x + 0.5) / W, v = 1 - (y + 0.5) / H: row 0 is the top
     uv = torch.stack([(xs + 0.5) / W, 1 - (ys + 0.5) / H], -1).float()
     for wrap in ('border', 'reflection', 'cyclic'):
-        for mode in ('bilinear', 'nearest'):
+        for mode in ('bilinear', 'nearest', 'bicubic'):
             out = texture.sample2d(tex.cuda(), uv.cuda(), wrap, mode).cpu()
             np.testing.assert_allclose(out.numpy(), tex.numpy(), rtol=0, atol=1e-4, err_msg=f'{wrap} {mode}')
     corner = texture.sample2d(tex.cuda(), torch.zeros(1, 2).cuda(), 'border').cpu()
