@@ -256,6 +256,45 @@ def cpu_baseline(S, views):
     }
 
 
+def other_configs(dev):
+    """Single-view configurations (reported beside the headline, not the bench value):
+    C1 = README-size mesh (icosphere s=3, r=0.8) at 256^2, forward only, 3 channels;
+    C2 = the noisy s=6 mesh at 1024^2, forward + backward, 8 channels.  Eager calls
+    (the diffrp usage shape) and, for C2, the HIP-graph replay of the same step."""
+    res = {}
+    S1 = Scene(dev, 1, 256, 3, 3, sigma=0.0)
+    with torch.no_grad():
+        def fwd():
+            pos = clip_space(S1.verts, S1.VP)
+            rast, _ = nvdr.rasterize(S1.ctx, pos, S1.tri, (256, 256), grad_db=False)
+            col, _ = nvdr.interpolate(S1.attr, rast, S1.tri)
+            return nvdr.antialias(col, rast, pos, S1.tri, topology_hash=S1.topo)
+        t1 = time_region(fwd, 20, 3)
+    res['C1'] = {'workload': 'icosphere s=3 (642 verts, 1280 faces), 256x256, 3 channels, forward, eager',
+                 'ms': round(t1 * 1e3, 4), 'mpix_s': round(256 * 256 / t1 / 1e6, 1)}
+    S2 = Scene(dev, 1, 1024, 6, 8)
+
+    def step():
+        S2.zero_grad()
+        S2.step_batched()
+    t2 = time_region(step, 10, 3)
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        for _ in range(2):
+            step()
+    torch.cuda.current_stream().wait_stream(side)
+    S2.zero_grad()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        S2.step_batched()
+    t2g = time_region(g.replay, 10, 3)
+    res['C2'] = {'workload': 'noisy icosphere s=6 (40962 verts, 81920 faces), 1024x1024, 8 channels, fwd+bwd',
+                 'eager_ms': round(t2 * 1e3, 4), 'graph_ms': round(t2g * 1e3, 4),
+                 'mpix_s': round(1024 * 1024 / t2g / 1e6, 1)}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -271,6 +310,7 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-ops', action='store_true')
     ap.add_argument('--no-tune', action='store_true')
+    ap.add_argument('--no-configs', action='store_true', help='skip the single-view C1 / C2 timings')
     ap.add_argument('--pmc-calibration', action='store_true',
                     help='first copy 256 MiB device-to-device three times: the reference for counter scaling')
     args = ap.parse_args()
@@ -341,6 +381,7 @@ def main():
                 'traffic': measured_traffic(dom, S, args.subdiv),
                 'bytes_per_launch': ops[dom]['bytes'], 'ms_per_launch': round(ops[dom]['ms'], 4)}
     cpu = None if args.no_cpu else cpu_baseline(S, args.cpu_views)
+    configs = None if args.no_configs else other_configs(dev)
 
     out = {
         'metric': METRIC,
@@ -366,6 +407,7 @@ def main():
         'ops': {k: {'ms': round(v['ms'], 4), 'gbs': round(v['gbs'], 1)} for k, v in ops.items()} if ops else None,
         'roofline': roof,
         'cpu_baseline': cpu,
+        'other_configs': configs,
         'autotune': None if tune is None else {
             'tile': list(tune['best']['tile']), 'bin_capacity': tune['best']['cap'],
             'candidates_ms': {k: round(v['ms'], 4) for k, v in tune['all'].items()}},
