@@ -132,11 +132,28 @@ def test_uneven_batch_full_size(cuda):
     ctx = nvdr.RasterizeCudaContext()
     tuned = batch.tune_batch(ctx, [p.cuda() for p in pos], [t.cuda() for t in tri], res)
     print('autotune', tuned, flush=True)
-    rasts, _ = batch.rasterize_batch(ctx, [p.cuda() for p in pos], [t.cuda() for t in tri], res, grad_db=False)
-    for i, (p, t, r) in enumerate(zip(pos, tri, res)):
-        single, _ = nvdr.rasterize(ctx, p.cuda(), t.cuda(), r, grad_db=False)
+    pc, tc = [p.cuda() for p in pos], [t.cuda() for t in tri]
+    rasts, _ = batch.rasterize_batch(ctx, pc, tc, res, grad_db=False)
+    for i, (p, t, r) in enumerate(zip(pc, tc, res)):
+        single, _ = nvdr.rasterize(ctx, p, t, r, grad_db=False)
         assert torch.equal(rasts[i].view(torch.int32), single.view(torch.int32)), f'item {i}'
     del rasts
+    # throughput of the one-launch batch against a loop of per-item calls (forward rasterisation)
+    import time
+    with torch.no_grad():
+        def timed(fn):
+            fn()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(5):
+                fn()
+            torch.cuda.synchronize()
+            return (time.perf_counter() - t0) / 5
+        tb = timed(lambda: batch.rasterize_batch(ctx, pc, tc, res, grad_db=False))
+        tl = timed(lambda: [nvdr.rasterize(ctx, p, t, r, grad_db=False) for p, t, r in zip(pc, tc, res)])
+    npx = sum(h * w for h, w in res)
+    print(f'forward: batch {tb * 1e3:.2f} ms ({npx / tb / 1e6:.0f} Mpixel/s), per-item loop {tl * 1e3:.2f} ms',
+          flush=True)
     g = torch.Generator().manual_seed(6)
     attrs = [torch.rand(p.shape[0], 4, generator=g).cuda() for p in pos]
     targets = [torch.rand(1, H, W, 4, generator=g).cuda() for H, W in res]
