@@ -21,7 +21,7 @@
 
 namespace drp {
 
-constexpr int kMaxBinTiles = 16;
+constexpr int kMaxBinTiles = 64;  // larger triangles (and full bins) go to the per-image list that every tile scans
 constexpr int kQueuePerLane = 2;  // fine raster: covered-pixel queue entries per lane and pass
 constexpr int kFineWavesPerEU = 6;  // occupancy target of k_fine (VGPR budget 512 / 6)
 
