@@ -51,6 +51,8 @@ int channel_variant(int C) { return (C == 1 || C == 2 || C == 3 || C == 4 || C =
 struct RasterWs {
     int *bin_count, *list_count, *bins, *lists;
     int2 *list_box;
+    char *split_base;
+    int split_slots;
     size_t bytes;
 };
 
@@ -61,13 +63,17 @@ RasterWs raster_ws_layout(void *base, long long n_tiles, int n_img, long long li
     size_t lt = (size_t)(list_total > 0 ? list_total : 1);
     w.bin_count = (int *)(b + off);
     w.list_count = w.bin_count + n_tiles;
-    off = align256(sizeof(int) * (size_t)(n_tiles + n_img));
+    off = align256(sizeof(int) * (size_t)(n_tiles + n_img + kSplitCtl));
     w.bins = (int *)(b + off);
     off += align256(sizeof(int) * (size_t)bin_slots);
     w.lists = (int *)(b + off);
     off += align256(sizeof(int) * lt);
     w.list_box = (int2 *)(b + off);
     off += align256(sizeof(int2) * lt);
+    // heavy-tile split region (its control words follow the list counters)
+    w.split_slots = split_slots(n_tiles);
+    w.split_base = b + off;
+    off += align256(split_region_bytes(w.split_slots));
     w.bytes = off;
     return w;
 }
@@ -138,6 +144,8 @@ int drp_rasterize_fwd(void *stream, const float *pos, int n_img, int V, int pos_
     L.bins = ws.bins;
     L.lists = ws.lists;
     L.list_box = ws.list_box;
+    L.split_base = ws.split_base;
+    L.split_slots = ws.split_slots;
     L.boxes = (ItemBox *)boxes;
     if (boxes && ((uintptr_t)boxes & 3)) return fail(-1, "boxes must be 4-byte aligned");
     L.n_tiles = (int)n_tiles;
@@ -290,6 +298,8 @@ int drp_rasterize_batch_fwd(void *stream, const float *pos, const int32_t *tri, 
     L.bins = ws.bins;
     L.lists = ws.lists;
     L.list_box = ws.list_box;
+    L.split_base = ws.split_base;
+    L.split_slots = ws.split_slots;
     L.boxes = (ItemBox *)boxes;
     L.n_tiles = (int)h.n_tiles;
     L.max_tris = h.max_count;

@@ -304,4 +304,13 @@ __device__ __forceinline__ int xcd_remap(int bid, int nblocks_padded) {
     return (bid & 7) * per + (bid >> 3);
 }
 
+// blocks -> work items in groups of `group` consecutive items per XCD, the
+// groups dealt round robin (block bid runs on XCD bid & 7): neighbouring items
+// share an L2 while every XCD gets a share of each region of the grid.  The
+// grid is padded to a multiple of 8 * group.
+__device__ __forceinline__ int xcd_deal(int bid, int group) {
+    const int i = bid >> 3, g = i / group;
+    return (g * 8 + (bid & 7)) * group + (i - g * group);
+}
+
 }  // namespace drp

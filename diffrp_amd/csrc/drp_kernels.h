@@ -12,13 +12,23 @@ struct ItemBox {
     int cc, rr, hit;
 };
 
+// heavy-tile split of the fine raster: control words after the bin and list
+// counters, at most this many split tiles per call
+constexpr int kSplitCtl = 4;
+constexpr int kSplitSlotsMax = 4096;
+inline int split_slots(long long n_tiles) { return (int)(n_tiles < kSplitSlotsMax ? n_tiles : kSplitSlotsMax); }
+// per slot: int4 of bookkeeping, a done counter, 63 queued chunks, 256 depth keys
+inline size_t split_region_bytes(int slots) { return (size_t)slots * (16 + 4 + 4 * 63 + 8 * 256); }
+
 struct RasterLaunch {
     Batch b;
     const float *pos;
     const int *tri;
     int tile_w, tile_h;
-    int *bin_count;   // [n_tiles], immediately followed by list_count [n_img]
+    int *bin_count;   // [n_tiles], immediately followed by list_count [n_img] and split_ctl [kSplitCtl]
     int *list_count;
+    char *split_base; // heavy-tile split region (split_region_bytes(split_slots))
+    int split_slots;
     int *bins;        // [n_tiles * bin_cap]
     int *lists;       // spill lists
     int2 *list_box;   // pixel box of each spill-list entry

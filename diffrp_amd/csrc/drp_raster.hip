@@ -24,6 +24,8 @@ namespace drp {
 constexpr int kMaxBinTiles = 64;  // larger triangles (and full bins) go to the per-image list that every tile scans
 constexpr int kQueuePerLane = 2;  // fine raster: covered-pixel queue entries per lane and pass
 constexpr int kFineWavesPerEU = 6;  // occupancy target of k_fine (VGPR budget 512 / 6)
+constexpr int kMaxFineNT = 256;     // largest tile (pixels)
+constexpr int kFineXcdGroup = 16;   // fine raster: consecutive tiles per XCD turn (0: one contiguous eighth each)
 
 __device__ __forceinline__ int2 pack_box(int c0, int c1, int r0, int r1) {
     return make_int2(c0 | (c1 << 16), r0 | (r1 << 16));
@@ -202,65 +204,113 @@ __device__ __forceinline__ float inv_ord_f32(uint32_t k) {
     return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
 }
 
-template <int TW, int TH, bool PEEL, bool DB>
-__global__ void __launch_bounds__(TW *TH, kFineWavesPerEU) k_fine(Batch b, const float *__restrict__ pos,
-                                                 const int *__restrict__ tri,
-                                                 const int *__restrict__ bin_count, const int *__restrict__ bins,
-                                                 const int *__restrict__ list_count, const int *__restrict__ lists,
-                                                 const int2 *__restrict__ list_box, int n_tiles, int n_tiles_pad,
-                                                 const float4 *__restrict__ prev, float4 *__restrict__ rast,
-                                                 float4 *__restrict__ rast_db, ItemBox *__restrict__ boxes) {
-    constexpr int NT = TW * TH;
-    static_assert(NT % 64 == 0, "tile must be a whole number of waves");
-    __shared__ FineRec rec[NT];
-    __shared__ unsigned short rowmap[NT];  // row item of a sub-round -> record << 5 | row within the record
-    constexpr int kQueue = NT * kQueuePerLane;
-    __shared__ int queue[kQueue];  // covered pixels: record | x << 8 | y << 16 (tile-local)
-    __shared__ unsigned long long zbuf[NT];
-    __shared__ unsigned long long pkey[PEEL ? NT : 1];
-    __shared__ int wsum[3 * (NT / 64)];  // scan partials: records; row spans (two, alternating)
-    int tile = xcd_remap(blockIdx.x, n_tiles_pad);
-    if (tile >= n_tiles) return;
-    int img = tile_image(b, tile);
-    ImageDesc d = image_desc(b, img);
-    int lt = tile - d.tile_off;
-    int tx = lt % d.tiles_x, ty = lt / d.tiles_x;
-    int tid = threadIdx.x;
-    int tc0 = tx * TW, tr0 = ty * TH;
-    int c = tc0 + (tid % TW), r = tr0 + (tid / TW);
-    bool inimg = c < d.W && r < d.H;
-    long long pix = d.pix_off + (long long)r * d.W + c;
-    zbuf[tid] = ~0ull;
+// LDS of one fine-raster block
+template <int TW, int TH, bool PEEL>
+struct FineLds {
+    static constexpr int NT = TW * TH;
+    static constexpr int kQueue = NT * kQueuePerLane;
+    FineRec rec[NT];
+    unsigned short rowmap[NT];          // row item of a sub-round -> record << 5 | row within the record
+    int queue[kQueue];                  // covered pixels: record | x << 8 | y << 16 (tile-local)
+    unsigned long long zbuf[NT];        // depth-test keys
+    unsigned long long pkey[PEEL ? NT : 1];
+    int wsum[3 * (NT / 64)];            // scan partials: records; row spans (two, alternating)
+    int bcast;                          // block-wide broadcast slot
+};
+
+// one tile's view of the work: its image, its pixel for this lane, and its
+// entry list (bin entries, then the image's spill list or every triangle)
+struct FineTile {
+    ImageDesc d;
+    int tile, tc0, tr0, tw_img, th_img;
+    int nbin, total;
+    bool full_scan;
+};
+
+__device__ __forceinline__ int rfl(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ long long rfl(long long x) {
+    return (long long)(((unsigned long long)(unsigned)rfl((int)(x >> 32)) << 32) | (unsigned)rfl((int)x));
+}
+// the block's image descriptor in scalar registers (the compiler cannot tell
+// that a load after the block's own stores is uniform)
+__device__ __forceinline__ ImageDesc uniform_desc(const ImageDesc &s) {
+    ImageDesc d;
+    d.pos_off = rfl(s.pos_off);
+    d.pix_off = rfl(s.pix_off);
+    d.V = rfl(s.V);
+    d.f0 = rfl(s.f0);
+    d.f1 = rfl(s.f1);
+    d.H = rfl(s.H);
+    d.W = rfl(s.W);
+    d.tiles_x = rfl(s.tiles_x);
+    d.tiles_y = rfl(s.tiles_y);
+    d.tile_off = rfl(s.tile_off);
+    d.list_off = rfl(s.list_off);
+    d.list_cap = rfl(s.list_cap);
+    d.bin_off = rfl(s.bin_off);
+    d.bin_cap = rfl(s.bin_cap);
+    return d;
+}
+
+template <int TW, int TH, bool PEEL>
+__device__ __forceinline__ FineTile fine_begin(FineLds<TW, TH, PEEL> &S, const Batch &b, int tile,
+                                               const int *__restrict__ bin_count,
+                                               const int *__restrict__ list_count, const float4 *__restrict__ prev) {
+    FineTile T;
+    T.tile = tile;
+    const int img = rfl(tile_image(b, tile));
+    T.d = uniform_desc(image_desc(b, img));
+    const ImageDesc &d = T.d;
+    const int lt = tile - d.tile_off;
+    const int tx = lt % d.tiles_x, ty = lt / d.tiles_x;
+    const int tid = threadIdx.x;
+    T.tc0 = tx * TW;
+    T.tr0 = ty * TH;
+    S.zbuf[tid] = ~0ull;
     if (PEEL) {
         unsigned long long pk = ~0ull;  // empty previous layer: nothing can follow
-        if (inimg) {
-            float4 pv = prev[pix];
+        const int c = T.tc0 + (tid % TW), r = T.tr0 + (tid / TW);
+        if (c < d.W && r < d.H) {
+            float4 pv = prev[d.pix_off + (long long)r * d.W + c];
             if (pv.w > 0.f) pk = frag_key(pv.z, float_to_triidx(pv.w) - 1 + (b.local_ids ? d.f0 : 0));
         }
-        pkey[tid] = pk;
+        S.pkey[tid] = pk;
     }
-    int tw_img = min(TW, d.W - tc0), th_img = min(TH, d.H - tr0);  // tile clipped to the image
+    T.tw_img = min(TW, d.W - T.tc0);  // tile clipped to the image
+    T.th_img = min(TH, d.H - T.tr0);
+    const int cnt = rfl(bin_count[tile]);
+    const int lcnt = rfl(list_count[img]);
+    T.full_scan = lcnt > d.list_cap;
+    T.nbin = T.full_scan ? 0 : min(cnt, d.bin_cap);
+    const int nlist = T.full_scan ? (d.f1 - d.f0) : lcnt;
+    T.total = T.nbin + nlist;
+    return T;
+}
 
-    int cnt = bin_count[tile];
-    int lcnt = list_count[img];
-    bool full_scan = lcnt > d.list_cap;
-    int nbin = full_scan ? 0 : min(cnt, d.bin_cap);
-    int nlist = full_scan ? (d.f1 - d.f0) : lcnt;
-    int total = nbin + nlist;
-
+// depth test of the tile's entries [e0, e1) into S.zbuf
+template <int TW, int TH, bool PEEL>
+__device__ __forceinline__ void fine_entries(FineLds<TW, TH, PEEL> &S, const FineTile &T, int e0, int e1,
+                                             const float *__restrict__ pos, const int *__restrict__ tri,
+                                             const int *__restrict__ bins, const int *__restrict__ lists,
+                                             const int2 *__restrict__ list_box) {
+    constexpr int NT = TW * TH;
+    constexpr int kQueue = NT * kQueuePerLane;
+    const ImageDesc &d = T.d;
+    const int tid = threadIdx.x;
+    const int tc0 = T.tc0, tr0 = T.tr0;
     int par = 0;  // scan partials of the row spans in use; alternates every sub-round
-    for (int base = 0; base < total; base += NT) {
+    for (int base = e0; base < e1; base += NT) {
         int idx = base + tid;
         int count = 0;
-        if (idx < total) {
+        if (idx < e1) {
             int t;
             bool maybe = true;
-            if (idx < nbin) {
-                t = bins[d.bin_off + (long long)(tile - d.tile_off) * d.bin_cap + idx];
-            } else if (full_scan) {
-                t = d.f0 + (idx - nbin);
+            if (idx < T.nbin) {
+                t = bins[d.bin_off + (long long)(T.tile - d.tile_off) * d.bin_cap + idx];
+            } else if (T.full_scan) {
+                t = d.f0 + (idx - T.nbin);
             } else {
-                long long li = (long long)d.list_off + idx - nbin;
+                long long li = (long long)d.list_off + idx - T.nbin;
                 t = lists[li];
                 int bc0, bc1, br0, br1;
                 unpack_box(list_box[li], bc0, bc1, br0, br1);
@@ -272,10 +322,10 @@ __global__ void __launch_bounds__(TW *TH, kFineWavesPerEU) k_fine(Batch b, const
                 TriSetup s;
                 setup_tri(v, d.H, d.W, s);
                 if (s.cls != kCull) {
-                    int x0 = max(s.c0 - tc0, 0), x1 = min(s.c1 - tc0, tw_img - 1);
-                    int y0 = max(s.r0 - tr0, 0), y1 = min(s.r1 - tr0, th_img - 1);
+                    int x0 = max(s.c0 - tc0, 0), x1 = min(s.c1 - tc0, T.tw_img - 1);
+                    int y0 = max(s.r0 - tr0, 0), y1 = min(s.r1 - tr0, T.th_img - 1);
                     if (x0 <= x1 && y0 <= y1) {
-                        FineRec &R = rec[tid];
+                        FineRec &R = S.rec[tid];
                         R.t = t;
                         int box = x0 | (y0 << 6) | (x1 << 12) | (s.cls << 18);
                         R.v = v;
@@ -295,7 +345,7 @@ __global__ void __launch_bounds__(TW *TH, kFineWavesPerEU) k_fine(Batch b, const
             }
         }
         int nrows;
-        const int ex = block_excl_scan<NT>(count, wsum, nrows);  // this lane's record owns rows [ex, ex + count)
+        const int ex = block_excl_scan<NT>(count, S.wsum, nrows);  // this lane's record owns rows [ex, ex + count)
         // Row items (record, tile row) in sub-rounds of NT: each lane finds the
         // exact covered span of its row; the covered pixels are queued (block
         // prefix sum, at most kQueue per pass) and evaluated by all lanes, so
@@ -304,14 +354,14 @@ __global__ void __launch_bounds__(TW *TH, kFineWavesPerEU) k_fine(Batch b, const
         for (int sb = 0; sb < nrows; sb += NT, par ^= 1) {
             // each record marks its rows of this sub-round's window (at most 32 rows per record)
             for (int rr = max(ex, sb), re = min(ex + count, sb + NT); rr < re; ++rr)
-                rowmap[rr - sb] = (unsigned short)((tid << 5) | (rr - ex));
+                S.rowmap[rr - sb] = (unsigned short)((tid << 5) | (rr - ex));
             __syncthreads();
             const int ri = sb + tid;
             int len = 0, k = 0, y = 0, xl = 0;
             if (ri < nrows) {
-                const int m = rowmap[tid];
+                const int m = S.rowmap[tid];
                 k = m >> 5;
-                const FineRec &R = rec[k];
+                const FineRec &R = S.rec[k];
                 const int box = R.box;
                 y = rec_y0(box) + (m & 31);
                 xl = rec_x0(box);
@@ -320,15 +370,15 @@ __global__ void __launch_bounds__(TW *TH, kFineWavesPerEU) k_fine(Batch b, const
                 len = max(xr - xl + 1, 0);
             }
             int npix;
-            const int off = block_excl_scan<NT>(len, wsum + (NT / 64) * (1 + par), npix);
+            const int off = block_excl_scan<NT>(len, S.wsum + (NT / 64) * (1 + par), npix);
             for (int q0 = 0; q0 < npix; q0 += kQueue) {
                 const int lo = max(off, q0), hi = min(off + len, q0 + kQueue);
-                for (int i = lo; i < hi; ++i) queue[i - q0] = k | ((xl + i - off) << 8) | (y << 16);
+                for (int i = lo; i < hi; ++i) S.queue[i - q0] = k | ((xl + i - off) << 8) | (y << 16);
                 __syncthreads();
                 const int nq = min(kQueue, npix - q0);
                 for (int qi = tid; qi < nq; qi += NT) {
-                    int en = queue[qi];
-                    const FineRec &R = rec[en & 255];
+                    int en = S.queue[qi];
+                    const FineRec &R = S.rec[en & 255];
                     int lx = (en >> 8) & 255, ly = en >> 16;
                     TriVerts v = R.v;
                     float px = pix_ndc(tc0 + lx, d.W), py = pix_ndc(tr0 + ly, d.H);
@@ -339,7 +389,7 @@ __global__ void __launch_bounds__(TW *TH, kFineWavesPerEU) k_fine(Batch b, const
                         if (zw >= -1.f && zw <= 1.f) {
                             unsigned long long key = frag_key(zw, R.t);
                             int q = ly * TW + lx;
-                            if (!PEEL || key > pkey[q]) atomicMin(&zbuf[q], key);
+                            if (!PEEL || key > S.pkey[q]) atomicMin(&S.zbuf[q], key);
                         }
                     }
                 }
@@ -348,8 +398,18 @@ __global__ void __launch_bounds__(TW *TH, kFineWavesPerEU) k_fine(Batch b, const
         }
         __syncthreads();
     }
-    if (!inimg) return;
-    unsigned long long key = zbuf[tid];
+}
+
+// output of this lane's pixel from the winning key (u, v recomputed for the winner)
+template <int TW, bool DB>
+__device__ __forceinline__ void fine_output(const FineTile &T, unsigned long long key, const Batch &b,
+                                            const float *__restrict__ pos, const int *__restrict__ tri,
+                                            float4 *__restrict__ rast, float4 *__restrict__ rast_db,
+                                            ItemBox *__restrict__ boxes) {
+    const ImageDesc &d = T.d;
+    const int c = T.tc0 + ((int)threadIdx.x % TW), r = T.tr0 + ((int)threadIdx.x / TW);
+    if (c >= d.W || r >= d.H) return;
+    const long long pix = d.pix_off + (long long)r * d.W + c;
     if (key == ~0ull) {
         rast[pix] = make_float4(0.f, 0.f, 0.f, 0.f);
         if (DB) rast_db[pix] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -366,6 +426,130 @@ __global__ void __launch_bounds__(TW *TH, kFineWavesPerEU) k_fine(Batch b, const
     rast[pix] = make_float4(u, vv, zw, triidx_to_float(t + 1 - (b.local_ids ? d.f0 : 0)));
     if (DB) rast_db[pix] = bary_db(v, p, u, vv, d.H, d.W);
     if (boxes) boxes[(long long)d.list_off + (t - d.f0)].hit = 1;  // same value from every winning pixel
+}
+
+// Heavy tiles (more than kSplitRounds rounds of entries) are split into
+// chunks of at least kChunkRounds rounds, at most kMaxChunks per tile: the
+// tile's own block takes chunk 0 and queues the rest for k_fine_split, whose
+// blocks merge their depth keys with device-scope 64-bit atomicMin; the last
+// chunk to finish writes the tile's output.  The minimum is order-free, so
+// the result is the same as one block's.
+constexpr int kChunkRounds = 4;
+constexpr int kSplitRounds = 8;
+constexpr int kMaxChunks = 64;
+
+// Split bookkeeping: three control words after the image's list counters
+// (slots taken, queue length, queue head) and one workspace region of
+// `slots` entries: int4 info (tile, chunk size, chunk count), int done
+// counter, 63 queued chunks (slot << 8 | chunk), kMaxFineNT depth keys.
+struct SplitWs {
+    int *ctl;
+    int4 *info;
+    int *done;
+    int *queue;
+    unsigned long long *gkey;
+};
+__device__ __forceinline__ SplitWs split_ws(int *list_count, int n_img, char *base, int slots) {
+    SplitWs w;
+    w.ctl = list_count + n_img;
+    w.info = reinterpret_cast<int4 *>(base);
+    w.done = reinterpret_cast<int *>(base + 16 * (size_t)slots);
+    w.queue = w.done + slots;
+    w.gkey = reinterpret_cast<unsigned long long *>(base + 272 * (size_t)slots);  // after 16 + 4 + 252 bytes per slot
+    return w;
+}
+static_assert(kMaxFineNT == 256, "split_region_bytes assumes 256 keys per slot");
+
+template <int TW, int TH, bool PEEL, bool DB>
+__global__ void __launch_bounds__(TW *TH, kFineWavesPerEU) k_fine(Batch b, const float *__restrict__ pos,
+                                                 const int *__restrict__ tri,
+                                                 const int *__restrict__ bin_count, const int *__restrict__ bins,
+                                                 int *__restrict__ list_count, const int *__restrict__ lists,
+                                                 const int2 *__restrict__ list_box, int n_tiles, int xcd_group,
+                                                 char *split_base, int split_slots, const float4 *__restrict__ prev,
+                                                 float4 *__restrict__ rast, float4 *__restrict__ rast_db,
+                                                 ItemBox *__restrict__ boxes) {
+    constexpr int NT = TW * TH;
+    static_assert(NT % 64 == 0, "tile must be a whole number of waves");
+    __shared__ FineLds<TW, TH, PEEL> S;
+    int tile = xcd_deal(blockIdx.x, xcd_group);
+    if (tile >= n_tiles) return;
+    const FineTile T = fine_begin<TW, TH, PEEL>(S, b, tile, bin_count, list_count, prev);
+    int e1 = T.total;
+    int slot = -1;
+    if (split_slots > 0 && T.total > kSplitRounds * NT) {
+        const int per = NT * max(kChunkRounds, T.total / (NT * kMaxChunks) + 1);
+        const int nch = (T.total + per - 1) / per;  // >= 2
+        if (threadIdx.x == 0) {
+            const SplitWs sp = split_ws(list_count, b.n_img, split_base, split_slots);
+            int s = atomicAdd(&sp.ctl[0], 1);
+            if (s < split_slots) {
+                sp.info[s] = make_int4(tile, per, nch, 0);
+                sp.done[s] = 0;
+                const int q = atomicAdd(&sp.ctl[1], nch - 1);  // at most (kMaxChunks - 1) per slot: fits
+                for (int k = 1; k < nch; ++k) sp.queue[q + k - 1] = (s << 8) | k;
+            } else {
+                s = -1;
+            }
+            S.bcast = s;
+        }
+        __syncthreads();
+        slot = __builtin_amdgcn_readfirstlane(S.bcast);
+        if (slot >= 0) e1 = per;
+    }
+    fine_entries<TW, TH, PEEL>(S, T, 0, e1, pos, tri, bins, lists, list_box);
+    if (slot >= 0) {  // chunk 0 of a split tile: its keys seed the merge
+        const SplitWs sp = split_ws(list_count, b.n_img, split_base, split_slots);
+        sp.gkey[(long long)slot * kMaxFineNT + threadIdx.x] = S.zbuf[threadIdx.x];
+        return;
+    }
+    fine_output<TW, DB>(T, S.zbuf[threadIdx.x], b, pos, tri, rast, rast_db, boxes);
+}
+
+// chunks 1.. of split tiles: a fixed grid drains the queue k_fine filled
+template <int TW, int TH, bool PEEL, bool DB>
+__global__ void __launch_bounds__(TW *TH, kFineWavesPerEU) k_fine_split(Batch b, const float *__restrict__ pos,
+                                                 const int *__restrict__ tri,
+                                                 const int *__restrict__ bin_count, const int *__restrict__ bins,
+                                                 int *__restrict__ list_count, const int *__restrict__ lists,
+                                                 const int2 *__restrict__ list_box, char *split_base,
+                                                 int split_slots, const float4 *__restrict__ prev,
+                                                 float4 *__restrict__ rast, float4 *__restrict__ rast_db,
+                                                 ItemBox *__restrict__ boxes) {
+    __shared__ FineLds<TW, TH, PEEL> S;
+    const SplitWs sp = split_ws(list_count, b.n_img, split_base, split_slots);
+    const int nq = sp.ctl[1];  // final: written by the previous kernel
+    if (nq == 0) return;
+    const int tid = threadIdx.x;
+    for (;;) {
+        if (tid == 0) S.bcast = atomicAdd(&sp.ctl[2], 1);
+        __syncthreads();
+        // the item is the same for every lane: keep it (and what follows from it) in scalar registers
+        const int it = __builtin_amdgcn_readfirstlane(S.bcast);
+        if (it >= nq) return;
+        const int e = __builtin_amdgcn_readfirstlane(sp.queue[it]);
+        const int slot = e >> 8, chunk = e & 255;
+        const int4 inf = sp.info[slot];
+        const int itile = __builtin_amdgcn_readfirstlane(inf.x), per = __builtin_amdgcn_readfirstlane(inf.y),
+                  nch = __builtin_amdgcn_readfirstlane(inf.z);
+        const FineTile T = fine_begin<TW, TH, PEEL>(S, b, itile, bin_count, list_count, prev);
+        const int e0 = chunk * per, e1 = min(e0 + per, T.total);
+        fine_entries<TW, TH, PEEL>(S, T, e0, e1, pos, tri, bins, lists, list_box);
+        unsigned long long *g = sp.gkey + (long long)slot * kMaxFineNT + tid;
+        const unsigned long long mine = S.zbuf[tid];
+        if (mine != ~0ull) atomicMin(g, mine);
+        __threadfence();
+        __syncthreads();
+        if (tid == 0) S.bcast = atomicAdd(&sp.done[slot], 1);
+        __syncthreads();
+        const bool last = S.bcast == nch - 2;  // chunks 1 .. nch-1 run here
+        if (last) {
+            __threadfence();
+            const unsigned long long key = atomicMin(g, ~0ull);  // read at the point of coherence
+            fine_output<TW, DB>(T, key, b, pos, tri, rast, rast_db, boxes);
+        }
+        __syncthreads();  // S is reused by the next item
+    }
 }
 
 // per-pixel fallback: atomics into grad_pos
@@ -474,10 +658,20 @@ int launch_count_covered(const float *rast, int n_img, long long npx, int *count
 
 template <int TW, int TH, bool PEEL, bool DB>
 static void launch_fine_v(const RasterLaunch &L, hipStream_t st) {
-    int pad = (L.n_tiles + 7) & ~7;
+    if (L.n_tiles <= 0) return;
+    const int group = kFineXcdGroup > 0 ? kFineXcdGroup : max(1, (L.n_tiles + 7) / 8);
+    const long long unit = 8ll * group;
+    const int pad = (int)((L.n_tiles + unit - 1) / unit * unit);
     hipLaunchKernelGGL((k_fine<TW, TH, PEEL, DB>), dim3(pad), dim3(TW * TH), 0, st, L.b, L.pos, L.tri,
-                       L.bin_count, L.bins, L.list_count, L.lists, L.list_box, L.n_tiles, pad,
-                       (const float4 *)L.prev, (float4 *)L.rast, (float4 *)L.rast_db, L.boxes);
+                       L.bin_count, L.bins, L.list_count, L.lists, L.list_box, L.n_tiles, group, L.split_base,
+                       L.split_slots, (const float4 *)L.prev, (float4 *)L.rast, (float4 *)L.rast_db, L.boxes);
+    if (L.split_slots > 0) {
+        // six blocks of 256 lanes per CU, whatever the tile
+        const int grid = 6 * 256 * (256 / (TW * TH));
+        hipLaunchKernelGGL((k_fine_split<TW, TH, PEEL, DB>), dim3(grid), dim3(TW * TH), 0, st, L.b, L.pos, L.tri,
+                           L.bin_count, L.bins, L.list_count, L.lists, L.list_box, L.split_base, L.split_slots,
+                           (const float4 *)L.prev, (float4 *)L.rast, (float4 *)L.rast_db, L.boxes);
+    }
 }
 
 template <int TW, int TH>
@@ -492,8 +686,8 @@ static void launch_fine_t(const RasterLaunch &L, hipStream_t st) {
 }
 
 int launch_rasterize(const RasterLaunch &L, hipStream_t st) {
-    // zero bin counters and spill-list counters (a kernel, not a memset node)
-    long long nz = (long long)L.n_tiles + L.b.n_img;
+    // zero bin counters, spill-list counters and the split controls (a kernel, not a memset node)
+    long long nz = (long long)L.n_tiles + L.b.n_img + kSplitCtl;
     hipLaunchKernelGGL(k_zero_i32, dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, st, L.bin_count, nz);
     if (L.max_tris > 0) {
         dim3 grid((L.max_tris + 255) / 256, L.b.n_img);
