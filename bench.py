@@ -197,7 +197,7 @@ def per_op_times(S, iters=10):
 
 # kernels each operator call launches (the unit of the per-operator timings)
 OP_KERNELS = {
-    'rasterize_fwd': ['k_zero_i32', 'k_setup_bin', 'k_fine'],
+    'rasterize_fwd': ['k_zero_i32', 'k_setup_bin', 'k_fine', 'k_fine_split'],
     'interpolate_fwd': ['k_interp_fwd'],
     'antialias_fwd': ['k_aa_project', 'k_aa_orient', 'k_aa_silhouette', 'k_aa_pairs', 'k_aa_gather'],
     'antialias_bwd': ['k_aa_pairs_bwd', 'k_aa_gather_bwd'],

@@ -42,8 +42,14 @@ __global__ void __launch_bounds__(256) k_interp_fwd(const float *__restrict__ at
     if (gid >= total_pix) return;
     InterpPix p = interp_pix(rast, tri, gid, F, V);
     float *o = out + gid * NC;
+    typedef float v4 __attribute__((ext_vector_type(4)));
     if (!p.hit) {
-        for (int ch = 0; ch < NC; ++ch) o[ch] = 0.f;
+        if (CV > 0 && (CV % 4) == 0) {
+            const v4 z = {0.f, 0.f, 0.f, 0.f};
+            for (int ch = 0; ch < NC; ch += 4) __builtin_nontemporal_store(z, reinterpret_cast<v4 *>(o + ch));
+        } else {
+            for (int ch = 0; ch < NC; ++ch) o[ch] = 0.f;
+        }
         if (out_da)
             for (int j = 0; j < 2 * n_diff; ++j) out_da[gid * 2 * n_diff + j] = 0.f;
         return;
@@ -60,7 +66,6 @@ __global__ void __launch_bounds__(256) k_interp_fwd(const float *__restrict__ at
             y.y = fma_(x0.y - x2.y, p.u, fma_(x1.y - x2.y, p.v, x2.y));
             y.z = fma_(x0.z - x2.z, p.u, fma_(x1.z - x2.z, p.v, x2.z));
             y.w = fma_(x0.w - x2.w, p.u, fma_(x1.w - x2.w, p.v, x2.w));
-            typedef float v4 __attribute__((ext_vector_type(4)));
             v4 yy = {y.x, y.y, y.z, y.w};
             __builtin_nontemporal_store(yy, reinterpret_cast<v4 *>(o + ch));
         }
