@@ -450,7 +450,8 @@ int drp_interpolate_bwd(void *stream, const float *attr, int n_img, int V, int a
         size_t need = drp_interpolate_bwd_workspace_size(n_img, H, W, F, C, attr_batched, L.views_per_group);
         if (!workspace || workspace_bytes < need)
             return fail(-1, "workspace too small: need %zu bytes, got %zu", need, workspace_bytes);
-        if (C % 4 == 0 && !aligned16(grad_out)) return fail(-1, "grad_out must be 16-byte aligned");
+        if (C % 4 == 0 && (!aligned16(grad_out) || (V > 0 && !aligned16(attr))))
+            return fail(-1, "grad_out and attr must be 16-byte aligned when C %% 4 == 0");
         L.boxes = (const ItemBox *)boxes;
         L.partial = (float *)workspace;
         L.records = (char *)workspace + interp_partial_bytes(n_img, F, C, attr_batched, L.views_per_group);

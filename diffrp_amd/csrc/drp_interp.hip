@@ -189,8 +189,16 @@ __global__ void __launch_bounds__(256) k_gather_attr(const float *__restrict__ p
         const float *P = partial + (long long)g * F * 3 * NC;
         for (int i = i0; i < i1; ++i) {
             const float *q = P + (long long)inc[i] * NC;
+            if (NC % 4 == 0) {  // corner partials are NC * 4 bytes from an aligned base
 #pragma unroll
-            for (int ch = 0; ch < NC; ++ch) s[ch] += q[ch];
+                for (int ch = 0; ch < NC; ch += 4) {
+                    const float4 x = *reinterpret_cast<const float4 *>(q + ch);
+                    s[ch] += x.x; s[ch + 1] += x.y; s[ch + 2] += x.z; s[ch + 3] += x.w;
+                }
+            } else {
+#pragma unroll
+                for (int ch = 0; ch < NC; ++ch) s[ch] += q[ch];
+            }
         }
     }
     float *o = grad_attr + ((long long)out * V + v) * NC;

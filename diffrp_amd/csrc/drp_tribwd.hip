@@ -140,11 +140,26 @@ __global__ void __launch_bounds__(256) k_tile_reduce(TriBwdLaunch L) {
                     const float *A = L.attr + (L.attr_batched ? (long long)img * L.V * C : 0);
                     const float *a0 = A + (long long)i0 * C, *a1 = A + (long long)i1 * C, *a2 = A + (long long)i2 * C;
                     float u = rv.x, v = rv.y, w2 = (1.f - u) - v, su = 0.f, sv = 0.f;
+                    float r0[C], r1[C], r2[C];  // the three attribute rows (float4 loads when C % 4 == 0)
+                    if (C % 4 == 0) {
+#pragma unroll
+                        for (int q = 0; q < C / 4; ++q) {
+                            const float4 x0 = reinterpret_cast<const float4 *>(a0)[q];
+                            const float4 x1 = reinterpret_cast<const float4 *>(a1)[q];
+                            const float4 x2 = reinterpret_cast<const float4 *>(a2)[q];
+                            r0[4 * q] = x0.x; r0[4 * q + 1] = x0.y; r0[4 * q + 2] = x0.z; r0[4 * q + 3] = x0.w;
+                            r1[4 * q] = x1.x; r1[4 * q + 1] = x1.y; r1[4 * q + 2] = x1.z; r1[4 * q + 3] = x1.w;
+                            r2[4 * q] = x2.x; r2[4 * q + 1] = x2.y; r2[4 * q + 2] = x2.z; r2[4 * q + 3] = x2.w;
+                        }
+                    } else {
+#pragma unroll
+                        for (int ch = 0; ch < C; ++ch) { r0[ch] = a0[ch]; r1[ch] = a1[ch]; r2[ch] = a2[ch]; }
+                    }
 #pragma unroll
                     for (int ch = 0; ch < C; ++ch) {
-                        float gc = gv[ch], x2 = a2[ch];
-                        su += gc * (a0[ch] - x2);
-                        sv += gc * (a1[ch] - x2);
+                        float gc = gv[ch], x2 = r2[ch];
+                        su += gc * (r0[ch] - x2);
+                        sv += gc * (r1[ch] - x2);
                         val[ch] = u * gc;
                         val[C + ch] = v * gc;
                         val[2 * C + ch] = w2 * gc;
@@ -263,8 +278,16 @@ __device__ __forceinline__ void gather_records(const TriBwdLaunch &L, int t, int
         for (int k = 0; k < kSearch; ++k) {
             if (hit[k] < 0) continue;
             const float *q = L.rec_v + (rb[k] + hit[k]) * NV;
+            if (NV % 4 == 0) {  // records are 16-byte aligned: NV * 4 bytes each from an aligned base
 #pragma unroll
-            for (int j = 0; j < NV; ++j) acc[j] += q[j];
+                for (int j = 0; j < NV; j += 4) {
+                    const float4 x = *reinterpret_cast<const float4 *>(q + j);
+                    acc[j] += x.x; acc[j + 1] += x.y; acc[j + 2] += x.z; acc[j + 3] += x.w;
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < NV; ++j) acc[j] += q[j];
+            }
         }
         if (n < kSearch) return;
     }
@@ -291,8 +314,14 @@ __global__ void __launch_bounds__(256) k_tile_gather(TriBwdLaunch L) {
         if (slot[2] >= 0) P[slot[2]] = make_float4(acc[8], acc[9], acc[10], acc[11]);
     } else {
         float *P = L.partial + ((long long)grp * L.F + t) * NV;
+        if (NV % 4 == 0) {
 #pragma unroll
-        for (int j = 0; j < NV; ++j) P[j] = acc[j];
+            for (int j = 0; j < NV; j += 4)
+                *reinterpret_cast<float4 *>(P + j) = make_float4(acc[j], acc[j + 1], acc[j + 2], acc[j + 3]);
+        } else {
+#pragma unroll
+            for (int j = 0; j < NV; ++j) P[j] = acc[j];
+        }
     }
 }
 
