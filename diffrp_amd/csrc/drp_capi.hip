@@ -50,7 +50,7 @@ int channel_variant(int C) { return (C == 1 || C == 2 || C == 3 || C == 4 || C =
 
 struct RasterWs {
     int *bin_count, *list_count, *bins, *lists;
-    int2 *list_box;
+    int4 *setup;
     char *split_base;
     int split_slots;
     size_t bytes;
@@ -68,8 +68,8 @@ RasterWs raster_ws_layout(void *base, long long n_tiles, int n_img, long long li
     off += align256(sizeof(int) * (size_t)bin_slots);
     w.lists = (int *)(b + off);
     off += align256(sizeof(int) * lt);
-    w.list_box = (int2 *)(b + off);
-    off += align256(sizeof(int2) * lt);
+    w.setup = (int4 *)(b + off);
+    off += align256(2 * sizeof(int4) * lt);
     // heavy-tile split region (its control words follow the list counters)
     w.split_slots = split_slots(n_tiles);
     w.split_base = b + off;
@@ -143,7 +143,7 @@ int drp_rasterize_fwd(void *stream, const float *pos, int n_img, int V, int pos_
     L.list_count = ws.list_count;
     L.bins = ws.bins;
     L.lists = ws.lists;
-    L.list_box = ws.list_box;
+    L.setup = ws.setup;
     L.split_base = ws.split_base;
     L.split_slots = ws.split_slots;
     L.boxes = (ItemBox *)boxes;
@@ -297,7 +297,7 @@ int drp_rasterize_batch_fwd(void *stream, const float *pos, const int32_t *tri, 
     L.list_count = ws.list_count;
     L.bins = ws.bins;
     L.lists = ws.lists;
-    L.list_box = ws.list_box;
+    L.setup = ws.setup;
     L.split_base = ws.split_base;
     L.split_slots = ws.split_slots;
     L.boxes = (ItemBox *)boxes;

@@ -143,6 +143,23 @@ __device__ __forceinline__ long long floor_div16(long long a) {  // floor(a / 16
     return a >> 4;
 }
 
+// pixel box (centres inside the snapped vertices' bounding box), clipped to
+// the image; false when empty
+__device__ __forceinline__ bool direct_box(const int X[3], const int Y[3], int H, int W, int &bc0, int &bc1,
+                                           int &br0, int &br1) {
+    int mnx = min(X[0], min(X[1], X[2])), mxx = max(X[0], max(X[1], X[2]));
+    int mny = min(Y[0], min(Y[1], Y[2])), mxy = max(Y[0], max(Y[1], Y[2]));
+    long long c0 = -floor_div16(-((long long)mnx - kHalfSub)), c1 = floor_div16((long long)mxx - kHalfSub);
+    long long r0 = -floor_div16(-((long long)mny - kHalfSub)), r1 = floor_div16((long long)mxy - kHalfSub);
+    c0 = c0 < 0 ? 0 : c0;
+    r0 = r0 < 0 ? 0 : r0;
+    c1 = c1 > W - 1 ? W - 1 : c1;
+    r1 = r1 > H - 1 ? H - 1 : r1;
+    if (c0 > c1 || r0 > r1) return false;
+    bc0 = (int)c0; bc1 = (int)c1; br0 = (int)r0; br1 = (int)r1;
+    return true;
+}
+
 __device__ __forceinline__ void setup_tri(const TriVerts &v, int H, int W, TriSetup &s) {
     s.cls = kCull;
     for (int k = 0; k < 3; ++k)
@@ -171,16 +188,7 @@ __device__ __forceinline__ void setup_tri(const TriVerts &v, int H, int W, TriSe
             int dx = s.X[j] - s.X[k], dy = s.Y[j] - s.Y[k];
             s.tl[k] = (dy < 0) || (dy == 0 && dx < 0);
         }
-        int mnx = min(X[0], min(X[1], X[2])), mxx = max(X[0], max(X[1], X[2]));
-        int mny = min(Y[0], min(Y[1], Y[2])), mxy = max(Y[0], max(Y[1], Y[2]));
-        long long c0 = -floor_div16(-((long long)mnx - kHalfSub)), c1 = floor_div16((long long)mxx - kHalfSub);
-        long long r0 = -floor_div16(-((long long)mny - kHalfSub)), r1 = floor_div16((long long)mxy - kHalfSub);
-        c0 = c0 < 0 ? 0 : c0;
-        r0 = r0 < 0 ? 0 : r0;
-        c1 = c1 > W - 1 ? W - 1 : c1;
-        r1 = r1 > H - 1 ? H - 1 : r1;
-        if (c0 > c1 || r0 > r1) return;
-        s.c0 = (int)c0; s.c1 = (int)c1; s.r0 = (int)r0; s.r1 = (int)r1;
+        if (!direct_box(X, Y, H, W, s.c0, s.c1, s.r0, s.r1)) return;
         s.cls = kDirect;
         return;
     }

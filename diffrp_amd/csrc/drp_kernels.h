@@ -31,7 +31,7 @@ struct RasterLaunch {
     int split_slots;
     int *bins;        // [n_tiles * bin_cap]
     int *lists;       // spill lists
-    int2 *list_box;   // pixel box of each spill-list entry
+    int4 *setup;      // setup records: two int4 per (image, triangle), at the image's list offset
     ItemBox *boxes;   // optional [n_img, F] pixel box + hit flag per (image, triangle)
     int n_tiles;
     int max_tris;     // largest triangle range of any image

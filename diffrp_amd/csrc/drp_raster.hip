@@ -1,9 +1,10 @@
 // Rasterizer kernels for gfx950.
 //
-// Forward: (1) k_setup_bin, one thread per (image, triangle): setup, per-(image,
+// Forward: (1) k_setup_bin, one thread per (image, triangle): setup (stored as
+// a 32-byte setup record: snapped vertices, class, top-left flags), per-(image,
 // triangle) pixel box, and wave-aggregated insertion into fixed-capacity
-// screen-tile bins (full bins and large triangles spill, with their box, to a
-// per-image list); (2) k_fine, one workgroup per tile: triangle records staged
+// screen-tile bins (full bins and large triangles spill to a per-image list);
+// (2) k_fine, one workgroup per tile: triangle records staged
 // in LDS, then coverage by rows -- lanes take (record, row of its box inside
 // the tile) items from a block prefix sum and find the row's exact covered
 // span; the covered pixels are queued and evaluated by all lanes -- with the
@@ -31,12 +32,27 @@ __device__ __forceinline__ int2 pack_box(int c0, int c1, int r0, int r1) {
     return make_int2(c0 | (c1 << 16), r0 | (r1 << 16));
 }
 __device__ __forceinline__ int2 empty_box() { return make_int2(0xffff, 0xffff); }  // c0 = 65535 > c1 = 0
-__device__ __forceinline__ void unpack_box(int2 b, int &c0, int &c1, int &r0, int &r1) {
-    c0 = b.x & 0xffff; c1 = (unsigned)b.x >> 16; r0 = b.y & 0xffff; r1 = (unsigned)b.y >> 16;
-}
 
 __device__ __forceinline__ unsigned long long lanemask_lt() {
     return (1ull << (threadIdx.x & 63)) - 1ull;
+}
+
+// ----------------------------------------------------------------------------
+// setup records: two int4 per (image, triangle), at the image's list offset
+//   direct: {X0, X1, X2, meta}, {Y0, Y1, Y2, 0}  (snapped, counter-clockwise)
+//   homogeneous: {c0 | c1 << 16, r0 | r1 << 16, 0, meta}, {0, 0, 0, 0}
+//   meta = class | top-left flags << 2 (class kCull: nothing to draw)
+// ----------------------------------------------------------------------------
+__device__ __forceinline__ void store_setup(int4 *__restrict__ setup, long long i, const TriSetup &s) {
+    int4 a = make_int4(0, 0, 0, s.cls), b = make_int4(0, 0, 0, 0);
+    if (s.cls == kDirect) {
+        a = make_int4(s.X[0], s.X[1], s.X[2], s.cls | (s.tl[0] << 2) | (s.tl[1] << 3) | (s.tl[2] << 4));
+        b = make_int4(s.Y[0], s.Y[1], s.Y[2], 0);
+    } else if (s.cls == kHomog) {
+        a = make_int4(s.c0 | (s.c1 << 16), s.r0 | (s.r1 << 16), 0, s.cls);
+    }
+    setup[2 * i] = a;
+    setup[2 * i + 1] = b;
 }
 
 // ----------------------------------------------------------------------------
@@ -46,7 +62,7 @@ __global__ void __launch_bounds__(256) k_setup_bin(Batch b, const float *__restr
                                                    const int *__restrict__ tri, int tile_w, int tile_h,
                                                    int *__restrict__ bin_count,
                                                    int *__restrict__ bins, int *__restrict__ list_count,
-                                                   int *__restrict__ lists, int2 *__restrict__ list_box,
+                                                   int *__restrict__ lists, int4 *__restrict__ setup,
                                                    ItemBox *__restrict__ boxes) {
     int img = blockIdx.y;
     ImageDesc d = image_desc(b, img);
@@ -57,6 +73,7 @@ __global__ void __launch_bounds__(256) k_setup_bin(Batch b, const float *__restr
     if (valid) {
         TriVerts v;
         if (load_tri(pos, tri, t, d, v)) setup_tri(v, d.H, d.W, s);
+        store_setup(setup, (long long)d.list_off + (t - d.f0), s);
         if (boxes) {
             int2 pb = s.cls == kCull ? empty_box() : pack_box(s.c0, s.c1, s.r0, s.r1);
             boxes[(long long)d.list_off + (t - d.f0)] = ItemBox{pb.x, pb.y, 0};  // hit: set by k_fine
@@ -120,10 +137,7 @@ __global__ void __launch_bounds__(256) k_setup_bin(Batch b, const float *__restr
         base = __shfl(base, leader);
         if (spill) {
             int slot = base + __popcll(sm & lanemask_lt());
-            if (slot < d.list_cap) {
-                lists[(long long)d.list_off + slot] = t;
-                list_box[(long long)d.list_off + slot] = pack_box(s.c0, s.c1, s.r0, s.r1);
-            }
+            if (slot < d.list_cap) lists[(long long)d.list_off + slot] = t;
         }
     }
 }
@@ -292,7 +306,7 @@ template <int TW, int TH, bool PEEL>
 __device__ __forceinline__ void fine_entries(FineLds<TW, TH, PEEL> &S, const FineTile &T, int e0, int e1,
                                              const float *__restrict__ pos, const int *__restrict__ tri,
                                              const int *__restrict__ bins, const int *__restrict__ lists,
-                                             const int2 *__restrict__ list_box) {
+                                             const int4 *__restrict__ setup) {
     constexpr int NT = TW * TH;
     constexpr int kQueue = NT * kQueuePerLane;
     const ImageDesc &d = T.d;
@@ -304,40 +318,38 @@ __device__ __forceinline__ void fine_entries(FineLds<TW, TH, PEEL> &S, const Fin
         int count = 0;
         if (idx < e1) {
             int t;
-            bool maybe = true;
-            if (idx < T.nbin) {
-                t = bins[d.bin_off + (long long)(T.tile - d.tile_off) * d.bin_cap + idx];
-            } else if (T.full_scan) {
-                t = d.f0 + (idx - T.nbin);
-            } else {
-                long long li = (long long)d.list_off + idx - T.nbin;
-                t = lists[li];
-                int bc0, bc1, br0, br1;
-                unpack_box(list_box[li], bc0, bc1, br0, br1);
-                maybe = bc1 >= tc0 && bc0 < tc0 + TW && br1 >= tr0 && br0 < tr0 + TH;
-            }
-            TriVerts v;
-            maybe = maybe && t >= d.f0 && t < d.f1;  // every entry names a triangle of this image
-            if (maybe && load_tri(pos, tri, t, d, v)) {
-                TriSetup s;
-                setup_tri(v, d.H, d.W, s);
-                if (s.cls != kCull) {
-                    int x0 = max(s.c0 - tc0, 0), x1 = min(s.c1 - tc0, T.tw_img - 1);
-                    int y0 = max(s.r0 - tr0, 0), y1 = min(s.r1 - tr0, T.th_img - 1);
-                    if (x0 <= x1 && y0 <= y1) {
+            if (idx < T.nbin) t = bins[d.bin_off + (long long)(T.tile - d.tile_off) * d.bin_cap + idx];
+            else if (T.full_scan) t = d.f0 + (idx - T.nbin);
+            else t = lists[(long long)d.list_off + idx - T.nbin];
+            if (t >= d.f0 && t < d.f1) {  // every entry names a triangle of this image
+                const long long si = 2 * ((long long)d.list_off + (t - d.f0));
+                const int4 sa = setup[si];
+                const int cls = sa.w & 3;
+                if (cls != kCull) {
+                    int4 sb = make_int4(0, 0, 0, 0);
+                    int c0, c1, r0, r1;
+                    if (cls == kDirect) {
+                        sb = setup[si + 1];
+                        const int X[3] = {sa.x, sa.y, sa.z}, Y[3] = {sb.x, sb.y, sb.z};
+                        direct_box(X, Y, d.H, d.W, c0, c1, r0, r1);  // not empty: the setup pass kept it
+                    } else {
+                        c0 = sa.x & 0xffff; c1 = (unsigned)sa.x >> 16;
+                        r0 = sa.y & 0xffff; r1 = (unsigned)sa.y >> 16;
+                    }
+                    const int x0 = max(c0 - tc0, 0), x1 = min(c1 - tc0, T.tw_img - 1);
+                    const int y0 = max(r0 - tr0, 0), y1 = min(r1 - tr0, T.th_img - 1);
+                    TriVerts v;
+                    if (x0 <= x1 && y0 <= y1 && load_tri(pos, tri, t, d, v)) {
                         FineRec &R = S.rec[tid];
                         R.t = t;
-                        int box = x0 | (y0 << 6) | (x1 << 12) | (s.cls << 18);
                         R.v = v;
                         count = y1 - y0 + 1;  // row items
-                        if (s.cls == kDirect) {
+                        int box = x0 | (y0 << 6) | (x1 << 12) | (cls << 18);
+                        if (cls == kDirect) {
                             const int Px = kSubpix * tc0 + kHalfSub, Py = kSubpix * tr0 + kHalfSub;
-#pragma unroll
-                            for (int e = 0; e < 3; ++e) {
-                                R.X[e] = s.X[e] - Px;
-                                R.Y[e] = s.Y[e] - Py;
-                                box |= (s.tl[e] ? 1 : 0) << (20 + e);
-                            }
+                            R.X[0] = sa.x - Px; R.X[1] = sa.y - Px; R.X[2] = sa.z - Px;
+                            R.Y[0] = sb.x - Py; R.Y[1] = sb.y - Py; R.Y[2] = sb.z - Py;
+                            box |= ((sa.w >> 2) & 7) << 20;  // top-left flags
                         }
                         R.box = box;
                     }
@@ -465,7 +477,7 @@ __global__ void __launch_bounds__(TW *TH, kFineWavesPerEU) k_fine(Batch b, const
                                                  const int *__restrict__ tri,
                                                  const int *__restrict__ bin_count, const int *__restrict__ bins,
                                                  int *__restrict__ list_count, const int *__restrict__ lists,
-                                                 const int2 *__restrict__ list_box, int n_tiles, int xcd_group,
+                                                 const int4 *__restrict__ setup, int n_tiles, int xcd_group,
                                                  char *split_base, int split_slots, const float4 *__restrict__ prev,
                                                  float4 *__restrict__ rast, float4 *__restrict__ rast_db,
                                                  ItemBox *__restrict__ boxes) {
@@ -497,7 +509,7 @@ __global__ void __launch_bounds__(TW *TH, kFineWavesPerEU) k_fine(Batch b, const
         slot = __builtin_amdgcn_readfirstlane(S.bcast);
         if (slot >= 0) e1 = per;
     }
-    fine_entries<TW, TH, PEEL>(S, T, 0, e1, pos, tri, bins, lists, list_box);
+    fine_entries<TW, TH, PEEL>(S, T, 0, e1, pos, tri, bins, lists, setup);
     if (slot >= 0) {  // chunk 0 of a split tile: its keys seed the merge
         const SplitWs sp = split_ws(list_count, b.n_img, split_base, split_slots);
         sp.gkey[(long long)slot * kMaxFineNT + threadIdx.x] = S.zbuf[threadIdx.x];
@@ -512,7 +524,7 @@ __global__ void __launch_bounds__(TW *TH, kFineWavesPerEU) k_fine_split(Batch b,
                                                  const int *__restrict__ tri,
                                                  const int *__restrict__ bin_count, const int *__restrict__ bins,
                                                  int *__restrict__ list_count, const int *__restrict__ lists,
-                                                 const int2 *__restrict__ list_box, char *split_base,
+                                                 const int4 *__restrict__ setup, char *split_base,
                                                  int split_slots, const float4 *__restrict__ prev,
                                                  float4 *__restrict__ rast, float4 *__restrict__ rast_db,
                                                  ItemBox *__restrict__ boxes) {
@@ -534,7 +546,7 @@ __global__ void __launch_bounds__(TW *TH, kFineWavesPerEU) k_fine_split(Batch b,
                   nch = __builtin_amdgcn_readfirstlane(inf.z);
         const FineTile T = fine_begin<TW, TH, PEEL>(S, b, itile, bin_count, list_count, prev);
         const int e0 = chunk * per, e1 = min(e0 + per, T.total);
-        fine_entries<TW, TH, PEEL>(S, T, e0, e1, pos, tri, bins, lists, list_box);
+        fine_entries<TW, TH, PEEL>(S, T, e0, e1, pos, tri, bins, lists, setup);
         unsigned long long *g = sp.gkey + (long long)slot * kMaxFineNT + tid;
         const unsigned long long mine = S.zbuf[tid];
         if (mine != ~0ull) atomicMin(g, mine);
@@ -663,13 +675,13 @@ static void launch_fine_v(const RasterLaunch &L, hipStream_t st) {
     const long long unit = 8ll * group;
     const int pad = (int)((L.n_tiles + unit - 1) / unit * unit);
     hipLaunchKernelGGL((k_fine<TW, TH, PEEL, DB>), dim3(pad), dim3(TW * TH), 0, st, L.b, L.pos, L.tri,
-                       L.bin_count, L.bins, L.list_count, L.lists, L.list_box, L.n_tiles, group, L.split_base,
+                       L.bin_count, L.bins, L.list_count, L.lists, L.setup, L.n_tiles, group, L.split_base,
                        L.split_slots, (const float4 *)L.prev, (float4 *)L.rast, (float4 *)L.rast_db, L.boxes);
     if (L.split_slots > 0) {
         // six blocks of 256 lanes per CU, whatever the tile
         const int grid = 6 * 256 * (256 / (TW * TH));
         hipLaunchKernelGGL((k_fine_split<TW, TH, PEEL, DB>), dim3(grid), dim3(TW * TH), 0, st, L.b, L.pos, L.tri,
-                           L.bin_count, L.bins, L.list_count, L.lists, L.list_box, L.split_base, L.split_slots,
+                           L.bin_count, L.bins, L.list_count, L.lists, L.setup, L.split_base, L.split_slots,
                            (const float4 *)L.prev, (float4 *)L.rast, (float4 *)L.rast_db, L.boxes);
     }
 }
@@ -692,7 +704,7 @@ int launch_rasterize(const RasterLaunch &L, hipStream_t st) {
     if (L.max_tris > 0) {
         dim3 grid((L.max_tris + 255) / 256, L.b.n_img);
         hipLaunchKernelGGL(k_setup_bin, grid, dim3(256), 0, st, L.b, L.pos, L.tri, L.tile_w, L.tile_h,
-                           L.bin_count, L.bins, L.list_count, L.lists, L.list_box, L.boxes);
+                           L.bin_count, L.bins, L.list_count, L.lists, L.setup, L.boxes);
     }
     if (L.tile_w == 8 && L.tile_h == 8) launch_fine_t<8, 8>(L, st);
     else if (L.tile_w == 16 && L.tile_h == 16) launch_fine_t<16, 16>(L, st);
