@@ -26,7 +26,7 @@ constexpr int kMaxBinTiles = 64;  // larger triangles (and full bins) go to the 
 constexpr int kQueuePerLane = 2;  // fine raster: covered-pixel queue entries per lane and pass
 constexpr int kFineWavesPerEU = 6;  // occupancy target of k_fine (VGPR budget 512 / 6)
 constexpr int kMaxFineNT = 256;     // largest tile (pixels)
-constexpr int kFineXcdGroup = 16;   // fine raster: consecutive tiles per XCD turn (0: one contiguous eighth each)
+constexpr int kFineXcdGroup = 16;   // fine raster: consecutive tiles per XCD turn when tiles are dealt
 
 __device__ __forceinline__ int2 pack_box(int c0, int c1, int r0, int r1) {
     return make_int2(c0 | (c1 << 16), r0 | (r1 << 16));
@@ -671,7 +671,12 @@ int launch_count_covered(const float *rast, int n_img, long long npx, int *count
 template <int TW, int TH, bool PEEL, bool DB>
 static void launch_fine_v(const RasterLaunch &L, hipStream_t st) {
     if (L.n_tiles <= 0) return;
-    const int group = kFineXcdGroup > 0 ? kFineXcdGroup : max(1, (L.n_tiles + 7) / 8);
+    // Eight or more images of one size: each XCD takes one contiguous eighth of
+    // the tiles (whole images, so its L2 holds one image's triangles and setup
+    // records at a time).  Otherwise groups of kFineXcdGroup tiles are dealt
+    // round robin, so no XCD is left with the heavy images or the heavy band.
+    const bool eighths = L.b.descs == nullptr && L.b.n_img >= 8;
+    const int group = eighths ? max(1, (L.n_tiles + 7) / 8) : kFineXcdGroup;
     const long long unit = 8ll * group;
     const int pad = (int)((L.n_tiles + unit - 1) / unit * unit);
     hipLaunchKernelGGL((k_fine<TW, TH, PEEL, DB>), dim3(pad), dim3(TW * TH), 0, st, L.b, L.pos, L.tri,
