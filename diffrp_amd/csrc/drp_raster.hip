@@ -9,7 +9,9 @@
 // the tile) items from a block prefix sum and find the row's exact covered
 // span; the covered pixels are queued and evaluated by all lanes -- with the
 // depth test resolved by 64-bit LDS atomicMin on (orderable(z/w) << 32 |
-// triangle); (u, v) are recomputed for the winner only.
+// triangle); (u, v) are recomputed for the winner only.  Tiles with many
+// entries are split into chunks: (3) k_fine_split drains the chunk queue and
+// merges the chunks' depth keys with a device-wide 64-bit minimum.
 //
 // The setup pass also records each (image, triangle)'s pixel box, and k_fine
 // flags the triangles that won a pixel; the deterministic backward
