@@ -63,6 +63,7 @@ class RasterizeCudaContext:
         self.tile = None if tile is None else tuple(tile)
         self.bin_capacity = bin_capacity
         self._tuned = {}
+        self._pending = {}  # shape -> (tile, capacity, pinned peak bin count, event) of its first eager call
         _lib.lib()
 
     def set_tuning(self, key, tile, bin_capacity):
@@ -75,11 +76,41 @@ class RasterizeCudaContext:
 
     def config(self, n_img, H, W, F) -> Tuple[Tuple[int, int], int]:
         key = self.tuning_key(n_img, H, W, F)
-        if key in self._tuned and self.tile is None and self.bin_capacity is None:
-            return self._tuned[key]
+        if self.tile is None and self.bin_capacity is None:
+            if key not in self._tuned:
+                self._settle(key)
+            if key in self._tuned:
+                return self._tuned[key]
         tile = self.tile or default_tile(H, W, F)
         cap = self.bin_capacity or default_bin_capacity(tile, H, W, F)
         return tile, cap
+
+    # Capacity learnt from the first eager call of a shape: the per-tile bin
+    # counts are exact whatever the capacity, so their maximum, read back
+    # without a host wait at a later call, sizes the bins of the calls that
+    # follow (a crumpled mesh can load its tiles far beyond the heuristic, and
+    # spilled triangles are scanned by every tile of the image).
+    def _watch(self, key, tile, cap, counts: torch.Tensor):
+        if (self.tile is not None or self.bin_capacity is not None or key in self._tuned or key in self._pending
+                or counts.numel() == 0 or torch.cuda.is_current_stream_capturing()):
+            return
+        host = torch.empty(1, dtype=torch.int32, pin_memory=True)
+        host.copy_(counts.max().view(1), non_blocking=True)
+        ev = torch.cuda.Event()
+        ev.record()
+        self._pending[key] = (tile, cap, host, ev)
+
+    def _settle(self, key):
+        ent = self._pending.get(key)
+        if ent is None or torch.cuda.is_current_stream_capturing() or not ent[3].query():
+            return
+        tile, cap, host, _ = self._pending.pop(key)
+        peak = int(host[0])
+        if peak > cap:
+            cap = 32
+            while cap < int(peak * 1.15) + 1 and cap < (1 << 16):
+                cap *= 2
+        self._tuned[key] = (tile, cap)
 
 
 RasterizeHipContext = RasterizeCudaContext
@@ -186,6 +217,8 @@ def _rasterize_launch(ctx, pos, tri, H, W, rng, prev, want_db):
                              ptr(rng.dev) if rng is not None else None, H, W, ptr(prev), ptr(rast), ptr(rast_db),
                              ptr(ws), ws_bytes, tw, th, cap, list_total, max_range, ptr(boxes))
     check(rc, 'rasterize')
+    n_tiles = n * ((W + tw - 1) // tw) * ((H + th - 1) // th)
+    ctx._watch(ctx.tuning_key(n, H, W, F), (tw, th), cap, ws[:4 * n_tiles].view(torch.int32))  # bin counts
     return rast, rast_db, boxes
 
 

@@ -32,3 +32,27 @@ def test_bin_load_reports_spill(cuda):
     assert peak > 4 and spilled > 0
     peak2, spilled2 = autotune.bin_load(pos.cuda(), tri.cuda(), (128, 128), (16, 16), probe_cap=4096)
     assert peak2 == peak and spilled2 == 0
+
+
+def test_capacity_learnt_from_first_call(cuda):
+    """A crumpled mesh loads its tiles far beyond the default bin capacity: the
+    first eager call's exact bin counts (read back at the next call, without a
+    host wait) size the capacity of the calls that follow; every call matches
+    the oracle bit for bit."""
+    import oracle
+    res = (128, 128)
+    pos, tri, _ = noisy_clip(7, 0.05, res[0])
+    ctx = nvdr.RasterizeCudaContext()
+    F = tri.shape[0]
+    tile0 = nvdr.default_tile(*res, F)
+    cap0 = nvdr.default_bin_capacity(tile0, *res, F)
+    peak, spilled = autotune.bin_load(pos.cuda(), tri.cuda(), res, tile0, probe_cap=cap0)
+    assert peak > cap0 and spilled > 0  # the heuristic's bins overflow here
+    ref = oracle.rasterize(pos.numpy(), tri.numpy(), res)
+    bits = np.ascontiguousarray(ref, np.float32).view(np.uint32)
+    for call in range(3):
+        r, _ = nvdr.rasterize(ctx, pos.cuda(), tri.cuda(), res, grad_db=False)
+        assert np.array_equal(r.cpu().numpy().view(np.uint32), bits), f'call {call}'  # .cpu() waits
+        if call >= 1:
+            tile, cap = ctx.config(1, *res, F)
+            assert tile == tile0 and cap >= peak, (tile, cap, peak)
