@@ -151,7 +151,8 @@ struct FineRec {
     int X[3], Y[3];   // snapped vertices minus the tile's first pixel centre (sub-pixels; direct class)
     TriVerts v;
     int t;
-    int box;          // tile-local box columns x0, x1 and first row y0 (6 bits each), class << 18, top-left flags << 20
+    int box;          // tile-local box columns x0, x1 and first row y0 (6 bits each), class << 18, top-left
+                      // flags << 20, small-coordinate flag << 23
 };
 __device__ __forceinline__ int rec_x0(int box) { return box & 63; }
 __device__ __forceinline__ int rec_y0(int box) { return (box >> 6) & 63; }
@@ -193,6 +194,42 @@ __device__ __forceinline__ void row_span(const FineRec &R, int box, int y, int &
     }
 }
 
+
+// The same span for records whose vertices lie within 2^14 sub-pixels of the
+// tile's first pixel centre (flag bit 23 of the box): every product then fits
+// in 32 bits (|C_e| < 2^30.2, |a_e x| < 2^24.1 for x in [-2, 33]) and a float
+// estimate suffices -- a crossing inside the clamp window has |C_e| < 2^24.1,
+// exact in float, so the estimate is off by far less than one column, and
+// one exact 32-bit step either way settles it (tests/test_rowspan_math.py
+// models this arithmetic against brute-force coverage: slivers and the
+// coordinate limit included, the reciprocal perturbed by one ulp).
+constexpr int kSmallRec = 1 << 14;
+__device__ __forceinline__ void row_span32(const FineRec &R, int box, int y, int &xl, int &xr) {
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+        const int j = e == 2 ? 0 : e + 1;
+        const int dx = R.X[j] - R.X[e], dy = R.Y[j] - R.Y[e];
+        const int a = -dy * kSubpix;
+        const int cc = dx * (kSubpix * y - R.Y[e]) + dy * R.X[e] - (((box >> (20 + e)) & 1) ? 0 : 1);
+        if (a == 0) {
+            if (cc < 0) xr = xl - 1;
+            continue;
+        }
+        float q = -(float)cc * __builtin_amdgcn_rcpf((float)(-dy) * (float)kSubpix);
+        q = fminf(fmaxf(q, (float)(xl - 1)), (float)(xr + 1));
+        if (a > 0) {
+            int x0 = (int)ceilf(q);
+            if (cc + a * (x0 - 1) >= 0) --x0;
+            else if (cc + a * x0 < 0) ++x0;
+            xl = max(xl, x0);
+        } else {
+            int x0 = (int)floorf(q);
+            if (cc + a * (x0 + 1) >= 0) ++x0;
+            else if (cc + a * x0 < 0) --x0;
+            xr = min(xr, x0);
+        }
+    }
+}
 
 template <int NT>
 __device__ __forceinline__ int block_excl_scan(int v, int *wsum, int &total) {
@@ -352,6 +389,9 @@ __device__ __forceinline__ void fine_entries(FineLds<TW, TH, PEEL> &S, const Fin
                             R.X[0] = sa.x - Px; R.X[1] = sa.y - Px; R.X[2] = sa.z - Px;
                             R.Y[0] = sb.x - Py; R.Y[1] = sb.y - Py; R.Y[2] = sb.z - Py;
                             box |= ((sa.w >> 2) & 7) << 20;  // top-left flags
+                            const int m = max(max(max(abs(R.X[0]), abs(R.X[1])), abs(R.X[2])),
+                                              max(max(abs(R.Y[0]), abs(R.Y[1])), abs(R.Y[2])));
+                            if (m < kSmallRec) box |= 1 << 23;  // 32-bit row spans
                         }
                         R.box = box;
                     }
@@ -380,7 +420,10 @@ __device__ __forceinline__ void fine_entries(FineLds<TW, TH, PEEL> &S, const Fin
                 y = rec_y0(box) + (m & 31);
                 xl = rec_x0(box);
                 int xr = rec_x1(box);
-                if (rec_cls(box) == kDirect) row_span(R, box, y, xl, xr);
+                if (rec_cls(box) == kDirect) {
+                    if ((box >> 23) & 1) row_span32(R, box, y, xl, xr);
+                    else row_span(R, box, y, xl, xr);
+                }
                 len = max(xr - xl + 1, 0);
             }
             int npix;
