@@ -92,8 +92,8 @@ class RasterizeCudaContext:
     # spilled triangles are scanned by every tile of the image).
     def _watch(self, key, tile, cap, counts: torch.Tensor):
         if (self.tile is not None or self.bin_capacity is not None or key in self._tuned or key in self._pending
-                or counts.numel() == 0 or torch.cuda.is_current_stream_capturing()):
-            return
+                or len(self._pending) >= 64 or counts.numel() == 0 or torch.cuda.is_current_stream_capturing()):
+            return  # (at most 64 shapes wait for their first read-back)
         host = torch.empty(1, dtype=torch.int32, pin_memory=True)
         host.copy_(counts.max().view(1), non_blocking=True)
         ev = torch.cuda.Event()
