@@ -91,18 +91,40 @@ __global__ void __launch_bounds__(256) k_setup_bin(Batch b, const float *__restr
     int mine = (live && !spill) ? ntiles : 0;
     int ncols = tx1 - tx0 + 1;
     const int lane = threadIdx.x & 63;
-    // Round k: every lane inserts its k-th tile.  The wave sorts its (tile,
-    // lane) keys; the first lane of each run of one tile reserves the run's
-    // slots with one atomic -- all runs at once -- and every sorted lane writes
-    // the triangle of the lane its key came from.  A full bin hands the flag
-    // back to that lane, whose triangle then goes to the spill list.
+    // The wave's (triangle, tile) pairs are numbered in lane order (lane i's
+    // triangle owns pairs [excl, incl)) and inserted 64 per round, so a round
+    // is full whatever the spread of tile counts.  Each lane finds its pair's
+    // triangle by a binary search over the lanes' inclusive counts; the wave
+    // sorts the pairs' (tile, lane) keys; the first lane of each run of one
+    // tile reserves the run's slots with one atomic -- all runs at once -- and
+    // every sorted lane writes the triangle of the pair its key came from.  A
+    // full bin hands the flag back to the pair's lane, and a triangle with a
+    // full pair goes to the spill list.  (Every shuffle and ballot below runs
+    // in wave-uniform control flow.)
     constexpr unsigned kNoTile = (1u << 26) - 1;  // tile field of a lane with nothing to insert
-    for (int k = 0;; ++k) {
-        const bool active = k < mine;
-        if (!__any(active)) break;
+    int incl = mine;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += y;
+    }
+    const int excl = incl - mine;
+    const int total = __shfl(incl, 63, 64);
+    for (int r0 = 0; r0 < total; r0 += 64) {
+        const int p = r0 + lane;
+        const bool active = p < total;
+        int sl = 0;  // the pair's lane: the number of lanes whose pairs end at or before p
+#pragma unroll
+        for (int step = 32; step >= 1; step >>= 1) {
+            const int v = __shfl(incl, sl + step - 1, 64);
+            if (v <= p) sl += step;
+        }
+        const int k = p - __shfl(excl, sl, 64);
+        const int nc = __shfl(ncols, sl, 64), px0 = __shfl(tx0, sl, 64), py0 = __shfl(ty0, sl, 64);
+        const int tp = __shfl(t, sl, 64);
         unsigned key = (kNoTile << 6) | (unsigned)lane;  // unique: the sorted lanes are a permutation
         if (active) {
-            int ty = ty0 + k / ncols, tx = tx0 + k % ncols;
+            const int ty = py0 + k / nc, tx = px0 + k % nc;
             key = ((unsigned)(ty * d.tiles_x + tx) << 6) | (unsigned)lane;
         }
         const unsigned sk = wave_sort64<unsigned>(key);
@@ -111,7 +133,6 @@ __global__ void __launch_bounds__(256) k_setup_bin(Batch b, const float *__restr
         const bool valid = tl != kNoTile;
         const unsigned prev = __shfl_up(sk, 1, 64);
         const bool head = valid && (lane == 0 || (prev >> 6) != tl);
-        // wave-wide ballots first: inside a conditional they would see only its lanes
         const unsigned long long heads = __ballot(head);
         const int nvalid = __popcll(__ballot(valid));
         const unsigned long long upto = (2ull << lane) - 1;  // lanes <= this one
@@ -121,14 +142,21 @@ __global__ void __launch_bounds__(256) k_setup_bin(Batch b, const float *__restr
         int base = 0;
         if (head) base = atomicAdd(&bin_count[d.tile_off + tl], e - lane);
         base = __shfl(base, h, 64);
-        const int tsrc = __shfl(t, src, 64);
+        const int tsrc = __shfl(tp, src, 64);
         bool full = false;
         if (valid) {
             const int slot = base + (lane - h);
             if (slot < d.bin_cap) bins[d.bin_off + (long long)tl * d.bin_cap + slot] = tsrc;
             else full = true;
         }
-        if (__builtin_amdgcn_ds_permute(src << 2, full ? 1 : 0) && active) spill = true;
+        const bool pair_full = __builtin_amdgcn_ds_permute(src << 2, full ? 1 : 0) != 0;
+        const unsigned long long fm = __ballot(pair_full && active);
+        // this lane's triangle's pairs of this round: lanes [lo, hi)
+        const int lo = max(excl, r0) - r0, hi = min(incl, r0 + 64) - r0;
+        if (hi > lo) {
+            const unsigned long long m = (hi - lo == 64 ? ~0ull : ((1ull << (hi - lo)) - 1)) << lo;
+            if (fm & m) spill = true;
+        }
     }
     // spill list (per image): wave-aggregated as well
     unsigned long long sm = __ballot(spill);
