@@ -299,35 +299,72 @@ __device__ __forceinline__ int silh_of(const AAImg &I, int id) { return id ? I.s
 
 // weights[pix] = (signed weight of pair (pix, right), of pair (pix, above));
 // decisions[pix] (when kept) the two pairs' decisions for the backward.
-// Blocks of 64 x 4 pixels (grid: columns, rows, images); the three rast values
-// and silhouette bytes of a pixel's two pairs are loaded before either pair is
-// evaluated.
+// Blocks of 64 x 4 pixels (grid: columns, rows, images).  Only pairs whose two
+// triangles differ and that have a silhouette-capable edge on either side can
+// blend; the block queues those (in LDS) and evaluates the queue densely, so
+// the double-precision crossing runs on full waves, then every pixel writes
+// its two results.
 __global__ void __launch_bounds__(256) k_aa_pairs(AALaunch L, float2 *__restrict__ weights,
                                                   int2 *__restrict__ decisions) {
-    const int c = blockIdx.x * 64 + (threadIdx.x & 63), r = blockIdx.y * 4 + (threadIdx.x >> 6), img = blockIdx.z;
-    if (c >= L.W || r >= L.H) return;
+    __shared__ int q[512];      // local pixel | direction << 8 (0: right, 1: above)
+    __shared__ float2 sw[256];
+    __shared__ int2 sd[256];
+    __shared__ int q_n;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int c = blockIdx.x * 64 + lane, r = blockIdx.y * 4 + (tid >> 6), img = blockIdx.z;
+    const bool in = c < L.W && r < L.H;
     const AAImg I = aa_img(L, img);
-    const long long q = (long long)r * L.W + c, g = (long long)img * L.H * L.W + q;
-    const bool has_r = c + 1 < L.W, has_u = r + 1 < L.H;
-    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-    const float4 p0 = I.rast[q], pr = has_r ? I.rast[q + 1] : z4, pu = has_u ? I.rast[q + L.W] : z4;
-    const int i0 = rast_id(p0, I.F), ir = rast_id(pr, I.F), iu = rast_id(pu, I.F);
-    const int s0 = silh_of(I, i0), sr = (has_r && ir != i0) ? silh_of(I, ir) : 0,
-              su = (has_u && iu != i0) ? silh_of(I, iu) : 0;
-    float2 w = make_float2(0.f, 0.f);
-    int2 dec = make_int2(0, 0);
-    if (has_r) {
-        PairRes p = eval_pair(I, L.tri, p0, pr, s0, sr, r, c, r, c + 1);
-        w.x = signed_w(p);
-        dec.x = pack_decision(p);
+    const long long q0 = (long long)r * L.W + c;
+    const bool has_r = in && c + 1 < L.W, has_u = in && r + 1 < L.H;
+    int cand = 0;  // bit 0: pair (pix, right) may blend; bit 1: pair (pix, above)
+    if (in) {
+        const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+        const float4 p0 = I.rast[q0], pr = has_r ? I.rast[q0 + 1] : z4, pu = has_u ? I.rast[q0 + L.W] : z4;
+        const int i0 = rast_id(p0, I.F), ir = rast_id(pr, I.F), iu = rast_id(pu, I.F);
+        const int s0 = silh_of(I, i0) & 7;
+        if (has_r && ir != i0 && (s0 || (silh_of(I, ir) & 7))) cand |= 1;
+        if (has_u && iu != i0 && (s0 || (silh_of(I, iu) & 7))) cand |= 2;
     }
-    if (has_u) {
-        PairRes p = eval_pair(I, L.tri, p0, pu, s0, su, r, c, r + 1, c);
-        w.y = signed_w(p);
-        dec.y = pack_decision(p);
+    sw[tid] = make_float2(0.f, 0.f);
+    sd[tid] = make_int2(0, 0);
+    if (tid == 0) q_n = 0;
+    __syncthreads();
+    // wave-aggregated append
+    const int nq = (cand & 1) + (cand >> 1);
+    int x = nq;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
     }
-    weights[g] = w;
-    if (decisions) decisions[g] = dec;
+    const int wtot = __shfl(x, 63, 64);
+    int base = 0;
+    if (lane == 63 && wtot > 0) base = atomicAdd(&q_n, wtot);
+    base = __shfl(base, 63, 64) + x - nq;
+    if (cand & 1) q[base++] = tid;
+    if (cand & 2) q[base] = tid | (1 << 8);
+    __syncthreads();
+    const int n = q_n;
+    for (int e = tid; e < n; e += 256) {
+        const int m = q[e], lp = m & 255, dir = m >> 8;
+        const int ra = blockIdx.y * 4 + (lp >> 6), ca = blockIdx.x * 64 + (lp & 63);
+        const int rb = dir ? ra + 1 : ra, cb = dir ? ca : ca + 1;
+        const float4 pa = I.rast[(long long)ra * L.W + ca], pb = I.rast[(long long)rb * L.W + cb];
+        const int ia = rast_id(pa, I.F), ib = rast_id(pb, I.F);
+        const PairRes p = eval_pair(I, L.tri, pa, pb, silh_of(I, ia), silh_of(I, ib), ra, ca, rb, cb);
+        if (dir) {
+            sw[lp].y = signed_w(p);
+            sd[lp].y = pack_decision(p);
+        } else {
+            sw[lp].x = signed_w(p);
+            sd[lp].x = pack_decision(p);
+        }
+    }
+    __syncthreads();
+    if (!in) return;
+    const long long g = (long long)img * L.H * L.W + q0;
+    weights[g] = sw[tid];
+    if (decisions) decisions[g] = sd[tid];
 }
 
 // channel chunk handled by one thread: 4 (one float4) when C is a multiple of 4
