@@ -187,15 +187,31 @@ __global__ void __launch_bounds__(256) k_gather_attr(const float *__restrict__ p
     int g0 = per_output_groups ? out : 0, g1 = per_output_groups ? out + 1 : n_groups;
     for (int g = g0; g < g1; ++g) {
         const float *P = partial + (long long)g * F * 3 * NC;
-        for (int i = i0; i < i1; ++i) {
-            const float *q = P + (long long)inc[i] * NC;
-            if (NC % 4 == 0) {  // corner partials are NC * 4 bytes from an aligned base
+        if (NC % 4 == 0 && NC <= 8) {
+            // four corners' loads in flight at once (corner partials are NC * 4 bytes from an
+            // aligned base), summed in corner order
+            for (int i = i0; i < i1; i += 4) {
+                float4 x[4][NC / 4];
 #pragma unroll
-                for (int ch = 0; ch < NC; ch += 4) {
-                    const float4 x = *reinterpret_cast<const float4 *>(q + ch);
-                    s[ch] += x.x; s[ch + 1] += x.y; s[ch + 2] += x.z; s[ch + 3] += x.w;
-                }
-            } else {
+                for (int k = 0; k < 4; ++k)
+                    if (i + k < i1) {
+                        const float4 *q = reinterpret_cast<const float4 *>(P + (long long)inc[i + k] * NC);
+#pragma unroll
+                        for (int h = 0; h < NC / 4; ++h) x[k][h] = q[h];
+                    }
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    if (i + k < i1) {
+#pragma unroll
+                        for (int h = 0; h < NC / 4; ++h) {
+                            s[4 * h] += x[k][h].x; s[4 * h + 1] += x[k][h].y;
+                            s[4 * h + 2] += x[k][h].z; s[4 * h + 3] += x[k][h].w;
+                        }
+                    }
+            }
+        } else {
+            for (int i = i0; i < i1; ++i) {
+                const float *q = P + (long long)inc[i] * NC;
 #pragma unroll
                 for (int ch = 0; ch < NC; ++ch) s[ch] += q[ch];
             }

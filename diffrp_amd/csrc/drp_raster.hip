@@ -687,9 +687,15 @@ __global__ void __launch_bounds__(256) k_gather_pos(const float4 *__restrict__ p
     int ia = pos_batched ? oimg : 0, ib = pos_batched ? oimg + 1 : n_img;
     for (int img = ia; img < ib; ++img) {
         const float4 *P = partial + (long long)img * F * 3;  // this vertex's corners are P[i0 .. i1)
-        for (int i = i0; i < i1; ++i) {
-            float4 q = P[i];
-            s.x += q.x; s.y += q.y; s.z += q.z; s.w += q.w;
+        // eight corners' loads in flight at once, summed in corner order
+        for (int i = i0; i < i1; i += 8) {
+            float4 q[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                if (i + k < i1) q[k] = P[i + k];
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                if (i + k < i1) { s.x += q[k].x; s.y += q[k].y; s.z += q[k].z; s.w += q[k].w; }
         }
     }
     float4 *o = grad_pos + (long long)oimg * V + v;
