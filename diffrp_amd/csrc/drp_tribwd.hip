@@ -294,7 +294,7 @@ __device__ __forceinline__ void gather_records(const TriBwdLaunch &L, int t, int
 }
 
 template <int MODE, int C>
-__global__ void __launch_bounds__(256) k_tile_gather(TriBwdLaunch L) {
+__global__ void __launch_bounds__(256, 6) k_tile_gather(TriBwdLaunch L) {  // 6 waves per SIMD: latency-bound searches
     constexpr int NV = MODE == 0 ? 12 : 3 * C;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int grp = blockIdx.y;
