@@ -82,7 +82,10 @@ __global__ void __launch_bounds__(256) k_tile_reduce(TriBwdLaunch L) {
         K run[kBTP];             // the waves' sorted runs (merge)
         int seg_start[kBTP + 1]; // segment starts (after the merge)
     } U;
-    __shared__ float contrib[NV * kRowStride];
+    // contributions: four components per float4, quad-major ([NV / 4][pixel]) when NV % 4 == 0,
+    // else component rows with a padded stride
+    constexpr bool QUAD = NV % 4 == 0;
+    __shared__ __align__(16) float contrib[QUAD ? NV * kBTP : NV * kRowStride];
     __shared__ int wsum[4];
     const int tid = threadIdx.x;
     const int tile = xcd_remap(blockIdx.x, L.n_tiles_pad);
@@ -172,8 +175,15 @@ __global__ void __launch_bounds__(256) k_tile_reduce(TriBwdLaunch L) {
             reinterpret_cast<float4 *>(L.grad_rast_out)[pix] = gr;
         }
     }
+    if (QUAD) {
+        float4 *c4 = reinterpret_cast<float4 *>(contrib);
 #pragma unroll
-    for (int j = 0; j < NV; ++j) contrib[j * kRowStride + tid] = val[j];
+        for (int j = 0; j < NV; j += 4)
+            c4[(j / 4) * kBTP + tid] = make_float4(val[j], val[j + 1], val[j + 2], val[j + 3]);
+    } else {
+#pragma unroll
+        for (int j = 0; j < NV; ++j) contrib[j * kRowStride + tid] = val[j];
+    }
     // sort the 256 keys (triangle << 8 | pixel; uncovered pixels last, keys unique):
     // each wave sorts its 64 across lanes, then every key's place is its lane
     // plus the keys below it in the other three runs
@@ -202,14 +212,31 @@ __global__ void __launch_bounds__(256) k_tile_reduce(TriBwdLaunch L) {
     const long long rbase = (long long)tile * kBTP;
     if (head) L.rec_t[rbase + seg] = (int)(sk >> 8);
     if (tid == 0) L.rec_count[tile] = nseg;
-    // one thread per (segment, component): consecutive threads -> consecutive record floats
-    for (int w = tid; w < nseg * NV; w += kBTP) {
-        int sg = w / NV, j = w - sg * NV;
-        int p0 = U.seg_start[sg], p1 = U.seg_start[sg + 1];
-        const float *row = contrib + j * kRowStride;
-        float s = row[(int)(key[p0] & 255)];
-        for (int p = p0 + 1; p < p1; ++p) s += row[(int)(key[p] & 255)];
-        L.rec_v[(rbase + sg) * NV + j] = s;
+    if (QUAD) {
+        // one thread per (segment, four components): each key read serves four sums (pixel order)
+        constexpr int NQ = NV / 4;
+        const float4 *c4 = reinterpret_cast<const float4 *>(contrib);
+        for (int w = tid; w < nseg * NQ; w += kBTP) {
+            const int sg = w / NQ, jq = w - sg * NQ;
+            const int p0 = U.seg_start[sg], p1 = U.seg_start[sg + 1];
+            const float4 *row = c4 + jq * kBTP;
+            float4 sum = row[(int)(key[p0] & 255)];
+            for (int p = p0 + 1; p < p1; ++p) {
+                const float4 x = row[(int)(key[p] & 255)];
+                sum.x += x.x; sum.y += x.y; sum.z += x.z; sum.w += x.w;
+            }
+            reinterpret_cast<float4 *>(L.rec_v + (rbase + sg) * NV)[jq] = sum;
+        }
+    } else {
+        // one thread per (segment, component): consecutive threads -> consecutive record floats
+        for (int w = tid; w < nseg * NV; w += kBTP) {
+            int sg = w / NV, j = w - sg * NV;
+            int p0 = U.seg_start[sg], p1 = U.seg_start[sg + 1];
+            const float *row = contrib + j * kRowStride;
+            float s = row[(int)(key[p0] & 255)];
+            for (int p = p0 + 1; p < p1; ++p) s += row[(int)(key[p] & 255)];
+            L.rec_v[(rbase + sg) * NV + j] = s;
+        }
     }
 }
 
