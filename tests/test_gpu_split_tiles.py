@@ -104,3 +104,17 @@ def test_split_tiles_in_uneven_batch(cuda):
             o, odb = oracle.rasterize(pos[i].numpy(), tri[i].numpy(), res[i], with_db=True)
             _assert_same(rasts[i].cpu().numpy(), o, f'tile {tile} item {i} rast')
             _assert_same(dbs[i].cpu().numpy(), odb, f'tile {tile} item {i} rast_db')
+
+
+def test_more_heavy_tiles_than_split_slots(cuda):
+    """Two crumpled s=8 views at 512² with 8x8 tiles: ~6,500 of the 8,192
+    tiles exceed the split threshold while at most 4,096 tiles are split per
+    call; the rest run whole in one block.  Both kinds must match the oracle."""
+    print('case: s=8 (1.3M triangles) x 2 views, 512^2, 8x8 tiles, more heavy tiles than split slots', flush=True)
+    res = (512, 512)
+    pos, tri, _ = noisy_clip(8, 0.02, res[0], views=2)
+    ref = oracle.rasterize(pos.numpy(), tri.numpy(), res)
+    ctx = nvdr.RasterizeCudaContext(tile=(8, 8), bin_capacity=8192)
+    rast, _ = nvdr.rasterize(ctx, pos.cuda(), tri.cuda(), res, grad_db=False)
+    _assert_same(rast.cpu().numpy(), ref, 'rast')
+    assert (ref[..., 3] > 0).mean() > 0.3
