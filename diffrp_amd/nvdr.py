@@ -94,10 +94,12 @@ class RasterizeCudaContext:
         if (self.tile is not None or self.bin_capacity is not None or key in self._tuned or key in self._pending
                 or len(self._pending) >= 64 or counts.numel() == 0 or torch.cuda.is_current_stream_capturing()):
             return  # (at most 64 shapes wait for their first read-back)
-        host = torch.empty(1, dtype=torch.int32, pin_memory=True)
-        host.copy_(counts.max().view(1), non_blocking=True)
-        ev = torch.cuda.Event()
-        ev.record()
+        st = torch.cuda.current_stream(counts.device)
+        with torch.cuda.stream(st):
+            host = torch.empty(1, dtype=torch.int32, pin_memory=True)
+            host.copy_(counts.max().view(1), non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record(st)
         self._pending[key] = (tile, cap, host, ev)
 
     def _settle(self, key):
