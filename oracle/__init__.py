@@ -51,7 +51,7 @@ def lib() -> ctypes.CDLL:
 
 
 def set_threads(n: int):
-    """Threads of the per-pixel forward loops (results do not depend on it)."""
+    """Threads of the oracle's loops (n <= 0: every processor); results do not depend on it."""
     lib().ora_set_threads(int(n))
 
 

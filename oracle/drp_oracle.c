@@ -82,10 +82,11 @@ float ora_triidx_to_float(int32_t n) {
 int32_t ora_float_to_triidx(float x) {
     return (x <= 16777216.0f) ? (int32_t)x : (int32_t)(f2u(x) - 0x4a800000u);
 }
-/* Threads of the per-pixel forward loops (rasterize_fixed, interpolate,
- * antialias); every pixel is computed on its own, so results do not depend
- * on it.  The other passes are sequential. */
-void ora_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+/* Threads of the oracle's loops.  Results do not depend on it: forward pixels
+ * are computed on their own, the depth test is an order-free minimum, and the
+ * backward passes sum into per-band buffers whose number is fixed by the
+ * image height and the buffer size, added in band order. */
+void ora_set_threads(int n) { omp_set_num_threads(n > 0 ? n : omp_get_num_procs()); }
 
 void ora_encode_ids(const int32_t *n, float *out, int64_t count) {
     for (int64_t i = 0; i < count; ++i) out[i] = ora_triidx_to_float(n[i]);
@@ -353,6 +354,26 @@ static inline const real *SFX(img_pos)(const real *pos, int pos_batched, int img
     return pos_batched ? pos + (int64_t)img * V * 4 : pos;
 }
 
+/* Deterministic parallel accumulation of the backward passes: the rows of an
+ * image are split into K bands (K fixed by H and the accumulator size, not by
+ * the thread count), each band sums its pixels' contributions, in row-major
+ * order, into a zeroed buffer of its own, and the bands are then added to the
+ * output in band order. */
+static int SFX(n_bands)(int H, int64_t n_acc) {
+    int64_t cap = ((int64_t)256 << 20) / (n_acc * (int64_t)sizeof(real) + 1);
+    int64_t K = H < 16 ? H : 16;
+    if (K > cap) K = cap;
+    return K < 1 ? 1 : (int)K;
+}
+static void SFX(add_bands)(real *out, const real *bands, int K, int64_t n_acc) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n_acc; ++i) {
+        real s = out[i];
+        for (int b = 0; b < K; ++b) s += bands[(int64_t)b * n_acc + i];
+        out[i] = s;
+    }
+}
+
 /* ------------------------------------------------------------------------- */
 /* rasterize forward                                                           */
 /* ------------------------------------------------------------------------- */
@@ -587,35 +608,46 @@ int SFX(ora_rasterize_bwd)(const real *pos, int n_img, int V, int pos_batched, c
                            real *grad_pos) {
     int64_t npx = (int64_t)H * W;
     const real kx = (real)2 / (real)W, ky = (real)2 / (real)H;
+    const int64_t n_acc = (int64_t)V * 4;
+    const int K = SFX(n_bands)(H, n_acc);
+    real *bands = (real *)malloc(sizeof(real) * (size_t)(K * n_acc + 1));
+    if (!bands) return -1;
     for (int img = 0; img < n_img; ++img) {
         const real *P = SFX(img_pos)(pos, pos_batched, img, V);
         real *G = pos_batched ? grad_pos + (int64_t)img * V * 4 : grad_pos;
-        for (int r = 0; r < H; ++r)
-            for (int c = 0; c < W; ++c) {
-                int64_t pi = (int64_t)img * npx + (int64_t)r * W + c;
-                int32_t id = rast[4 * pi + 3] > 0 ? SFX(decode_id)(rast[4 * pi + 3]) : 0;
-                if (id <= 0 || id > F) continue;
-                const int32_t *t3 = tri + 3 * (int64_t)(id - 1);
-                if (t3[0] < 0 || t3[0] >= V || t3[1] < 0 || t3[1] >= V || t3[2] < 0 || t3[2] >= V) continue;
-                double cx[3], cy[3], cz[3], cw[3];
-                for (int k = 0; k < 3; ++k) {
-                    const real *pv = P + 4 * (int64_t)t3[k];
-                    cx[k] = pv[0]; cy[k] = pv[1]; cz[k] = pv[2]; cw[k] = pv[3];
+        memset(bands, 0, sizeof(real) * (size_t)(K * n_acc));
+#pragma omp parallel for schedule(dynamic, 1)
+        for (int band = 0; band < K; ++band) {
+            real *B = bands + (int64_t)band * n_acc;
+            for (int r = (int)((int64_t)band * H / K); r < (int)((int64_t)(band + 1) * H / K); ++r)
+                for (int c = 0; c < W; ++c) {
+                    int64_t pi = (int64_t)img * npx + (int64_t)r * W + c;
+                    int32_t id = rast[4 * pi + 3] > 0 ? SFX(decode_id)(rast[4 * pi + 3]) : 0;
+                    if (id <= 0 || id > F) continue;
+                    const int32_t *t3 = tri + 3 * (int64_t)(id - 1);
+                    if (t3[0] < 0 || t3[0] >= V || t3[1] < 0 || t3[1] >= V || t3[2] < 0 || t3[2] >= V) continue;
+                    double cx[3], cy[3], cz[3], cw[3];
+                    for (int k = 0; k < 3; ++k) {
+                        const real *pv = P + 4 * (int64_t)t3[k];
+                        cx[k] = pv[0]; cy[k] = pv[1]; cz[k] = pv[2]; cw[k] = pv[3];
+                    }
+                    double gd[4], go[3][4];
+                    if (grad_db) {
+                        const real *q = grad_db + 4 * pi;
+                        gd[0] = (double)(q[0] * kx); gd[1] = (double)(q[1] * ky);
+                        gd[2] = (double)(q[2] * kx); gd[3] = (double)(q[3] * ky);
+                    }
+                    pix_grad_d(cx, cy, cz, cw, r, c, H, W, grad_rast[4 * pi + 0], grad_rast[4 * pi + 1],
+                               grad_rast[4 * pi + 2], grad_db ? gd : NULL, go);
+                    for (int k = 0; k < 3; ++k) {
+                        real *g = B + 4 * (int64_t)t3[k];
+                        for (int j = 0; j < 4; ++j) g[j] += (real)go[k][j];
+                    }
                 }
-                double gd[4], go[3][4];
-                if (grad_db) {
-                    const real *q = grad_db + 4 * pi;
-                    gd[0] = (double)(q[0] * kx); gd[1] = (double)(q[1] * ky);
-                    gd[2] = (double)(q[2] * kx); gd[3] = (double)(q[3] * ky);
-                }
-                pix_grad_d(cx, cy, cz, cw, r, c, H, W, grad_rast[4 * pi + 0], grad_rast[4 * pi + 1],
-                           grad_rast[4 * pi + 2], grad_db ? gd : NULL, go);
-                for (int k = 0; k < 3; ++k) {
-                    real *g = G + 4 * (int64_t)t3[k];
-                    for (int j = 0; j < 4; ++j) g[j] += (real)go[k][j];
-                }
-            }
+        }
+        SFX(add_bands)(G, bands, K, n_acc);
     }
+    free(bands);
     return 0;
 }
 
@@ -672,54 +704,65 @@ int SFX(ora_interpolate_bwd)(const real *attr, int n_img, int V, int attr_batche
                              int n_diff, const real *grad_out, const real *grad_da, real *grad_attr,
                              real *grad_rast, real *grad_db) {
     int64_t npx = (int64_t)H * W;
+    const int64_t n_acc = (int64_t)V * C;
+    const int K = SFX(n_bands)(H, n_acc);
+    real *bands = (real *)malloc(sizeof(real) * (size_t)(K * n_acc + 1));
+    if (!bands) return -1;
     for (int img = 0; img < n_img; ++img) {
         const real *A = attr_batched ? attr + (int64_t)img * V * C : attr;
         real *GA = attr_batched ? grad_attr + (int64_t)img * V * C : grad_attr;
-        for (int64_t q = 0; q < npx; ++q) {
-            int64_t pi = (int64_t)img * npx + q;
-            real *gr = grad_rast + 4 * pi;
-            gr[0] = gr[1] = gr[2] = gr[3] = 0;
-            if (grad_db) grad_db[4 * pi] = grad_db[4 * pi + 1] = grad_db[4 * pi + 2] = grad_db[4 * pi + 3] = 0;
-            int32_t id = rast[4 * pi + 3] > 0 ? SFX(decode_id)(rast[4 * pi + 3]) : 0;
-            const int32_t *t3 = (id > 0 && id <= F) ? tri + 3 * (int64_t)(id - 1) : NULL;
-            if (t3 && (t3[0] < 0 || t3[0] >= V || t3[1] < 0 || t3[1] >= V || t3[2] < 0 || t3[2] >= V)) t3 = NULL;
-            if (!t3) continue;
-            real u = rast[4 * pi], v = rast[4 * pi + 1];
-            real w2 = ((real)1 - u) - v;
-            const real *a0 = A + (int64_t)t3[0] * C, *a1 = A + (int64_t)t3[1] * C, *a2 = A + (int64_t)t3[2] * C;
-            real *g0 = GA + (int64_t)t3[0] * C, *g1 = GA + (int64_t)t3[1] * C, *g2 = GA + (int64_t)t3[2] * C;
-            const real *g = grad_out + pi * C;
-            real su = 0, sv = 0;
-            for (int ch = 0; ch < C; ++ch) {
-                g0[ch] += u * g[ch];
-                g1[ch] += v * g[ch];
-                g2[ch] += w2 * g[ch];
-                su += g[ch] * (a0[ch] - a2[ch]);
-                sv += g[ch] * (a1[ch] - a2[ch]);
-            }
-            gr[0] = su;
-            gr[1] = sv;
-            if (grad_da && diff_idx && n_diff > 0) {
-                const real *d = db + 4 * pi;
-                const real *gd = grad_da + pi * 2 * n_diff;
-                real s0 = 0, s1 = 0, s2 = 0, s3 = 0;
-                for (int j = 0; j < n_diff; ++j) {
-                    int ch = diff_idx[j];
-                    real gx = gd[2 * j], gy = gd[2 * j + 1];
-                    real d0 = a0[ch] - a2[ch], d1 = a1[ch] - a2[ch];
-                    real k0 = gx * d[0] + gy * d[1], k1 = gx * d[2] + gy * d[3];
-                    g0[ch] += k0;
-                    g1[ch] += k1;
-                    g2[ch] -= k0 + k1;
-                    s0 += gx * d0; s1 += gy * d0; s2 += gx * d1; s3 += gy * d1;
+        memset(bands, 0, sizeof(real) * (size_t)(K * n_acc));
+#pragma omp parallel for schedule(dynamic, 1)
+        for (int band = 0; band < K; ++band) {
+            real *B = bands + (int64_t)band * n_acc;
+            for (int64_t q = (int64_t)band * H / K * W; q < (int64_t)(band + 1) * H / K * W; ++q) {
+                int64_t pi = (int64_t)img * npx + q;
+                real *gr = grad_rast + 4 * pi;
+                gr[0] = gr[1] = gr[2] = gr[3] = 0;
+                if (grad_db) grad_db[4 * pi] = grad_db[4 * pi + 1] = grad_db[4 * pi + 2] = grad_db[4 * pi + 3] = 0;
+                int32_t id = rast[4 * pi + 3] > 0 ? SFX(decode_id)(rast[4 * pi + 3]) : 0;
+                const int32_t *t3 = (id > 0 && id <= F) ? tri + 3 * (int64_t)(id - 1) : NULL;
+                if (t3 && (t3[0] < 0 || t3[0] >= V || t3[1] < 0 || t3[1] >= V || t3[2] < 0 || t3[2] >= V)) t3 = NULL;
+                if (!t3) continue;
+                real u = rast[4 * pi], v = rast[4 * pi + 1];
+                real w2 = ((real)1 - u) - v;
+                const real *a0 = A + (int64_t)t3[0] * C, *a1 = A + (int64_t)t3[1] * C, *a2 = A + (int64_t)t3[2] * C;
+                real *g0 = B + (int64_t)t3[0] * C, *g1 = B + (int64_t)t3[1] * C, *g2 = B + (int64_t)t3[2] * C;
+                const real *g = grad_out + pi * C;
+                real su = 0, sv = 0;
+                for (int ch = 0; ch < C; ++ch) {
+                    g0[ch] += u * g[ch];
+                    g1[ch] += v * g[ch];
+                    g2[ch] += w2 * g[ch];
+                    su += g[ch] * (a0[ch] - a2[ch]);
+                    sv += g[ch] * (a1[ch] - a2[ch]);
                 }
-                if (grad_db) {
-                    grad_db[4 * pi + 0] = s0; grad_db[4 * pi + 1] = s1;
-                    grad_db[4 * pi + 2] = s2; grad_db[4 * pi + 3] = s3;
+                gr[0] = su;
+                gr[1] = sv;
+                if (grad_da && diff_idx && n_diff > 0) {
+                    const real *d = db + 4 * pi;
+                    const real *gd = grad_da + pi * 2 * n_diff;
+                    real s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+                    for (int j = 0; j < n_diff; ++j) {
+                        int ch = diff_idx[j];
+                        real gx = gd[2 * j], gy = gd[2 * j + 1];
+                        real d0 = a0[ch] - a2[ch], d1 = a1[ch] - a2[ch];
+                        real k0 = gx * d[0] + gy * d[1], k1 = gx * d[2] + gy * d[3];
+                        g0[ch] += k0;
+                        g1[ch] += k1;
+                        g2[ch] -= k0 + k1;
+                        s0 += gx * d0; s1 += gy * d0; s2 += gx * d1; s3 += gy * d1;
+                    }
+                    if (grad_db) {
+                        grad_db[4 * pi + 0] = s0; grad_db[4 * pi + 1] = s1;
+                        grad_db[4 * pi + 2] = s2; grad_db[4 * pi + 3] = s3;
+                    }
                 }
             }
         }
+        SFX(add_bands)(GA, bands, K, n_acc);
     }
+    free(bands);
     return 0;
 }
 
@@ -867,11 +910,22 @@ int SFX(ora_antialias)(const real *color, int n_img, int H, int W, int C, const 
     return 0;
 }
 
-/* grad_color written; grad_pos accumulated (+=) with pos's layout. */
+/* grad_color written; grad_pos accumulated (+=) with pos's layout.
+ * Pass 1 evaluates the pairs to the right of and above every pixel (bands of
+ * rows in parallel, position gradients into per-band buffers) and keeps each
+ * pair's decision; pass 2 forms every pixel's colour gradient from the four
+ * pairs it belongs to, in the order (left, p), (p, right), (below, p),
+ * (p, above). */
 int SFX(ora_antialias_bwd)(const real *color, int n_img, int H, int W, int C, const real *rast, const real *pos,
                            int V, int pos_batched, const int32_t *tri, int F, const int32_t *opp,
                            const real *grad_out, real *grad_color, real *grad_pos, real boost) {
     int64_t npx = (int64_t)H * W;
+    const int64_t n_acc = (int64_t)V * 4;
+    const int K = SFX(n_bands)(H, n_acc);
+    real *bands = (real *)malloc(sizeof(real) * (size_t)(K * n_acc + 1));
+    signed char *prc = (signed char *)malloc((size_t)(2 * npx + 1));
+    real *pw = (real *)malloc(sizeof(real) * (size_t)(2 * npx + 1));
+    if (!bands || !prc || !pw) { free(bands); free(prc); free(pw); return -1; }
     for (int img = 0; img < n_img; ++img) {
         const real *P = SFX(img_pos)(pos, pos_batched, img, V);
         real *GP = pos_batched ? grad_pos + (int64_t)img * V * 4 : grad_pos;
@@ -879,67 +933,97 @@ int SFX(ora_antialias_bwd)(const real *color, int n_img, int H, int W, int C, co
         const real *Cc = color + (int64_t)img * npx * C;
         const real *Go = grad_out + (int64_t)img * npx * C;
         real *Gc = grad_color + (int64_t)img * npx * C;
-        for (int64_t q = 0; q < npx * C; ++q) Gc[q] = Go[q];
-        for (int dir = 0; dir < 2; ++dir)
-            for (int ra = 0; ra < H; ++ra)
-                for (int ca = 0; ca < W; ++ca) {
-                    int rb = dir ? ra + 1 : ra, cb = dir ? ca : ca + 1;
-                    if (rb >= H || cb >= W) continue;
-                    SFX(ora_pair) pr;
-                    SFX(eval_pair)(P, V, tri, F, opp, R, H, W, ra, ca, rb, cb, &pr);
-                    if (pr.recv < 0) continue;
-                    int64_t qa = (int64_t)ra * W + ca, qb = (int64_t)rb * W + cb;
-                    int64_t qr = pr.recv == 0 ? qa : qb, qs = pr.recv == 0 ? qb : qa;
-                    double dLdw = 0;
-                    for (int ch = 0; ch < C; ++ch) {
-                        real g = Go[qr * C + ch];
-                        Gc[qs * C + ch] += pr.wgt * g;
-                        Gc[qr * C + ch] -= pr.wgt * g;
-                        dLdw += (double)g * (double)(Cc[qs * C + ch] - Cc[qr * C + ch]);
+        memset(bands, 0, sizeof(real) * (size_t)(K * n_acc));
+#pragma omp parallel for schedule(dynamic, 1)
+        for (int band = 0; band < K; ++band) {
+            real *GB = bands + (int64_t)band * n_acc;
+            const int rlo = (int)((int64_t)band * H / K), rhi = (int)((int64_t)(band + 1) * H / K);
+            for (int dir = 0; dir < 2; ++dir)
+                for (int ra = rlo; ra < rhi; ++ra)
+                    for (int ca = 0; ca < W; ++ca) {
+                        int rb = dir ? ra + 1 : ra, cb = dir ? ca : ca + 1;
+                        int64_t qa = (int64_t)ra * W + ca, qb = (int64_t)rb * W + cb;
+                        prc[dir * npx + qa] = -1;
+                        if (rb >= H || cb >= W) continue;
+                        SFX(ora_pair) pr;
+                        SFX(eval_pair)(P, V, tri, F, opp, R, H, W, ra, ca, rb, cb, &pr);
+                        if (pr.recv < 0) continue;
+                        prc[dir * npx + qa] = (signed char)pr.recv;
+                        pw[dir * npx + qa] = pr.wgt;
+                        int64_t qr = pr.recv == 0 ? qa : qb, qs = pr.recv == 0 ? qb : qa;
+                        double dLdw = 0;
+                        for (int ch = 0; ch < C; ++ch)
+                            dLdw += (double)Go[qr * C + ch] * (double)(Cc[qs * C + ch] - Cc[qr * C + ch]);
+                        if (pr.clamped) continue;
+                        /* The decisions above are the forward's (in `real`); the
+                         * derivative of the crossing t = fa / (fa - fb) is taken in
+                         * double from the clip coordinates in both variants: it is
+                         * ill-conditioned for edges nearly parallel to the pair. */
+                        /* w = 1/2 - t when the receiver is `from`, t - 1/2 otherwise */
+                        int recv_is_from = (pr.recv == 0) == (pr.from_a != 0);
+                        double dLdt = recv_is_from ? -dLdw : dLdw;
+                        const int32_t *t3 = tri + 3 * (int64_t)pr.tri;
+                        int o = SFX(orient)(P, V, t3);
+                        double so = o < 0 ? -1.0 : 1.0;
+                        int i = pr.edge, j = (pr.edge + 1) % 3;
+                        int vv[2] = {t3[i], t3[j]};
+                        double hw = (double)W * 0.5, hh = (double)H * 0.5;
+                        double cx[2], cy[2], cw[2], qx[2], qy[2];
+                        for (int m = 0; m < 2; ++m) {
+                            const real *pv = P + 4 * (int64_t)vv[m];
+                            cx[m] = pv[0]; cy[m] = pv[1]; cw[m] = pv[3];
+                            qx[m] = (cx[m] / cw[m] + 1.0) * hw;
+                            qy[m] = (cy[m] / cw[m] + 1.0) * hh;
+                        }
+                        double fxp = (double)(pr.from_a ? ca : cb) + 0.5, fyp = (double)(pr.from_a ? ra : rb) + 0.5;
+                        double txp = (double)(pr.from_a ? cb : ca) + 0.5, typ = (double)(pr.from_a ? rb : ra) + 0.5;
+                        double ex = qx[1] - qx[0], ey = qy[1] - qy[0];
+                        double fa = so * (ex * (fyp - qy[0]) - ey * (fxp - qx[0]));
+                        double fb = so * (ex * (typ - qy[0]) - ey * (txp - qx[0]));
+                        double den = fa - fb;
+                        double dt_dfa = -fb / (den * den), dt_dfb = fa / (den * den);
+                        /* d f / d(Xi, Yi, Xj, Yj) at P: so*(Yj-PY), so*(PX-Xj), so*(PY-Yi), so*(Xi-PX) */
+                        double k = dLdt * so * (double)boost;
+                        double gx2[2], gy2[2];
+                        gx2[0] = k * (dt_dfa * (qy[1] - fyp) + dt_dfb * (qy[1] - typ));
+                        gy2[0] = k * (dt_dfa * (fxp - qx[1]) + dt_dfb * (txp - qx[1]));
+                        gx2[1] = k * (dt_dfa * (fyp - qy[0]) + dt_dfb * (typ - qy[0]));
+                        gy2[1] = k * (dt_dfa * (qx[0] - fxp) + dt_dfb * (qx[0] - txp));
+                        for (int m = 0; m < 2; ++m) {
+                            real *g = GB + 4 * (int64_t)vv[m];
+                            double gx = gx2[m] * hw / cw[m], gy = gy2[m] * hh / cw[m];
+                            g[0] += (real)gx;
+                            g[1] += (real)gy;
+                            g[3] += (real)(-(gx * cx[m] + gy * cy[m]) / cw[m]);
+                        }
                     }
-                    if (pr.clamped) continue;
-                    /* The decisions above are the forward's (in `real`); the
-                     * derivative of the crossing t = fa / (fa - fb) is taken in
-                     * double from the clip coordinates in both variants: it is
-                     * ill-conditioned for edges nearly parallel to the pair. */
-                    /* w = 1/2 - t when the receiver is `from`, t - 1/2 otherwise */
-                    int recv_is_from = (pr.recv == 0) == (pr.from_a != 0);
-                    double dLdt = recv_is_from ? -dLdw : dLdw;
-                    const int32_t *t3 = tri + 3 * (int64_t)pr.tri;
-                    int o = SFX(orient)(P, V, t3);
-                    double so = o < 0 ? -1.0 : 1.0;
-                    int i = pr.edge, j = (pr.edge + 1) % 3;
-                    int vv[2] = {t3[i], t3[j]};
-                    double hw = (double)W * 0.5, hh = (double)H * 0.5;
-                    double cx[2], cy[2], cw[2], qx[2], qy[2];
-                    for (int m = 0; m < 2; ++m) {
-                        const real *pv = P + 4 * (int64_t)vv[m];
-                        cx[m] = pv[0]; cy[m] = pv[1]; cw[m] = pv[3];
-                        qx[m] = (cx[m] / cw[m] + 1.0) * hw;
-                        qy[m] = (cy[m] / cw[m] + 1.0) * hh;
-                    }
-                    double fxp = (double)(pr.from_a ? ca : cb) + 0.5, fyp = (double)(pr.from_a ? ra : rb) + 0.5;
-                    double txp = (double)(pr.from_a ? cb : ca) + 0.5, typ = (double)(pr.from_a ? rb : ra) + 0.5;
-                    double ex = qx[1] - qx[0], ey = qy[1] - qy[0];
-                    double fa = so * (ex * (fyp - qy[0]) - ey * (fxp - qx[0]));
-                    double fb = so * (ex * (typ - qy[0]) - ey * (txp - qx[0]));
-                    double den = fa - fb;
-                    double dt_dfa = -fb / (den * den), dt_dfb = fa / (den * den);
-                    /* d f / d(Xi, Yi, Xj, Yj) at P: so*(Yj-PY), so*(PX-Xj), so*(PY-Yi), so*(Xi-PX) */
-                    double k = dLdt * so * (double)boost;
-                    double gx2[2], gy2[2];
-                    gx2[0] = k * (dt_dfa * (qy[1] - fyp) + dt_dfb * (qy[1] - typ));
-                    gy2[0] = k * (dt_dfa * (fxp - qx[1]) + dt_dfb * (txp - qx[1]));
-                    gx2[1] = k * (dt_dfa * (fyp - qy[0]) + dt_dfb * (typ - qy[0]));
-                    gy2[1] = k * (dt_dfa * (qx[0] - fxp) + dt_dfb * (qx[0] - txp));
-                    for (int m = 0; m < 2; ++m) {
-                        real *g = GP + 4 * (int64_t)vv[m];
-                        double gx = gx2[m] * hw / cw[m], gy = gy2[m] * hh / cw[m];
-                        g[0] += (real)gx;
-                        g[1] += (real)gy;
-                        g[3] += (real)(-(gx * cx[m] + gy * cy[m]) / cw[m]);
-                    }
+        }
+        SFX(add_bands)(GP, bands, K, n_acc);
+#pragma omp parallel for schedule(static)
+        for (int r = 0; r < H; ++r)
+            for (int c = 0; c < W; ++c) {
+                int64_t q = (int64_t)r * W + c;
+                for (int ch = 0; ch < C; ++ch) Gc[q * C + ch] = Go[q * C + ch];
+                /* (left, p), (p, right), (below, p), (p, above): pair index (dir, first pixel) */
+                int64_t first[4] = {q - 1, q, q - W, q};
+                int dirs[4] = {0, 0, 1, 1};
+                int ok[4] = {c > 0, c + 1 < W, r > 0, r + 1 < H};
+                for (int k = 0; k < 4; ++k) {
+                    if (!ok[k]) continue;
+                    int64_t qa = first[k], qb = dirs[k] ? qa + W : qa + 1;
+                    int rc = prc[dirs[k] * npx + qa];
+                    if (rc < 0) continue;
+                    real w = pw[dirs[k] * npx + qa];
+                    int64_t qr = rc == 0 ? qa : qb;
+                    if (q == qr)
+                        for (int ch = 0; ch < C; ++ch) Gc[q * C + ch] -= w * Go[qr * C + ch];
+                    else
+                        for (int ch = 0; ch < C; ++ch) Gc[q * C + ch] += w * Go[qr * C + ch];
                 }
+            }
     }
+    free(bands);
+    free(prc);
+    free(pw);
     return 0;
 }
