@@ -21,6 +21,7 @@ Inputs are resident in HBM before the timed region.
 Prints ONE JSON line.  ``--gpus`` other than 1 exits: this build measures one GPU.
 """
 import argparse
+import faulthandler
 import json
 import os
 import platform
@@ -34,6 +35,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from diffrp_amd import autotune, nvdr  # noqa: E402
+from diffrp_amd.loss import l2_loss  # noqa: E402
 from diffrp_amd.camera import PerspectiveCamera, clip_space  # noqa: E402
 from diffrp_amd.geometry import fibonacci_directions, noisy_icosphere  # noqa: E402
 
@@ -62,7 +64,7 @@ def op_bytes(op, n, N, V, F, C):
 
 
 class Scene:
-    def __init__(self, dev, views, res, subdiv, C, sigma=0.01, radius=0.8):
+    def __init__(self, dev, views, res, subdiv, C, sigma=0.01, radius=0.8, cpu_only=False):
         verts, tri = noisy_icosphere(subdiv, radius, sigma, seed=0)
         self.V, self.F = verts.shape[0], tri.shape[0]
         self.H = self.W = res
@@ -76,6 +78,10 @@ class Scene:
         for d in fibonacci_directions(views):
             cam = PerspectiveCamera.looking_at_origin_from(d * 3.2, h=res, w=res)
             VP.append(cam.P('cpu') @ cam.V('cpu'))
+        self.verts_cpu, self.tri_cpu, self.attr_cpu, self.VP_cpu = verts, tri, attr, torch.stack(VP)
+        self.target_cpu = target
+        if cpu_only:
+            return
         self.verts = verts.to(dev).requires_grad_(True)
         self.attr = attr.to(dev).requires_grad_(True)
         self.tri = tri.to(dev)
@@ -83,16 +89,18 @@ class Scene:
         self.target = target.to(dev)
         self.ctx = nvdr.RasterizeCudaContext(dev)
         self.topo = nvdr.antialias_construct_topology_hash(self.tri)
-        self.verts_cpu, self.tri_cpu, self.attr_cpu, self.VP_cpu = verts, tri, attr, torch.stack(VP)
-        self.target_cpu = target
 
     def step_batched(self):
         pos = clip_space(self.verts, self.VP)
         rast, _ = nvdr.rasterize(self.ctx, pos, self.tri, (self.H, self.W), grad_db=False)
         col, _ = nvdr.interpolate(self.attr, rast, self.tri)
         y = nvdr.antialias(col, rast, pos, self.tri, topology_hash=self.topo)
-        loss = torch.nn.functional.mse_loss(y, self.target, reduction='sum')
+        loss = l2_loss(y, self.target)
         loss.backward()
+        # the step's buffers, detached (a captured step's buffers hold each replay's values; keeping
+        # the tensors themselves would keep this step's autograd graph, and the AccumulateGrad
+        # nodes of the leaves on the stream that made them, alive into the next step or capture)
+        self.last = (pos.detach(), rast.detach(), col.detach(), y.detach())
         return loss
 
     def step_plain(self):
@@ -102,7 +110,7 @@ class Scene:
             rast, _ = nvdr.rasterize(self.ctx, pos, self.tri, (self.H, self.W), grad_db=False)
             col, _ = nvdr.interpolate(self.attr, rast, self.tri)
             y = nvdr.antialias(col, rast, pos, self.tri)
-            loss = torch.nn.functional.mse_loss(y, self.target[v:v + 1], reduction='sum')
+            loss = l2_loss(y, self.target[v:v + 1])
             loss.backward()
             total = total + loss.detach()
         return total
@@ -113,6 +121,7 @@ class Scene:
 
 
 def time_region(fn, steps, warmup):
+    """Wall-clock mean of ``steps`` calls after ``warmup`` untimed ones, synchronised on both sides."""
     for _ in range(warmup):
         fn()
     torch.cuda.synchronize()
@@ -121,6 +130,25 @@ def time_region(fn, steps, warmup):
         fn()
     torch.cuda.synchronize()
     return (time.perf_counter() - t0) / steps
+
+
+def time_steps(fn, steps, warmup):
+    """``warmup`` untimed calls, then exactly ``steps`` timed ones between two
+    device synchronisations, each bracketed by HIP events on the current
+    stream.  Returns (median event time, wall-clock mean) in seconds."""
+    for _ in range(warmup):
+        fn()
+    st = torch.cuda.current_stream()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for e0, e1 in evs:
+        e0.record(st)
+        fn()
+        e1.record(st)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / steps
+    return float(np.median([e0.elapsed_time(e1) for e0, e1 in evs])) * 1e-3, wall
 
 
 def time_events(fn, iters):
@@ -225,34 +253,84 @@ def measured_traffic(op, S, subdiv):
     return None
 
 
+def host_threads():
+    """Host threads this process may use: OMP_NUM_THREADS when the environment
+    sets it (the CPU share of a shared host), else every logical CPU."""
+    n = os.cpu_count() or 1
+    try:
+        return max(1, min(n, int(os.environ.get('OMP_NUM_THREADS', n))))
+    except ValueError:
+        return n
+
+
+def cpu_model():
+    try:
+        with open('/proc/cpuinfo') as f:
+            for line in f:
+                if line.startswith('model name'):
+                    return line.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
+def _cpu_step(oracle, pos, tri, opp, attr, target, H, W, backward):
+    """One view through the oracle: rasterize, interpolate, antialias, L2 (and its backward)."""
+    rast = oracle.rasterize(pos, tri, (H, W))
+    col = oracle.interpolate(attr, rast, tri)
+    y = oracle.antialias(col, rast, pos, tri, opp)
+    if not backward:
+        return y
+    gy = 2.0 * (y - target)
+    gcol, gpos_aa = oracle.antialias_bwd(col, rast, pos, tri, gy, opp)
+    gattr, grast = oracle.interpolate_bwd(attr, rast, tri, gcol)
+    gpos = oracle.rasterize_bwd(pos, tri, rast, grast) + gpos_aa
+    return gpos, gattr
+
+
 def cpu_baseline(S, views):
-    """The CPU oracle (scalar C restatement + torch-CPU loss) on a bounded sample of views."""
+    """The CPU oracle (the C restatement, OpenMP over the host threads this
+    process may use, plus the torch-CPU clip transform and loss) on the same workload: ``views`` of the
+    64 views fwd+bwd (the headline), and the single-view configurations C1
+    (README-size mesh at 256^2, forward) and C2 (the s=6 mesh at 1024^2,
+    fwd+bwd)."""
     import oracle
     oracle.build()
-    oracle.set_threads(1)
+    cores = host_threads()
+    oracle.set_threads(cores)
+    torch.set_num_threads(cores)
     tri = S.tri_cpu.numpy()
     opp = oracle.topology(tri)
     attr = S.attr_cpu.numpy()
     t0 = time.perf_counter()
     for v in range(views):
         pos = clip_space(S.verts_cpu, S.VP_cpu[v]).numpy()
-        rast = oracle.rasterize(pos, tri, (S.H, S.W))
-        col = oracle.interpolate(attr, rast, tri)
-        y = oracle.antialias(col, rast, pos, tri, opp)
-        gy = 2.0 * (y - S.target_cpu[v:v + 1].numpy())
-        gcol, gpos_aa = oracle.antialias_bwd(col, rast, pos, tri, gy, opp)
-        gattr, grast = oracle.interpolate_bwd(attr, rast, tri, gcol)
-        gpos = oracle.rasterize_bwd(pos, tri, rast, grast) + gpos_aa
-        _ = gpos.sum() + gattr.sum()
+        gpos, gattr = _cpu_step(oracle, pos, tri, opp, attr, S.target_cpu[v:v + 1].numpy(), S.H, S.W, True)
+        _ = float(gpos.sum()) + float(gattr.sum())
     dt = time.perf_counter() - t0
+    # C1: forward of the README-size mesh at 256^2, 3 channels
+    S1 = Scene('cpu', 1, 256, 3, 3, sigma=0.0, cpu_only=True)
+    p1 = clip_space(S1.verts_cpu, S1.VP_cpu[0]).numpy()
+    t1 = time.perf_counter()
+    for _ in range(5):
+        _cpu_step(oracle, p1, S1.tri_cpu.numpy(), None, S1.attr_cpu.numpy(), None, 256, 256, False)
+    t1 = (time.perf_counter() - t1) / 5
+    # C2: the s=6 mesh at 1024^2, 8 channels, fwd+bwd
+    S2 = Scene('cpu', 1, 1024, 6, 8, cpu_only=True)
+    p2 = clip_space(S2.verts_cpu, S2.VP_cpu[0]).numpy()
+    t2 = time.perf_counter()
+    _cpu_step(oracle, p2, S2.tri_cpu.numpy(), opp, S2.attr_cpu.numpy(), S2.target_cpu.numpy(), 1024, 1024, True)
+    t2 = time.perf_counter() - t2
     return {
         'value': views * S.H * S.W / dt / 1e6,
         'unit': 'Mpixel/s',
-        'cores': 1,
+        'cores': cores,
         'kind': 'port',
-        'sample': f'{views} of the 64 views at {S.H}x{S.W}, fwd+bwd, scalar C oracle on one core '
-                  f'({dt:.1f} s; host {platform.processor() or platform.machine()}, '
-                  f'{os.cpu_count()} logical cpus)',
+        'sample': f'{views} of the 64 views at {S.H}x{S.W}, fwd+bwd, C oracle with OpenMP on {cores} threads '
+                  f'+ torch-CPU ({dt:.1f} s; {cpu_model()})',
+        'cpu_model': cpu_model(),
+        'C1_fwd_ms': round(t1 * 1e3, 2),
+        'C2_fwd_bwd_mpix_s': round(1024 * 1024 / t2 / 1e6, 3),
     }
 
 
@@ -292,10 +370,153 @@ def other_configs(dev):
     res['C2'] = {'workload': 'noisy icosphere s=6 (40962 verts, 81920 faces), 1024x1024, 8 channels, fwd+bwd',
                  'eager_ms': round(t2 * 1e3, 4), 'graph_ms': round(t2g * 1e3, 4),
                  'mpix_s': round(1024 * 1024 / t2g / 1e6, 1)}
+    del S2, g
+    progress('C1, C2 timed')
+    res['C4'] = uneven_config(dev)
+    progress('C4 timed')
+    res['C5'] = layered_config(dev)
+    progress('C5 timed')
     return res
 
 
+def capture(step):
+    """``step`` captured in a HIP graph after two warm-up runs on a side stream."""
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        for _ in range(2):
+            step()
+    torch.cuda.current_stream().wait_stream(side)
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        step()
+    return g
+
+
+def uneven_draw(seed=3, n=32, C=4, sigma=0.02):
+    """The uneven batch: ``n`` icospheres of subdivision uniform in 3..8 with
+    N(0, sigma) vertex noise, each at a resolution drawn from {256, 512, 1024,
+    2048}^2, seen from Fibonacci directions at distance 3.2 (draws from ``seed``)."""
+    rng = np.random.default_rng(seed)
+    subdivs = rng.integers(3, 9, n)
+    sizes = rng.choice([256, 512, 1024, 2048], n)
+    dirs = fibonacci_directions(n)
+    pos, tri, res = [], [], []
+    for i, (sd, sz) in enumerate(zip(subdivs, sizes)):
+        v, f = noisy_icosphere(int(sd), 0.8, sigma, seed=int(rng.integers(1 << 30)))
+        cam = PerspectiveCamera.looking_at_origin_from(np.asarray(dirs[i]) * 3.2, h=int(sz), w=int(sz))
+        pos.append(clip_space(v, cam.P('cpu') @ cam.V('cpu')).contiguous())
+        tri.append(f.int().contiguous())
+        res.append((int(sz), int(sz)))
+    return pos, tri, res
+
+
+def uneven_config(dev, steps=10):
+    """C4: the 32-mesh uneven batch (seed 3, C=4), tuned, fwd+bwd of rasterize,
+    interpolate, antialias and an L2 loss as one captured HIP graph of batched
+    launches, against a loop of per-item eager steps."""
+    from diffrp_amd import batch
+    pos, tri, res = uneven_draw()
+    ctx = nvdr.RasterizeCudaContext(dev)
+    tune = batch.tune_batch(ctx, [p.to(dev) for p in pos], [t.to(dev) for t in tri], res)
+    ub = batch.UnevenBatch(ctx, [t.to(dev) for t in tri], [p.shape[0] for p in pos], res)
+    g = torch.Generator().manual_seed(6)
+    P = torch.cat(pos).to(dev).contiguous().requires_grad_(True)
+    A = torch.rand(P.shape[0], 4, generator=g).to(dev).requires_grad_(True)
+    T = torch.rand(ub.n_pix, 4, generator=g).to(dev)
+
+    def step():
+        P.grad = None
+        A.grad = None
+        rast, boxes = ub.rasterize(P)
+        out = ub.antialias(ub.interpolate(A, rast, boxes), rast, P)
+        l2_loss(out, T).backward()
+    graph = capture(step)
+    t, wall = time_steps(graph.replay, steps, 2)
+    items = [(p.shape[0], t_.shape[0], h * w) for p, t_, (h, w) in zip(pos, tri, res)]
+    npx = sum(n for _, _, n in items)
+    nf = sum(f for _, f, _ in items)
+    alg = sum(240 * n + 128 * v + 72 * f for v, f, n in items)
+    # the per-item loop: one eager drop-in step per mesh
+    pcs = [p.to(dev).requires_grad_(True) for p in pos]
+    tcs = [t_.to(dev) for t_ in tri]
+    acs = [A.detach()[int(ub.plan.vbase[i]):int(ub.plan.vbase[i]) + pos[i].shape[0]].clone().requires_grad_(True)
+           for i in range(len(pos))]
+    tgs = ub.split(T)
+
+    def loop():
+        for p, t_, a, tg, r in zip(pcs, tcs, acs, tgs, res):
+            rast, _ = nvdr.rasterize(ctx, p, t_, r, grad_db=False)
+            y = nvdr.antialias(nvdr.interpolate(a, rast, t_)[0], rast, p, t_)
+            l2_loss(y, tg).backward()
+    t_loop = time_region(loop, 3, 1)
+    return {'workload': f'{len(pos)} noisy icospheres s=3..8 ({nf} faces), 256^2..2048^2 ({npx} pixels), '
+                        f'4 channels, fwd+bwd, batched launches in one HIP graph',
+            'ms': round(t * 1e3, 4), 'wall_ms': round(wall * 1e3, 4), 'mpix_s': round(npx / t / 1e6, 1),
+            'mtri_s': round(nf / t / 1e6, 1), 'algorithmic_bytes': alg,
+            'roofline_frac': round(alg / t / 1e9 / HBM_PEAK_GBS, 4),
+            'per_item_loop_ms': round(t_loop * 1e3, 3), 'batched_vs_loop': round(t_loop / t, 2),
+            'autotune': {'tile': list(tune['tile']), 'candidates_ms': tune['candidates_ms']}}
+
+
+def layered_config(dev):
+    """C5: icosphere s=9 (5,242,880 faces, N(0, 0.005) noise), 2048^2, 4 peeled
+    layers composited, 16 channels, antialias, fwd+bwd of an L2 loss: with each
+    layer recomputed in the backward pass, and with the layers' buffers kept."""
+    from diffrp_amd import layered
+    H = W = 2048
+    layers, C = 4, 16
+    v, f = noisy_icosphere(9, 0.8, 0.005, seed=0)
+    cam = PerspectiveCamera.looking_at_origin_from([0.6, 0.5, 2.0], h=H, w=W)
+    g = torch.Generator().manual_seed(9)
+    attr = torch.rand(v.shape[0], C, generator=g).to(dev).requires_grad_(True)
+    target = torch.rand(1, H, W, 3, generator=g).to(dev)
+    verts = v.to(dev).requires_grad_(True)
+    tri = f.int().to(dev)
+    VP = (cam.P('cpu') @ cam.V('cpu')).to(dev)
+    ctx = nvdr.RasterizeCudaContext(dev)
+    light = torch.tensor([0.3, 0.5, 0.8], device=dev)
+
+    def shade(gb, rast):
+        n = torch.nn.functional.normalize(gb[..., 6:9] - 0.5, dim=-1, eps=1e-6)
+        lit = 0.5 + 0.5 * (n * light).sum(-1, keepdim=True)
+        rgb = gb[..., 9:12] * lit + 0.05 * (gb[..., 0:2].sum(-1, keepdim=True) + gb[..., 2:6].sum(-1, keepdim=True)
+                                            + gb[..., 13:16].sum(-1, keepdim=True))
+        return torch.cat([rgb, 0.3 + 0.6 * gb[..., 12:13]], -1)
+
+    out = {}
+    for mode, recompute in (('recompute', True), ('stored', False)):
+        def step():
+            verts.grad = None
+            attr.grad = None
+            pos = clip_space(verts, VP)
+            img = layered.render_layers(ctx, pos, tri, attr, (H, W), layers, shade, recompute=recompute)
+            l2_loss(img, target).backward()
+        step()
+        torch.cuda.synchronize()
+        torch.cuda.reset_peak_memory_stats(dev)
+        base = torch.cuda.memory_allocated(dev)
+        t, wall = time_steps(step, 3, 1)
+        peak = torch.cuda.max_memory_allocated(dev) - base
+        out[mode] = {'ms': round(t * 1e3, 3), 'mpix_s': round(layers * H * W / t / 1e6, 1),
+                     'peak_hbm_gib': round(peak / 2 ** 30, 3)}
+    N, V, F = H * W, v.shape[0], f.shape[0]
+    alg = layers * (48 * N + 48 * V + 24 * F) + layers * ((48 + 8 * C) * N + 8 * C * V + 24 * F) + \
+        ((32 + 20 * 3) * N + 48 * V + 24 * F) + (layers - 1) * 16 * N
+    out['workload'] = (f'icosphere s=9 ({F} faces), 2048x2048, {layers} peeled layers, {C} channels, '
+                       f'antialias, fwd+bwd, eager')
+    out['algorithmic_bytes'] = alg
+    out['roofline_frac_recompute'] = round(alg / (out['recompute']['ms'] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+    return out
+
+
+def progress(msg):
+    """One line per finished stage on stderr (stdout carries only the result line)."""
+    print(f'[bench] {msg}', file=sys.stderr, flush=True)
+
+
 def main():
+    faulthandler.enable()
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=20)
@@ -333,6 +554,7 @@ def main():
             pos_all = clip_space(S.verts, S.VP)
             tune = autotune.tune_rasterize(S.ctx, pos_all, S.tri, (S.H, S.W))
             autotune.tune_rasterize(S.ctx, pos_all[0], S.tri, (S.H, S.W))  # the plain loop's single view
+        progress(f'autotuned: {tune["best"]["tile"]}, capacity {tune["best"]["cap"]}')
 
     # eager reference step, then capture the whole fwd+bwd in a HIP graph
     S.zero_grad()
@@ -359,7 +581,9 @@ def main():
             if err > 1e-4:
                 raise RuntimeError(f'graph replay gradient mismatch on {name}: {err:.3e}')
 
-    t_batched = time_region(graph.replay, args.steps, args.warmup)
+    progress('captured step replays match the eager gradients')
+    t_batched, t_wall = time_steps(graph.replay, args.steps, args.warmup)
+    progress(f'batched: {t_batched * 1e3:.4f} ms per step')
     pix = S.n * S.H * S.W
     value = pix / t_batched / 1e6
 
@@ -372,7 +596,9 @@ def main():
         plain = {'mpix_s': pix / t_plain / 1e6, 'ms_per_step': t_plain * 1e3, 'steps': args.plain_steps}
 
     alg = algorithmic_bytes(S.n, S.H, S.W, S.V, S.F, S.C)
+    progress('plain loop timed')
     ops = None if args.no_ops else per_op_times(S)
+    progress('operators timed')
     roof = None
     if ops:
         dom = max(ops, key=lambda k: ops[k]['ms'])
@@ -381,6 +607,7 @@ def main():
                 'traffic': measured_traffic(dom, S, args.subdiv),
                 'bytes_per_launch': ops[dom]['bytes'], 'ms_per_launch': round(ops[dom]['ms'], 4)}
     cpu = None if args.no_cpu else cpu_baseline(S, args.cpu_views)
+    progress('cpu baseline timed')
     configs = None if args.no_configs else other_configs(dev)
 
     out = {
@@ -391,6 +618,8 @@ def main():
         'steps': args.steps,
         'warmup': args.warmup,
         'ms_per_step': round(t_batched * 1e3, 4),
+        'timing': f'median of {args.steps} HIP-event step times (graph replays); wall-clock mean '
+                  f'{t_wall * 1e3:.4f} ms',
         'higher_is_better': True,
         'scaling': 'weak',
         'vs_baseline': None,

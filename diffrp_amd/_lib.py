@@ -38,7 +38,21 @@ _SIGS = {
     'drp_batch_plan_info': (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     'drp_rasterize_batch_workspace_size': (c_size, [c_void_p]),
     'drp_rasterize_batch_fwd': (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
-                                        c_void_p, c_void_p, c_size, c_void_p]),
+                                        c_void_p, c_void_p, c_size, c_void_p, c_int]),
+    'drp_batch_interpolate_fwd': (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
+    'drp_batch_interpolate_bwd_workspace_size': (c_size, [c_void_p, c_int]),
+    'drp_batch_interpolate_bwd': (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
+                                          c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                          c_size]),
+    'drp_batch_rasterize_bwd_workspace_size': (c_size, [c_void_p]),
+    'drp_batch_rasterize_bwd': (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
+                                        c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                        c_size]),
+    'drp_batch_antialias_workspace_size': (c_size, [c_void_p]),
+    'drp_batch_antialias_fwd': (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
+                                        c_void_p, c_void_p, c_void_p, c_size]),
+    'drp_batch_antialias_bwd': (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
+                                        c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_int, c_void_p, c_size]),
     'drp_sample2d_fwd': (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p, c_ll, c_int, c_int, c_void_p]),
     'drp_sample2d_bwd': (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p, c_ll, c_int, c_int, c_void_p,
                                  c_void_p, c_void_p]),
@@ -58,6 +72,10 @@ _SIGS = {
     'drp_interpolate_bwd': (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_int, c_int,
                                     c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
                                     c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_size]),
+    'drp_interpolate_rasterize_bwd_workspace_size': (c_size, [c_int, c_int, c_int, c_int, c_int, c_int]),
+    'drp_interpolate_rasterize_bwd': (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p, c_int, c_int,
+                                              c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
+                                              c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_size]),
     'drp_edge_grad_bwd': (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p,
                                   c_int, c_void_p]),
     'drp_topology_workspace_size': (c_size, [c_int]),
@@ -68,6 +86,11 @@ _SIGS = {
     'drp_antialias_bwd': (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_int, c_int,
                                   c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_int, c_void_p,
                                   c_size]),
+    'drp_clip_transform_fwd': (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, c_void_p]),
+    'drp_clip_transform_bwd': (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, c_void_p]),
+    'drp_l2_loss_workspace_size': (c_size, []),
+    'drp_l2_loss_fwd': (c_int, [c_void_p, c_void_p, c_void_p, c_ll, c_void_p, c_void_p, c_size]),
+    'drp_l2_loss_bwd': (c_int, [c_void_p, c_void_p, c_void_p, c_ll, c_void_p, c_void_p, c_void_p]),
 }
 
 EXPORTED = tuple(_SIGS)

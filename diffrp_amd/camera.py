@@ -123,10 +123,50 @@ class PerspectiveCamera(Camera):
         return cam.lookat([0.0, 0.0, 0.0])
 
 
+class _ClipSpace(torch.autograd.Function):
+    """pos [n,V,4] = VP_b [x, y, z, 1]^T on the HIP kernels of ``drp_clip.hip``;
+    the backward sums the views into the world vertices in view order."""
+
+    @staticmethod
+    def forward(ctx, xyz, VP):
+        from . import _lib
+        from ._lib import check, ptr, stream_of
+        n, V = VP.shape[0], xyz.shape[0]
+        pos = torch.empty((n, V, 4), dtype=torch.float32, device=xyz.device)
+        check(_lib.lib().drp_clip_transform_fwd(stream_of(xyz), ptr(xyz), V, ptr(VP), n, ptr(pos)), 'clip transform')
+        ctx.save_for_backward(xyz, VP)
+        return pos
+
+    @staticmethod
+    def backward(ctx, g):
+        from . import _lib
+        from ._lib import check, ptr, stream_of
+        xyz, VP = ctx.saved_tensors
+        g = g.float().contiguous()
+        gx = gvp = None
+        if ctx.needs_input_grad[0]:
+            gx = torch.empty_like(xyz)
+            check(_lib.lib().drp_clip_transform_bwd(stream_of(g), ptr(g), xyz.shape[0], ptr(VP), VP.shape[0],
+                                                    ptr(gx)), 'clip transform backward')
+        if ctx.needs_input_grad[1]:
+            h = torch.nn.functional.pad(xyz, (0, 1), value=1.0)
+            gvp = torch.matmul(g.transpose(-1, -2), h)
+        return gx, gvp
+
+
 def clip_space(world_pos: torch.Tensor, VP: torch.Tensor) -> torch.Tensor:
     """[x, y, z, 1] @ VP^T (``diffrp/utils/shader_ops.py:123-136``, ``surface_deferred.py:229-238``).
 
-    ``VP`` may be [4,4] (one view) or [B,4,4] (one clip-space buffer per view).
+    ``VP`` may be [4,4] (one view, result [V,4]) or [B,4,4] (one clip-space
+    buffer per view, [B,V,4]).  Device tensors run the HIP transform (one fma
+    chain per row); host tensors, which only prepare inputs, use torch.
     """
-    h = torch.nn.functional.pad(world_pos, (0, 1), value=1.0)
-    return torch.matmul(h, VP.transpose(-1, -2)).contiguous()
+    if not world_pos.is_cuda:
+        h = torch.nn.functional.pad(world_pos, (0, 1), value=1.0)
+        return torch.matmul(h, VP.transpose(-1, -2)).contiguous()
+    if world_pos.dim() != 2 or world_pos.shape[-1] != 3 or VP.shape[-2:] != (4, 4) or VP.dim() not in (2, 3):
+        raise ValueError('clip_space takes world_pos [V, 3] and VP [4, 4] or [B, 4, 4]')
+    xyz = world_pos.float().contiguous()
+    vp = VP.float().to(xyz.device).reshape(-1, 4, 4).contiguous()
+    pos = _ClipSpace.apply(xyz, vp)
+    return pos[0] if VP.dim() == 2 else pos

@@ -115,10 +115,15 @@ __device__ __forceinline__ int orient3(float4 a, float4 b, float4 c) {
 
 // pixel coordinates of every vertex, in double (NaN when w <= 0): the pair
 // search reads them for every candidate triangle
-__global__ void k_aa_project(const float *__restrict__ pos, int V, int n_proj, int H, int W,
+__global__ void k_aa_project(const float *__restrict__ pos, int V, int n_proj, int H, int W, UnevenMap um,
                              double2 *__restrict__ proj) {
     long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= (long long)n_proj * V) return;
+    if (um.n) {  // uneven batch: the vertex's item sets the image size
+        const int item = um_find(um.vert, um.n, (int)g);
+        H = um.hw[2 * item];
+        W = um.hw[2 * item + 1];
+    }
     float4 p = reinterpret_cast<const float4 *>(pos)[g];
     double2 o;
     if (p.w > 0.f) {
@@ -265,6 +270,17 @@ __device__ __forceinline__ float signed_w(const PairRes &p) {
 
 __device__ __forceinline__ AAImg aa_img(const AALaunch &L, int img) {
     AAImg I;
+    if (L.um.n) {  // uneven batch: the item's pixels; vertices and triangles are global
+        I.rast = reinterpret_cast<const float4 *>(L.rast) + L.um.pix[img];
+        I.proj = L.proj;
+        I.orient = L.orient;
+        I.silh = L.silh;
+        I.V = L.V;
+        I.F = L.F;
+        I.H = L.um.hw[2 * img];
+        I.W = L.um.hw[2 * img + 1];
+        return I;
+    }
     long long npx = (long long)L.H * L.W;
     I.rast = reinterpret_cast<const float4 *>(L.rast) + img * npx;
     int pimg = L.pos_batched ? img : 0;
@@ -311,15 +327,23 @@ __global__ void __launch_bounds__(256) k_aa_pairs(AALaunch L, float2 *__restrict
     __shared__ int2 sd[256];
     __shared__ int q_n;
     const int tid = threadIdx.x, lane = tid & 63;
-    const int c = blockIdx.x * 64 + lane, r = blockIdx.y * 4 + (tid >> 6), img = blockIdx.z;
-    const bool in = c < L.W && r < L.H;
+    int bx = blockIdx.x, by = blockIdx.y, img = blockIdx.z;
+    if (L.um.n) {  // uneven batch: one grid row of every item's blocks
+        img = um_find(L.um.blk, L.um.n, (int)blockIdx.x);
+        const int lb = blockIdx.x - L.um.blk[img], nbx = (L.um.hw[2 * img + 1] + 63) / 64;
+        bx = lb % nbx;
+        by = lb / nbx;
+    }
     const AAImg I = aa_img(L, img);
-    const long long q0 = (long long)r * L.W + c;
-    const bool has_r = in && c + 1 < L.W, has_u = in && r + 1 < L.H;
+    const int H = I.H, W = I.W;
+    const int c = bx * 64 + lane, r = by * 4 + (tid >> 6);
+    const bool in = c < W && r < H;
+    const long long q0 = (long long)r * W + c;
+    const bool has_r = in && c + 1 < W, has_u = in && r + 1 < H;
     int cand = 0;  // bit 0: pair (pix, right) may blend; bit 1: pair (pix, above)
     if (in) {
         const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-        const float4 p0 = I.rast[q0], pr = has_r ? I.rast[q0 + 1] : z4, pu = has_u ? I.rast[q0 + L.W] : z4;
+        const float4 p0 = I.rast[q0], pr = has_r ? I.rast[q0 + 1] : z4, pu = has_u ? I.rast[q0 + W] : z4;
         const int i0 = rast_id(p0, I.F), ir = rast_id(pr, I.F), iu = rast_id(pu, I.F);
         const int s0 = silh_of(I, i0) & 7;
         if (has_r && ir != i0 && (s0 || (silh_of(I, ir) & 7))) cand |= 1;
@@ -347,9 +371,9 @@ __global__ void __launch_bounds__(256) k_aa_pairs(AALaunch L, float2 *__restrict
     const int n = q_n;
     for (int e = tid; e < n; e += 256) {
         const int m = q[e], lp = m & 255, dir = m >> 8;
-        const int ra = blockIdx.y * 4 + (lp >> 6), ca = blockIdx.x * 64 + (lp & 63);
+        const int ra = by * 4 + (lp >> 6), ca = bx * 64 + (lp & 63);
         const int rb = dir ? ra + 1 : ra, cb = dir ? ca : ca + 1;
-        const float4 pa = I.rast[(long long)ra * L.W + ca], pb = I.rast[(long long)rb * L.W + cb];
+        const float4 pa = I.rast[(long long)ra * W + ca], pb = I.rast[(long long)rb * W + cb];
         const int ia = rast_id(pa, I.F), ib = rast_id(pb, I.F);
         const PairRes p = eval_pair(I, L.tri, pa, pb, silh_of(I, ia), silh_of(I, ib), ra, ca, rb, cb);
         if (dir) {
@@ -362,7 +386,7 @@ __global__ void __launch_bounds__(256) k_aa_pairs(AALaunch L, float2 *__restrict
     }
     __syncthreads();
     if (!in) return;
-    const long long g = (long long)img * L.H * L.W + q0;
+    const long long g = (L.um.n ? L.um.pix[img] : (long long)img * L.H * L.W) + q0;
     weights[g] = sw[tid];
     if (decisions) decisions[g] = sd[tid];
 }
@@ -396,23 +420,39 @@ __device__ __forceinline__ void st(float *p, const float (&x)[CH]) {
 }
 
 // one thread per (pixel, channel chunk): consecutive threads touch consecutive bytes
+// row, column and image width of packed pixel g (uniform batch: images of
+// H x W; uneven batch: the item map)
+__device__ __forceinline__ void pixel_rc(const AALaunch &L, long long g, int &r, int &c, int &W) {
+    if (L.um.n) {
+        const int img = um_find(L.um.pix, L.um.n, g);
+        W = L.um.hw[2 * img + 1];
+        const long long q = g - L.um.pix[img];
+        r = (int)(q / W);
+        c = (int)(q - (long long)r * W);
+        return;
+    }
+    const long long npx = (long long)L.H * L.W;
+    const long long q = g % npx;
+    W = L.W;
+    r = (int)(q / L.W);
+    c = (int)(q % L.W);
+}
+
 template <int CV>
 __global__ void __launch_bounds__(256) k_aa_gather(AALaunch L, const float2 *__restrict__ weights) {
     constexpr int CH = AAChunk<CV>::CH;
     const int C = CV > 0 ? CV : L.C;
     const int nch = C / CH;
     long long gt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    long long npx = (long long)L.H * L.W;
     long long g = gt / nch;
-    if (g >= (long long)L.n_img * npx) return;
+    if (g >= L.total_pix) return;
     int ch = (int)(gt - g * nch) * CH;
-    int img = (int)(g / npx);
-    long long q = g - img * npx;
-    int r = (int)(q / L.W), c = (int)(q % L.W);
+    int r, c, W;
+    pixel_rc(L, g, r, c, W);
     const float *col = L.color + ch;
     float2 wp = weights[g];
     float wl = c > 0 ? weights[g - 1].x : 0.f;        // pair (left, p): > 0 -> p receives
-    float wd = r > 0 ? weights[g - L.W].y : 0.f;      // pair (below, p): > 0 -> p receives
+    float wd = r > 0 ? weights[g - W].y : 0.f;        // pair (below, p): > 0 -> p receives
     float cp[CH], o[CH], nb[CH];
     ld<CH>(col + g * C, cp);
 #pragma unroll
@@ -428,12 +468,12 @@ __global__ void __launch_bounds__(256) k_aa_gather(AALaunch L, const float2 *__r
         for (int i = 0; i < CH; ++i) o[i] = fma_(-wp.x, nb[i] - cp[i], o[i]);
     }
     if (wd > 0.f) {
-        ld<CH>(col + (g - L.W) * C, nb);
+        ld<CH>(col + (g - W) * C, nb);
 #pragma unroll
         for (int i = 0; i < CH; ++i) o[i] = fma_(wd, nb[i] - cp[i], o[i]);
     }
     if (wp.y < 0.f) {
-        ld<CH>(col + (g + L.W) * C, nb);
+        ld<CH>(col + (g + W) * C, nb);
 #pragma unroll
         for (int i = 0; i < CH; ++i) o[i] = fma_(-wp.y, nb[i] - cp[i], o[i]);
     }
@@ -487,7 +527,7 @@ __device__ __forceinline__ PairGrad pair_pos_grad(const AALaunch &L, const AAImg
                                                   int r, int c, int dir, const PairRes &pr) {
     PairGrad out;
     const int rb = dir ? r + 1 : r, cb = dir ? c : c + 1;
-    const long long ga = g, gb = dir ? g + L.W : g + 1;
+    const long long ga = g, gb = dir ? g + I.W : g + 1;
     const long long gr = pr.recv == 0 ? ga : gb, gs = pr.recv == 0 ? gb : ga;
     const int *t3 = L.tri + 3 * (long long)pr.tri;
     const int i = pr.edge, j = pr.edge == 2 ? 0 : pr.edge + 1;
@@ -526,7 +566,7 @@ __device__ __forceinline__ PairGrad pair_pos_grad(const AALaunch &L, const AAImg
     gy2[0] = k * (dt_dfa * (fxp - q1.x) + dt_dfb * (txp - q1.x));
     gx2[1] = k * (dt_dfa * (fyp - q0.y) + dt_dfb * (typ - q0.y));
     gy2[1] = k * (dt_dfa * (q0.x - fxp) + dt_dfb * (q0.x - txp));
-    const double hw = (double)L.W * 0.5, hh = (double)L.H * 0.5;
+    const double hw = (double)I.W * 0.5, hh = (double)I.H * 0.5;
 #pragma unroll
     for (int m = 0; m < 2; ++m) {
         double iw = 1.0 / (double)pv[m].w;
@@ -563,8 +603,18 @@ __global__ void __launch_bounds__(256) k_aa_pairs_bwd(AALaunch L, float2 *__rest
     __shared__ int q_n;
     const int tid = threadIdx.x;
     const int bid = xcd_remap(blockIdx.x, n_blocks_pad);
-    const int img = bid / tiles_per_img, lt = bid - img * tiles_per_img;
-    const bool live_block = img < L.n_img;
+    int img, lt;
+    bool live_block;
+    if (L.um.n) {  // uneven batch: the item map names the tile's item
+        live_block = bid < L.total_t32;
+        img = live_block ? um_find(L.um.t32, L.um.n, bid) : 0;
+        lt = bid - L.um.t32[img];
+        tiles_x = (L.um.hw[2 * img + 1] + kAAT - 1) / kAAT;
+    } else {
+        img = bid / tiles_per_img;
+        lt = bid - img * tiles_per_img;
+        live_block = img < L.n_img;
+    }
     const bool want_pos = L.grad_pos != nullptr;
     for (int e = tid; e < kAAHash; e += 256) {
         A.key[e] = -1;
@@ -575,16 +625,17 @@ __global__ void __launch_bounds__(256) k_aa_pairs_bwd(AALaunch L, float2 *__rest
     if (tid == 0) q_n = 0;
     __syncthreads();
     const int r0 = (lt / tiles_x) * kAAT, c0 = (lt % tiles_x) * kAAT;
-    const long long npx = (long long)L.H * L.W;
-    const AAImg I = aa_img(L, img < L.n_img ? img : 0);
+    const AAImg I = aa_img(L, live_block ? img : 0);
+    const int H = I.H, W = I.W;
+    const long long pix0 = L.um.n ? L.um.pix[live_block ? img : 0] : (long long)img * H * W;
     for (int k = 0; k < kAAPix / 256; ++k) {
         const int px = tid + 256 * k, r = r0 + (px >> 5), c = c0 + (px & 31);
         // the pixel's blending, unclamped pairs: slot a, slot b (direction: 0 right, 1 above)
         PairRes qa, qb;
         int da = 0, db = 0, nq = 0;
         qa.recv = qb.recv = -1;
-        if (live_block && r < L.H && c < L.W) {
-            long long g = (long long)img * npx + (long long)r * L.W + c;
+        if (live_block && r < H && c < W) {
+            long long g = pix0 + (long long)r * W + c;
             if (decisions) {
                 int2 d = want_pos ? decisions[g] : make_int2(0, 0);
                 const PairRes p0 = unpack_decision(d.x), p1 = unpack_decision(d.y);
@@ -598,8 +649,8 @@ __global__ void __launch_bounds__(256) k_aa_pairs_bwd(AALaunch L, float2 *__rest
                 float2 w = make_float2(0.f, 0.f);
                 for (int dir = 0; dir < 2; ++dir) {
                     int rb = dir ? r + 1 : r, cb = dir ? c : c + 1;
-                    if (rb >= L.H || cb >= L.W) continue;
-                    const float4 pa = I.rast[(long long)r * L.W + c], pb = I.rast[(long long)rb * L.W + cb];
+                    if (rb >= H || cb >= W) continue;
+                    const float4 pa = I.rast[(long long)r * W + c], pb = I.rast[(long long)rb * W + cb];
                     const int ia = rast_id(pa, I.F), ib = rast_id(pb, I.F);
                     PairRes pr = eval_pair(I, L.tri, pa, pb, silh_of(I, ia), ia != ib ? silh_of(I, ib) : 0, r, c,
                                            rb, cb);
@@ -649,7 +700,7 @@ __global__ void __launch_bounds__(256) k_aa_pairs_bwd(AALaunch L, float2 *__rest
             pr.from_a = (bits >> 2) & 1;
             pr.edge = bits & 3;
             const int pr_r = r0 + (px >> 5), pr_c = c0 + (px & 31);
-            const long long g = (long long)img * npx + (long long)pr_r * L.W + pr_c;
+            const long long g = pix0 + (long long)pr_r * W + pr_c;
             pair_grad_add(A, GP, pair_pos_grad(L, I, P, g, pr_r, pr_c, dir, pr));
         }
     }
@@ -671,17 +722,15 @@ __global__ void __launch_bounds__(256) k_aa_gather_bwd(AALaunch L, const float2 
     const int C = CV > 0 ? CV : L.C;
     const int nch = C / CH;
     long long gt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    long long npx = (long long)L.H * L.W;
     long long g = gt / nch;
-    if (g >= (long long)L.n_img * npx) return;
+    if (g >= L.total_pix) return;
     int ch = (int)(gt - g * nch) * CH;
-    int img = (int)(g / npx);
-    long long q = g - img * npx;
-    int r = (int)(q / L.W), c = (int)(q % L.W);
+    int r, c, W;
+    pixel_rc(L, g, r, c, W);
     const float *go = L.grad_out + ch;
     float2 wp = weights[g];
     float wl = c > 0 ? weights[g - 1].x : 0.f;
-    float wd = r > 0 ? weights[g - L.W].y : 0.f;
+    float wd = r > 0 ? weights[g - W].y : 0.f;
     // p receives: - w g[p];  p is the source of a neighbour's blend: + w g[neighbour]
     float self = 0.f;
     if (wl > 0.f) self += wl;
@@ -703,12 +752,12 @@ __global__ void __launch_bounds__(256) k_aa_gather_bwd(AALaunch L, const float2 
         for (int i = 0; i < CH; ++i) acc[i] = fma_(wp.x, nb[i], acc[i]);
     }
     if (wd < 0.f) {
-        ld<CH>(go + (g - L.W) * C, nb);
+        ld<CH>(go + (g - W) * C, nb);
 #pragma unroll
         for (int i = 0; i < CH; ++i) acc[i] = fma_(-wd, nb[i], acc[i]);
     }
     if (wp.y > 0.f) {
-        ld<CH>(go + (g + L.W) * C, nb);
+        ld<CH>(go + (g + W) * C, nb);
 #pragma unroll
         for (int i = 0; i < CH; ++i) acc[i] = fma_(wp.y, nb[i], acc[i]);
     }
@@ -729,7 +778,7 @@ static void launch_orient(const AALaunch &L, hipStream_t st) {
     int n_proj = L.pos_batched ? L.n_img : 1;
     long long nv = (long long)n_proj * L.V, nf = (long long)n_proj * L.F;
     if (nv) hipLaunchKernelGGL(k_aa_project, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, st, L.pos, L.V, n_proj,
-                               L.H, L.W, L.proj);
+                               L.H, L.W, L.um, L.proj);
     if (nf) {
         hipLaunchKernelGGL(k_aa_orient, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, st, L.pos, L.V, L.tri, L.F,
                            n_proj, L.orient);
@@ -743,24 +792,34 @@ static dim3 gather_grid(const AALaunch &L, long long total) {
     return dim3((unsigned)((total * nch + 255) / 256));
 }
 
-int launch_antialias(const AALaunch &L, hipStream_t st) {
-    long long total = (long long)L.n_img * L.H * L.W;
+// a uniform launch's totals in the fields the uneven path fills from its plan
+static AALaunch with_totals(const AALaunch &L) {
+    AALaunch K = L;
+    if (!K.um.n) K.total_pix = (long long)L.n_img * L.H * L.W;
+    return K;
+}
+
+int launch_antialias(const AALaunch &L0, hipStream_t st) {
+    const AALaunch L = with_totals(L0);
+    const long long total = L.total_pix;
     if (total == 0) return 0;
     launch_orient(L, st);
     dim3 grid((unsigned)((L.W + 63) / 64), (unsigned)((L.H + 3) / 4), (unsigned)L.n_img), block(256);
+    if (L.um.n) grid = dim3((unsigned)L.total_blk);
     hipLaunchKernelGGL(k_aa_pairs, grid, block, 0, st, L, L.weights, L.decisions);
     grid = gather_grid(L, total);
     DRP_AA_DISPATCH(k_aa_gather, L, (const float2 *)L.weights)
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
-int launch_antialias_bwd(const AALaunch &L, hipStream_t st) {
-    long long total = (long long)L.n_img * L.H * L.W;
+int launch_antialias_bwd(const AALaunch &L0, hipStream_t st) {
+    const AALaunch L = with_totals(L0);
+    const long long total = L.total_pix;
     if (total == 0) return 0;
     // with the forward's state, projections, orientations, weights and decisions are in place
     if (!L.reuse) launch_orient(L, st);
     int tiles_x = (L.W + kAAT - 1) / kAAT, tiles_per_img = tiles_x * ((L.H + kAAT - 1) / kAAT);
-    long long nb = (long long)tiles_per_img * L.n_img, nb_pad = (nb + 7) & ~7ll;
+    long long nb = L.um.n ? (long long)L.total_t32 : (long long)tiles_per_img * L.n_img, nb_pad = (nb + 7) & ~7ll;
     dim3 block(256);
     hipLaunchKernelGGL(k_aa_pairs_bwd, dim3((unsigned)nb_pad), block, 0, st, L, L.weights,
                        L.reuse ? (const int2 *)L.decisions : (const int2 *)nullptr, tiles_x, tiles_per_img,

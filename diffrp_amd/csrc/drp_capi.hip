@@ -177,6 +177,27 @@ static int auto_bin_cap(int F, long long tiles) {
 
 static size_t plan_desc_off() { return (sizeof(BatchPlanHeader) + 15) & ~(size_t)15; }
 static size_t plan_t2i_off(int n_img) { return plan_desc_off() + (size_t)n_img * sizeof(ImageDesc); }
+// item map (UnevenMap) after the tile -> image table: pixel prefix (int64), the
+// vertex, triangle, 16 x 16 tile, 64 x 4 block and 32 x 32 tile prefixes
+// (int32, n + 1 each), then (H, W) per item
+static size_t plan_map_off(int n_img, long long n_tiles) {
+    return (plan_t2i_off(n_img) + sizeof(int32_t) * (size_t)n_tiles + 15) & ~(size_t)15;
+}
+static size_t plan_map_bytes(int n_img) { return 8 * (size_t)(n_img + 1) + 20 * (size_t)(n_img + 1) + 8 * (size_t)n_img; }
+static UnevenMap plan_map(const void *plan, int n_img, long long n_tiles) {
+    UnevenMap m;
+    const char *b = (const char *)plan + plan_map_off(n_img, n_tiles);
+    m.pix = (const long long *)b;
+    const int32_t *q = (const int32_t *)(b + 8 * (size_t)(n_img + 1));
+    m.vert = q;
+    m.tri = q + (n_img + 1);
+    m.bt16 = q + 2 * (n_img + 1);
+    m.blk = q + 3 * (n_img + 1);
+    m.t32 = q + 4 * (n_img + 1);
+    m.hw = q + 5 * (n_img + 1);
+    m.n = n_img;
+    return m;
+}
 
 static long long plan_tiles(const int32_t *items, int n_img, int tile_w, int tile_h) {
     long long n = 0;
@@ -187,7 +208,7 @@ static long long plan_tiles(const int32_t *items, int n_img, int tile_w, int til
 
 size_t drp_batch_plan_size(const int32_t *items, int n_img, int tile_w, int tile_h) {
     if (!items || n_img <= 0 || !tile_ok(tile_w, tile_h)) return 0;
-    return plan_t2i_off(n_img) + sizeof(int32_t) * (size_t)plan_tiles(items, n_img, tile_w, tile_h);
+    return plan_map_off(n_img, plan_tiles(items, n_img, tile_w, tile_h)) + plan_map_bytes(n_img);
 }
 
 int drp_batch_plan(const int32_t *items, int n_img, int tile_w, int tile_h, void *plan, size_t plan_bytes) {
@@ -209,13 +230,39 @@ int drp_batch_plan(const int32_t *items, int n_img, int tile_w, int tile_h, void
         max_count = F > max_count ? F : max_count;
     }
     if (tb > 0x7fffffffll || fb > 0x7fffffffll || vb > 0x7fffffffll) return fail(-1, "batch too large");
-    size_t need = plan_t2i_off(n_img) + sizeof(int32_t) * (size_t)tb;
+    size_t need = plan_map_off(n_img, tb) + plan_map_bytes(n_img);
     if (!plan || plan_bytes < need) return fail(-1, "plan buffer too small: need %zu bytes", need);
     char *base = (char *)plan;
     BatchPlanHeader h{n_img, tile_w, tile_h, max_count, tb, pb, fb, vb, sb};
     memcpy(base, &h, sizeof(h));
     ImageDesc *D = (ImageDesc *)(base + plan_desc_off());
     int32_t *t2i = (int32_t *)(base + plan_t2i_off(n_img));
+    {
+        UnevenMap m = plan_map(plan, n_img, tb);
+        long long *pix = (long long *)m.pix;
+        int32_t *vert = (int32_t *)m.vert, *tri = (int32_t *)m.tri, *bt16 = (int32_t *)m.bt16;
+        int32_t *blk = (int32_t *)m.blk, *t32 = (int32_t *)m.t32, *hw = (int32_t *)m.hw;
+        long long p = 0, v = 0, f = 0, a = 0, k = 0, c = 0;
+        for (int i = 0; i <= n_img; ++i) {
+            pix[i] = p;
+            vert[i] = (int32_t)v;
+            tri[i] = (int32_t)f;
+            bt16[i] = (int32_t)a;
+            blk[i] = (int32_t)k;
+            t32[i] = (int32_t)c;
+            if (i == n_img) break;
+            int V = items[5 * i], F = items[5 * i + 1], H = items[5 * i + 2], W = items[5 * i + 3];
+            hw[2 * i] = H;
+            hw[2 * i + 1] = W;
+            p += (long long)H * W;
+            v += V;
+            f += F;
+            a += (long long)((W + 15) / 16) * ((H + 15) / 16);
+            k += (long long)((W + 63) / 64) * ((H + 3) / 4);
+            c += (long long)((W + 31) / 32) * ((H + 31) / 32);
+        }
+        if (a > 0x7fffffffll || k > 0x7fffffffll || c > 0x7fffffffll) return fail(-1, "batch too large");
+    }
     vb = fb = pb = tb = sb = 0;
     for (int i = 0; i < n_img; ++i) {
         int V = items[5 * i], F = items[5 * i + 1], H = items[5 * i + 2], W = items[5 * i + 3];
@@ -266,7 +313,7 @@ size_t drp_rasterize_batch_workspace_size(const void *plan) {
 
 int drp_rasterize_batch_fwd(void *stream, const float *pos, const int32_t *tri, const void *plan,
                             const void *plan_dev, const float *prev, float *rast, float *rast_db, void *workspace,
-                            size_t workspace_bytes, int32_t *boxes) {
+                            size_t workspace_bytes, int32_t *boxes, int global_ids) {
     if (!plan || !plan_dev) return fail(-1, "null plan");
     BatchPlanHeader h;
     memcpy(&h, plan, sizeof(h));
@@ -286,7 +333,7 @@ int drp_rasterize_batch_fwd(void *stream, const float *pos, const int32_t *tri, 
     b.n_img = h.n_img;
     b.V = (int)h.n_vert;
     b.F = (int)h.n_tri;
-    b.local_ids = 1;
+    b.local_ids = global_ids ? 0 : 1;
     RasterLaunch L{};
     L.b = b;
     L.pos = pos;
@@ -308,6 +355,256 @@ int drp_rasterize_batch_fwd(void *stream, const float *pos, const int32_t *tri, 
     L.rast_db = rast_db;
     int rc = launch_rasterize(L, (hipStream_t)stream);
     if (rc) return fail(rc, "batch rasterize launch failed: %s", hipGetErrorString(hipGetLastError()));
+    return 0;
+}
+
+// ---- uneven batches: interpolate, antialias and the backward passes --------
+// The items' buffers packed back to back (rast [n_pix, 4] with global triangle
+// ids from drp_rasterize_batch_fwd(..., global_ids = 1)); pos [n_vert, 4] and
+// attr [n_vert, C] concatenated; tri [n_tri, 3] with global vertex indices.
+namespace {
+struct PlanView {
+    BatchPlanHeader h;
+    UnevenMap um;
+    long long bt16, blk, t32;  // totals of the item map's tilings
+};
+
+int plan_view(const void *plan, const void *plan_dev, PlanView &v) {
+    if (!plan || !plan_dev) return fail(-1, "null plan");
+    if (!aligned16(plan_dev)) return fail(-1, "plan must be 16-byte aligned");
+    memcpy(&v.h, plan, sizeof(v.h));
+    if (v.h.n_img <= 0 || !tile_ok(v.h.tile_w, v.h.tile_h)) return fail(-1, "invalid plan");
+    const UnevenMap host = plan_map(plan, v.h.n_img, v.h.n_tiles);
+    v.bt16 = host.bt16[v.h.n_img];
+    v.blk = host.blk[v.h.n_img];
+    v.t32 = host.t32[v.h.n_img];
+    v.um = plan_map(plan_dev, v.h.n_img, v.h.n_tiles);
+    return 0;
+}
+}  // namespace
+
+int drp_batch_interpolate_fwd(void *stream, const void *plan, const float *attr, int C, const float *rast,
+                              const int32_t *tri, float *out) {
+    if (!plan) return fail(-1, "null plan");
+    BatchPlanHeader h;
+    memcpy(&h, plan, sizeof(h));
+    if (C < 1) return fail(-1, "attribute channel count must be >= 1");
+    if (h.n_pix == 0) return 0;
+    if (!rast || !out || (h.n_vert > 0 && !attr) || (h.n_tri > 0 && !tri)) return fail(-1, "null pointer argument");
+    if (!aligned16(rast) || (C % 4 == 0 && (!aligned16(out) || (attr && !aligned16(attr)))))
+        return fail(-1, "rast (and out, attr when C %% 4 == 0) must be 16-byte aligned");
+    InterpLaunch L{};
+    L.attr = attr;
+    L.V = (int)h.n_vert;
+    L.C = C;
+    L.rast = rast;
+    L.tri = tri;
+    L.F = (int)h.n_tri;
+    L.total_pix = h.n_pix;
+    L.pix_per_img = h.n_pix;
+    L.cv = channel_variant(C);
+    L.out = out;
+    int rc = launch_interpolate(L, (hipStream_t)stream);
+    if (rc) return fail(rc, "batch interpolate launch failed: %s", hipGetErrorString(hipGetLastError()));
+    return 0;
+}
+
+static size_t batch_interp_partial_bytes(const BatchPlanHeader &h, int C) {
+    return align256((size_t)h.n_tri * 3 * (size_t)C * 4);
+}
+
+size_t drp_batch_interpolate_bwd_workspace_size(const void *plan, int C) {
+    PlanView v;
+    if (!plan || C <= 0) return 0;
+    memcpy(&v.h, plan, sizeof(v.h));
+    long long tiles = plan_map(plan, v.h.n_img, v.h.n_tiles).bt16[v.h.n_img];
+    return batch_interp_partial_bytes(v.h, C) + tile_records_bytes_tiles(tiles, 3 * C);
+}
+
+int drp_batch_interpolate_bwd(void *stream, const void *plan, const void *plan_dev, const float *attr, int C,
+                              const float *rast, const int32_t *tri, const float *grad_out, float *grad_attr,
+                              float *grad_rast, const int32_t *boxes, const int32_t *inc_off, const int32_t *inc,
+                              void *workspace, size_t workspace_bytes) {
+    PlanView v;
+    if (int e = plan_view(plan, plan_dev, v)) return e;
+    const int cv = channel_variant(C);
+    if (cv == 0) return fail(-1, "batch interpolate backward supports C in {1, 2, 3, 4, 8, 16} (got %d)", C);
+    if (v.h.n_pix == 0) return 0;
+    if (!rast || !grad_out || !grad_rast || (v.h.n_vert > 0 && (!attr || !grad_attr)) ||
+        (v.h.n_tri > 0 && (!tri || !boxes || !inc_off || !inc)))
+        return fail(-1, "null pointer argument");
+    if (!aligned16(rast) || !aligned16(grad_rast) || (C % 4 == 0 && (!aligned16(grad_out) || !aligned16(attr))))
+        return fail(-1, "rast-shaped buffers (and attr, grad_out when C %% 4 == 0) must be 16-byte aligned");
+    size_t need = drp_batch_interpolate_bwd_workspace_size(plan, C);
+    if (!workspace || workspace_bytes < need)
+        return fail(-1, "workspace too small: need %zu bytes, got %zu", need, workspace_bytes);
+    InterpLaunch L{};
+    L.attr = attr;
+    L.V = (int)v.h.n_vert;
+    L.C = C;
+    L.rast = rast;
+    L.tri = tri;
+    L.F = (int)v.h.n_tri;
+    L.total_pix = v.h.n_pix;
+    L.pix_per_img = v.h.n_pix;
+    L.cv = cv;
+    L.grad_out = grad_out;
+    L.grad_attr = grad_attr;
+    L.grad_rast = grad_rast;
+    L.views_per_group = 1;
+    L.boxes = (const ItemBox *)boxes;
+    L.partial = (float *)workspace;
+    L.records = (char *)workspace + batch_interp_partial_bytes(v.h, C);
+    L.inc_off = inc_off;
+    L.inc = inc;
+    L.um = v.um;
+    L.um_tiles = v.bt16;
+    if (v.h.n_tri == 0) {  // nothing covered: zero grad_rast through the per-pixel path
+        L.boxes = nullptr;
+        L.partial = nullptr;
+    }
+    int rc = launch_interpolate_bwd(L, (hipStream_t)stream);
+    if (rc) return fail(rc, "batch interpolate backward launch failed: %s", hipGetErrorString(hipGetLastError()));
+    return 0;
+}
+
+size_t drp_batch_rasterize_bwd_workspace_size(const void *plan) {
+    if (!plan) return 0;
+    BatchPlanHeader h;
+    memcpy(&h, plan, sizeof(h));
+    if (h.n_tri <= 0) return 0;
+    long long tiles = plan_map(plan, h.n_img, h.n_tiles).bt16[h.n_img];
+    return align256((size_t)h.n_tri * 3 * 16) + tile_records_bytes_tiles(tiles, 12) + align256((size_t)h.n_tri * 3 * 4);
+}
+
+int drp_batch_rasterize_bwd(void *stream, const void *plan, const void *plan_dev, const float *pos, const int32_t *tri,
+                            const float *rast, int local_ids, const float *grad_rast, const float *grad_db,
+                            float *grad_pos, const int32_t *boxes, const int32_t *inc_off, const int32_t *inc,
+                            void *workspace, size_t workspace_bytes) {
+    PlanView v;
+    if (int e = plan_view(plan, plan_dev, v)) return e;
+    if (v.h.n_pix == 0 || v.h.n_tri == 0) return 0;
+    if (!pos || !tri || !rast || !grad_rast || !grad_pos || !boxes || !inc_off || !inc)
+        return fail(-1, "null pointer argument");
+    if (!aligned16(pos) || !aligned16(rast) || !aligned16(grad_rast) || !aligned16(grad_pos) ||
+        (grad_db && !aligned16(grad_db)))
+        return fail(-1, "pos, rast, grad_rast, grad_db and grad_pos must be 16-byte aligned");
+    size_t need = drp_batch_rasterize_bwd_workspace_size(plan);
+    if (!workspace || workspace_bytes < need)
+        return fail(-1, "workspace too small: need %zu bytes, got %zu", need, workspace_bytes);
+    RasterBwdLaunch L{};
+    Batch b{};
+    b.n_img = 1;
+    b.V = (int)v.h.n_vert;
+    b.F = (int)v.h.n_tri;
+    b.H = 1;
+    b.W = 1;
+    b.pos_batched = 0;
+    L.b = b;
+    L.pos = pos;
+    L.tri = tri;
+    L.rast = rast;
+    L.grad_rast = grad_rast;
+    L.grad_db = grad_db;
+    L.grad_pos = grad_pos;
+    L.total_pix = v.h.n_pix;
+    L.pix_per_img = 1;
+    L.boxes = (const ItemBox *)boxes;
+    L.partial = (float *)workspace;
+    L.records = (char *)workspace + align256((size_t)v.h.n_tri * 3 * 16);
+    L.corner_slot = (int *)((char *)L.records + tile_records_bytes_tiles(v.bt16, 12));
+    L.inc_off = inc_off;
+    L.inc = inc;
+    L.um = v.um;
+    L.um_tiles = v.bt16;
+    L.local_ids = local_ids ? 1 : 0;
+    int rc = launch_rasterize_bwd(L, (hipStream_t)stream);
+    if (rc) return fail(rc, "batch rasterize backward launch failed: %s", hipGetErrorString(hipGetLastError()));
+    return 0;
+}
+
+size_t drp_batch_antialias_workspace_size(const void *plan) {
+    if (!plan) return 0;
+    BatchPlanHeader h;
+    memcpy(&h, plan, sizeof(h));
+    return align256((size_t)h.n_vert * 16) + 2 * align256((size_t)h.n_tri) + 2 * align256((size_t)h.n_pix * 8);
+}
+
+static int batch_aa_common(AALaunch &L, const PlanView &v, const float *color, int C, const float *rast,
+                           const float *pos, const int32_t *tri, const int32_t *opp, void *workspace,
+                           size_t workspace_bytes, const void *plan) {
+    if (C < 1) return fail(-1, "color channel count must be >= 1");
+    if (!color || !rast || (v.h.n_vert > 0 && !pos) || (v.h.n_tri > 0 && (!tri || !opp)))
+        return fail(-1, "null pointer argument");
+    if (!aligned16(pos) || !aligned16(rast)) return fail(-1, "pos/rast must be 16-byte aligned");
+    if (C % 4 == 0 && !aligned16(color)) return fail(-1, "color must be 16-byte aligned when C %% 4 == 0");
+    size_t need = drp_batch_antialias_workspace_size(plan);
+    if (!workspace || workspace_bytes < need)
+        return fail(-1, "workspace too small: need %zu bytes, got %zu", need, workspace_bytes);
+    char *b = (char *)workspace;
+    L.color = color;
+    L.rast = rast;
+    L.pos = pos;
+    L.tri = tri;
+    L.opp = opp;
+    L.n_img = v.h.n_img;
+    L.H = 1;
+    L.W = 1;
+    L.C = C;
+    L.V = (int)v.h.n_vert;
+    L.F = (int)v.h.n_tri;
+    L.pos_batched = 0;
+    L.cv = (C == 1 || C == 3 || C == 4 || C == 8 || C == 16) ? C : 0;
+    size_t o_orient = align256((size_t)v.h.n_vert * 16);
+    size_t o_silh = o_orient + align256((size_t)v.h.n_tri), o_w = o_silh + align256((size_t)v.h.n_tri);
+    size_t o_dec = o_w + align256((size_t)v.h.n_pix * 8);
+    L.proj = (double2 *)b;
+    L.orient = (signed char *)(b + o_orient);
+    L.silh = (unsigned char *)(b + o_silh);
+    L.weights = (float2 *)(b + o_w);
+    L.decisions = v.h.n_tri < kMaxDecisionTris ? (int2 *)(b + o_dec) : nullptr;
+    L.um = v.um;
+    L.total_pix = v.h.n_pix;
+    L.total_blk = (int)v.blk;
+    L.total_t32 = (int)v.t32;
+    return 0;
+}
+
+int drp_batch_antialias_fwd(void *stream, const void *plan, const void *plan_dev, const float *color, int C,
+                            const float *rast, const float *pos, const int32_t *tri, const int32_t *opp, float *out,
+                            void *workspace, size_t workspace_bytes) {
+    PlanView v;
+    if (int e = plan_view(plan, plan_dev, v)) return e;
+    if (v.h.n_pix == 0) return 0;
+    AALaunch L{};
+    if (int e = batch_aa_common(L, v, color, C, rast, pos, tri, opp, workspace, workspace_bytes, plan)) return e;
+    if (!out) return fail(-1, "null output");
+    if (C % 4 == 0 && !aligned16(out)) return fail(-1, "out must be 16-byte aligned when C %% 4 == 0");
+    L.out = out;
+    int rc = launch_antialias(L, (hipStream_t)stream);
+    if (rc) return fail(rc, "batch antialias launch failed: %s", hipGetErrorString(hipGetLastError()));
+    return 0;
+}
+
+int drp_batch_antialias_bwd(void *stream, const void *plan, const void *plan_dev, const float *color, int C,
+                            const float *rast, const float *pos, const int32_t *tri, const int32_t *opp,
+                            const float *grad_out, float *grad_color, float *grad_pos, float pos_gradient_boost,
+                            int reuse_forward, void *workspace, size_t workspace_bytes) {
+    PlanView v;
+    if (int e = plan_view(plan, plan_dev, v)) return e;
+    if (v.h.n_pix == 0) return 0;
+    AALaunch L{};
+    if (int e = batch_aa_common(L, v, color, C, rast, pos, tri, opp, workspace, workspace_bytes, plan)) return e;
+    if (!grad_out || !grad_color) return fail(-1, "null gradient buffer");
+    if (C % 4 == 0 && (!aligned16(grad_out) || !aligned16(grad_color)))
+        return fail(-1, "grad_out/grad_color must be 16-byte aligned when C %% 4 == 0");
+    L.grad_out = grad_out;
+    L.grad_color = grad_color;
+    L.grad_pos = grad_pos;
+    L.boost = pos_gradient_boost;
+    L.reuse = reuse_forward && L.decisions != nullptr;
+    int rc = launch_antialias_bwd(L, (hipStream_t)stream);
+    if (rc) return fail(rc, "batch antialias backward launch failed: %s", hipGetErrorString(hipGetLastError()));
     return 0;
 }
 
@@ -460,6 +757,79 @@ int drp_interpolate_bwd(void *stream, const float *attr, int n_img, int V, int a
     }
     int rc = launch_interpolate_bwd(L, (hipStream_t)stream);
     if (rc) return fail(rc, "interpolate_bwd launch failed: %s", hipGetErrorString(hipGetLastError()));
+    return 0;
+}
+
+// interpolate backward and the backward of the rasterize call that produced
+// rast, from one pass over the pixels (tile records carry both sums)
+size_t drp_interpolate_rasterize_bwd_workspace_size(int n_img, int H, int W, int F, int C, int views_per_group) {
+    if (n_img <= 0 || F <= 0 || C <= 0 || H <= 0 || W <= 0) return 0;
+    const int vpg = views_per_block(views_per_group);
+    const size_t groups = (size_t)((n_img + vpg - 1) / vpg);
+    return align256(groups * (size_t)F * 3 * (size_t)C * 4) + align256((size_t)n_img * (size_t)F * 3 * 16) +
+           tile_records_bytes(n_img, H, W, 3 * C + 12) + align256((size_t)F * 3 * 4);
+}
+
+int drp_interpolate_rasterize_bwd(void *stream, const float *attr, int n_img, int V, int C, const float *rast, int H,
+                                  int W, const int32_t *tri, int F, const float *pos, int pos_batched,
+                                  const float *grad_out, float *grad_attr, float *grad_rast, float *grad_pos,
+                                  const int32_t *boxes, int views_per_group, const int32_t *inc_off,
+                                  const int32_t *inc, void *workspace, size_t workspace_bytes) {
+    if (int e = check_dims(n_img, H, W)) return e;
+    if (n_img == 0) return 0;
+    if (!(C == 4 || C == 8)) return fail(-1, "the fused backward takes C in {4, 8} (got %d)", C);
+    if (F <= 0 || V <= 0) return fail(-1, "the fused backward needs triangles and vertices");
+    if (!attr || !rast || !tri || !pos || !grad_out || !grad_attr || !grad_rast || !grad_pos || !boxes || !inc_off ||
+        !inc)
+        return fail(-1, "null pointer argument");
+    if (!aligned16(rast) || !aligned16(grad_rast) || !aligned16(pos) || !aligned16(grad_pos) ||
+        !aligned16(grad_out) || !aligned16(attr))
+        return fail(-1, "rast, grad_rast, pos, grad_pos, grad_out and attr must be 16-byte aligned");
+    const size_t need = drp_interpolate_rasterize_bwd_workspace_size(n_img, H, W, F, C, views_per_group);
+    if (!workspace || workspace_bytes < need)
+        return fail(-1, "workspace too small: need %zu bytes, got %zu", need, workspace_bytes);
+    const int vpg = views_per_block(views_per_group);
+    const int groups = (n_img + vpg - 1) / vpg;
+    char *b = (char *)workspace;
+    float *partial_attr = (float *)b;
+    b += align256((size_t)groups * (size_t)F * 3 * (size_t)C * 4);
+    float *partial_pos = (float *)b;
+    b += align256((size_t)n_img * (size_t)F * 3 * 16);
+    void *records = b;
+    b += tile_records_bytes(n_img, H, W, 3 * C + 12);
+    int *slot = (int *)b;
+    hipStream_t st = (hipStream_t)stream;
+    if (int rc = launch_corner_slots(inc, inc_off, V, F, slot, st))
+        return fail(rc, "corner slots launch failed: %s", hipGetErrorString(hipGetLastError()));
+    TriBwdLaunch T{};
+    T.b.n_img = n_img;
+    T.b.V = V;
+    T.b.F = F;
+    T.b.H = H;
+    T.b.W = W;
+    T.b.pos_batched = pos_batched;
+    T.pos = pos;
+    T.tri = tri;
+    T.rast = rast;
+    T.grad_out = grad_out;
+    T.attr = attr;
+    T.V = V;
+    T.attr_batched = 0;
+    T.grad_rast_out = grad_rast;
+    T.boxes = (const ItemBox *)boxes;
+    T.partial = partial_attr;
+    T.partial_pos = partial_pos;
+    T.corner_slot = slot;
+    T.n_img = n_img;
+    T.F = F;
+    T.H = H;
+    T.W = W;
+    T.views_per_block = vpg;
+    tile_records_layout(T, records, n_img, H, W, 3 * C + 12);
+    int rc = launch_tri_bwd_fused(T, C, st);
+    if (!rc) rc = launch_gather_attr(partial_attr, groups, F, V, C, inc_off, inc, grad_attr, st);
+    if (!rc) rc = launch_gather_pos(partial_pos, n_img, F, V, pos_batched, inc_off, grad_pos, st);
+    if (rc) return fail(rc, "fused backward launch failed: %s", hipGetErrorString(hipGetLastError()));
     return 0;
 }
 
@@ -696,6 +1066,50 @@ int drp_antialias_bwd(void *stream, const float *color, int n_img, int H, int W,
     L.reuse = reuse_forward && L.decisions != nullptr;
     int rc = launch_antialias_bwd(L, (hipStream_t)stream);
     if (rc) return fail(rc, "antialias_bwd launch failed: %s", hipGetErrorString(hipGetLastError()));
+    return 0;
+}
+
+// ---- clip-space transform and the L2 image loss ----------------------------
+int drp_clip_transform_fwd(void *stream, const float *xyz, int V, const float *VP, int n, float *pos) {
+    if (V < 0 || n < 0) return fail(-1, "negative V (%d) or view count (%d)", V, n);
+    if (V == 0 || n == 0) return 0;
+    if (!xyz || !VP || !pos) return fail(-1, "null pointer argument");
+    if (!aligned16(pos)) return fail(-1, "pos must be 16-byte aligned");
+    int rc = launch_clip_fwd(xyz, V, VP, n, pos, (hipStream_t)stream);
+    if (rc) return fail(rc, "clip transform launch failed: %s", hipGetErrorString(hipGetLastError()));
+    return 0;
+}
+
+int drp_clip_transform_bwd(void *stream, const float *grad_pos, int V, const float *VP, int n, float *grad_xyz) {
+    if (V < 0 || n < 0) return fail(-1, "negative V (%d) or view count (%d)", V, n);
+    if (V == 0) return 0;
+    if (!grad_xyz || (n > 0 && (!grad_pos || !VP))) return fail(-1, "null pointer argument");
+    if (n > 0 && !aligned16(grad_pos)) return fail(-1, "grad_pos must be 16-byte aligned");
+    int rc = launch_clip_bwd(grad_pos, V, VP, n, grad_xyz, (hipStream_t)stream);
+    if (rc) return fail(rc, "clip transform backward launch failed: %s", hipGetErrorString(hipGetLastError()));
+    return 0;
+}
+
+size_t drp_l2_loss_workspace_size(void) { return l2_loss_ws_bytes(); }
+
+int drp_l2_loss_fwd(void *stream, const float *a, const float *b, long long n, float *loss, void *workspace,
+                    size_t workspace_bytes) {
+    if (n < 0) return fail(-1, "negative element count");
+    if (!loss || (n > 0 && (!a || !b))) return fail(-1, "null pointer argument");
+    if (!workspace || workspace_bytes < l2_loss_ws_bytes())
+        return fail(-1, "workspace too small: need %zu bytes, got %zu", l2_loss_ws_bytes(), workspace_bytes);
+    int rc = launch_l2_loss(a, b, n, loss, workspace, (hipStream_t)stream);
+    if (rc) return fail(rc, "l2 loss launch failed: %s", hipGetErrorString(hipGetLastError()));
+    return 0;
+}
+
+int drp_l2_loss_bwd(void *stream, const float *a, const float *b, long long n, const float *grad_loss, float *grad_a,
+                    float *grad_b) {
+    if (n < 0) return fail(-1, "negative element count");
+    if (n == 0) return 0;
+    if (!a || !b || !grad_loss || (!grad_a && !grad_b)) return fail(-1, "null pointer argument");
+    int rc = launch_l2_loss_bwd(a, b, n, grad_loss, grad_a, grad_b, (hipStream_t)stream);
+    if (rc) return fail(rc, "l2 loss backward launch failed: %s", hipGetErrorString(hipGetLastError()));
     return 0;
 }
 

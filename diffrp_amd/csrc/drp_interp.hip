@@ -224,7 +224,7 @@ __global__ void __launch_bounds__(256) k_gather_attr(const float *__restrict__ p
 
 template <int NC>
 static void launch_interp_bwd_tri_t(const InterpLaunch &L, hipStream_t st) {
-    int n_img = (int)(L.total_pix / L.pix_per_img);
+    int n_img = L.um.n ? 1 : (int)(L.total_pix / L.pix_per_img);  // an uneven batch sums as one group
     int vpg = L.attr_batched ? 1 : L.views_per_group;
     int groups = (n_img + vpg - 1) / vpg;
     TriBwdLaunch T{};
@@ -242,11 +242,31 @@ static void launch_interp_bwd_tri_t(const InterpLaunch &L, hipStream_t st) {
     T.H = L.H;
     T.W = L.W;
     T.views_per_block = vpg;
-    tile_records_layout(T, L.records, n_img, L.H, L.W, 3 * NC);
+    if (L.um.n) {
+        T.um = L.um;
+        T.views_per_block = 1;
+        tile_records_layout_tiles(T, L.records, L.um_tiles);
+    } else {
+        tile_records_layout(T, L.records, n_img, L.H, L.W, 3 * NC);
+    }
     launch_tri_bwd_attr(T, NC, st);
     int outs = L.attr_batched ? n_img : 1;
     hipLaunchKernelGGL(k_gather_attr<NC>, dim3((L.V + 255) / 256, outs), dim3(256), 0, st, L.partial, groups, L.F,
                        L.V, L.attr_batched, L.inc_off, L.inc, L.grad_attr);
+}
+
+int launch_gather_attr(const float *partial, int groups, int F, int V, int C, const int *inc_off, const int *inc,
+                       float *grad_attr, hipStream_t st) {
+    if (V == 0) return 0;
+    dim3 grid((V + 255) / 256, 1);
+    switch (C) {
+        case 4: hipLaunchKernelGGL(k_gather_attr<4>, grid, dim3(256), 0, st, partial, groups, F, V, 0, inc_off, inc,
+                                   grad_attr); break;
+        case 8: hipLaunchKernelGGL(k_gather_attr<8>, grid, dim3(256), 0, st, partial, groups, F, V, 0, inc_off, inc,
+                                   grad_attr); break;
+        default: return -1;
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
 int launch_interpolate_bwd(const InterpLaunch &L, hipStream_t st) {

@@ -5,6 +5,33 @@
 
 namespace drp {
 
+// Uneven batches (items of different resolutions and meshes, packed back to
+// back): prefix sums over the items of their pixels, vertices, triangles and of
+// the tiles each pass uses.  The item a work unit (block, tile, pixel) belongs
+// to is the last item whose prefix is <= the unit's index (um_find).  n = 0:
+// uniform batch, every image [H, W] of the launch.
+struct UnevenMap {
+    const long long *pix;   // [n + 1] pixel offsets
+    const int *vert;        // [n + 1] vertex offsets
+    const int *tri;         // [n + 1] triangle offsets
+    const int *bt16;        // [n + 1] 16 x 16 tiles (deterministic backward)
+    const int *blk;         // [n + 1] 64 x 4 blocks (antialias pairs)
+    const int *t32;         // [n + 1] 32 x 32 tiles (antialias backward)
+    const int *hw;          // [2 n] (H, W) per item
+    int n;
+};
+
+template <typename T>
+__device__ __forceinline__ int um_find(const T *pre, int n, T x) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pre[mid] <= x) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 // Pixel box of one (image, triangle) from the forward setup: columns c0..c1 in
 // cc (c0 | c1 << 16), rows r0..r1 in rr; c0 > c1 when empty.  hit is set by the
 // forward when the triangle won at least one pixel of that image.
@@ -54,6 +81,9 @@ struct RasterBwdLaunch {
     void *records;           // tile records (tile_records_bytes(n_img, H, W, 12))
     int *corner_slot;        // [3F]: position of corner 3t+k in the incidence list
     const int *inc_off, *inc;
+    UnevenMap um;            // uneven batch (um.n > 0): packed items, global vertices
+    long long um_tiles;      // uneven batch: 16 x 16 tiles of all items
+    int local_ids;           // uneven batch: rast ids count from each item's first triangle (else global)
 };
 
 struct InterpLaunch {
@@ -77,6 +107,8 @@ struct InterpLaunch {
     void *records;           // tile records (tile_records_bytes(n_img, H, W, 3 C))
     const int *inc_off, *inc;
     int views_per_group;
+    UnevenMap um;            // uneven batch (um.n > 0): packed items, global ids and vertices, one group
+    long long um_tiles;      // uneven batch: 16 x 16 tiles of all items
 };
 
 struct TopoLaunch {
@@ -104,6 +136,9 @@ struct AALaunch {
     const float *grad_out;
     float *grad_color, *grad_pos;
     float boost;
+    UnevenMap um;                // uneven batch (um.n > 0): n_img items, packed; pos / tri global
+    long long total_pix;         // uneven batch: pixels of all items
+    int total_blk, total_t32;    // uneven batch: pair blocks and backward tiles of all items
 };
 
 struct TriBwdLaunch {
@@ -116,6 +151,7 @@ struct TriBwdLaunch {
     float *grad_rast_out;    // attribute mode: grad_rast written by the tile pass
     const ItemBox *boxes;
     float *partial;
+    float *partial_pos;      // fused mode: the position partial [n_img, 3F] float4 (partial: attributes)
     const int *corner_slot;  // positions mode: partial index of corner 3t+k (vertex order)
     int n_img, F, H, W;
     int views_per_block;     // images summed per partial (attribute mode); 1 for positions
@@ -123,13 +159,24 @@ struct TriBwdLaunch {
     int *rec_count, *rec_t;
     float *rec_v;
     int tiles_x, tiles_per_img, n_tiles, n_tiles_pad;
+    UnevenMap um;            // uneven batch (um.n > 0): tiles of all items, one group
+    int local_ids;           // uneven batch: rast ids count from each item's first triangle (else global)
 };
 
 int launch_tri_bwd_pos(const TriBwdLaunch &L, hipStream_t st);
 int launch_tri_bwd_attr(const TriBwdLaunch &L, int C, hipStream_t st);
+int launch_tri_bwd_fused(const TriBwdLaunch &L, int C, hipStream_t st);
+int launch_corner_slots(const int *inc, const int *inc_off, int V, int F, int *slot, hipStream_t st);
+int launch_gather_pos(const float *partial, int n_img, int F, int V, int pos_batched, const int *inc_off,
+                      float *grad_pos, hipStream_t st);
+int launch_gather_attr(const float *partial, int groups, int F, int V, int C, const int *inc_off, const int *inc,
+                       float *grad_attr, hipStream_t st);
 // workspace of the tile records for nv floats per record, and its layout
 size_t tile_records_bytes(int n_img, int H, int W, int nv);
 void tile_records_layout(TriBwdLaunch &T, void *ws, int n_img, int H, int W, int nv);
+// the same for an uneven batch of n_tiles 16 x 16 tiles in all
+size_t tile_records_bytes_tiles(long long n_tiles, int nv);
+void tile_records_layout_tiles(TriBwdLaunch &T, void *ws, long long n_tiles);
 
 struct EdgeLaunch {
     const float *grad_out, *rgb, *rast, *frag_xy, *tris_xy;
@@ -159,5 +206,11 @@ int launch_stencil_partition(const float *rast, long long npx, const int *stenci
                              int *starts, int *ends, void *ws, hipStream_t st);
 int launch_count_covered(const float *rast, int n_img, long long npx, int *counts, hipStream_t st);
 int launch_incidence(const int *tri, int F, int V, int *inc_off, int *inc, void *ws, hipStream_t st);
+size_t l2_loss_ws_bytes();
+int launch_l2_loss(const float *a, const float *b, long long n, float *loss, void *ws, hipStream_t st);
+int launch_l2_loss_bwd(const float *a, const float *b, long long n, const float *grad_loss, float *ga, float *gb,
+                       hipStream_t st);
+int launch_clip_fwd(const float *xyz, int V, const float *VP, int n, float *pos, hipStream_t st);
+int launch_clip_bwd(const float *grad_pos, int V, const float *VP, int n, float *grad_xyz, hipStream_t st);
 
 }  // namespace drp

@@ -1,0 +1,123 @@
+// Image L2 loss of the training step, sum over all elements of (a - b)^2, and
+// its gradient 2 g (a - b): one read of both images forward, one read of both
+// and one write of the gradient backward.
+//
+// The sum is deterministic: a fixed grid of kLossBlocks blocks, each thread
+// summing its grid-stride elements in double, the block's threads reduced in
+// a fixed tree, then one block adding the block sums in block order.
+#include "drp_common.h"
+#include "drp_kernels.h"
+
+namespace drp {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+constexpr int kLossBlocks = 2048;
+constexpr int kLossThreads = 256;
+
+__device__ __forceinline__ double block_sum(double v, double *red) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < kLossThreads / 64; ++w) s += red[w];
+    return s;
+}
+
+__global__ void __launch_bounds__(kLossThreads) k_l2_partial(const v4f *__restrict__ a,
+                                                             const v4f *__restrict__ b, long long n4,
+                                                             const float *__restrict__ a1,
+                                                             const float *__restrict__ b1, long long n,
+                                                             double *__restrict__ partial) {
+    __shared__ double red[kLossThreads / 64];
+    double s = 0.0;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) {
+        const v4f x = __builtin_nontemporal_load(a + i), y = __builtin_nontemporal_load(b + i);
+        const float d0 = x.x - y.x, d1 = x.y - y.y, d2 = x.z - y.z, d3 = x.w - y.w;
+        s += (double)(d0 * d0) + (double)(d1 * d1) + (double)(d2 * d2) + (double)(d3 * d3);
+    }
+    // the tail that does not fill a float4
+    for (long long i = 4 * n4 + (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const float d = a1[i] - b1[i];
+        s += (double)(d * d);
+    }
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(kLossThreads) k_l2_finish(const double *__restrict__ partial, int n_part,
+                                                            float *__restrict__ loss) {
+    __shared__ double red[kLossThreads / 64];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n_part; i += blockDim.x) s += partial[i];
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) *loss = (float)s;
+}
+
+__global__ void __launch_bounds__(kLossThreads) k_l2_grad(const v4f *__restrict__ a, const v4f *__restrict__ b,
+                                                          long long n4, const float *__restrict__ a1,
+                                                          const float *__restrict__ b1, long long n,
+                                                          const float *__restrict__ grad_loss,
+                                                          v4f *__restrict__ ga, v4f *__restrict__ gb,
+                                                          float *__restrict__ ga1, float *__restrict__ gb1) {
+    const float k = 2.0f * *grad_loss;  // d/da of g (a - b)^2
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n4) {
+        const v4f x = __builtin_nontemporal_load(a + i), y = __builtin_nontemporal_load(b + i);
+        const v4f g = {k * (x.x - y.x), k * (x.y - y.y), k * (x.z - y.z), k * (x.w - y.w)};
+        if (ga) __builtin_nontemporal_store(g, ga + i);
+        if (gb) __builtin_nontemporal_store(-g, gb + i);
+    }
+    const long long t = 4 * n4 + i;
+    if (i < 4 && t < n) {
+        const float g = k * (a1[t] - b1[t]);
+        if (ga1) ga1[t] = g;
+        if (gb1) gb1[t] = -g;
+    }
+}
+
+__global__ void __launch_bounds__(kLossThreads) k_l2_grad_scalar(const float *__restrict__ a,
+                                                                 const float *__restrict__ b, long long n,
+                                                                 const float *__restrict__ grad_loss,
+                                                                 float *__restrict__ ga, float *__restrict__ gb) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float g = (2.0f * *grad_loss) * (a[i] - b[i]);
+    if (ga) ga[i] = g;
+    if (gb) gb[i] = -g;
+}
+
+size_t l2_loss_ws_bytes() { return sizeof(double) * kLossBlocks; }
+
+int launch_l2_loss(const float *a, const float *b, long long n, float *loss, void *ws, hipStream_t st) {
+    const long long n4 = ((((uintptr_t)a | (uintptr_t)b) & 15) == 0) ? n / 4 : 0;
+    long long want = (n4 + kLossThreads - 1) / kLossThreads;
+    const int blocks = (int)(want < 1 ? 1 : (want > kLossBlocks ? kLossBlocks : want));
+    hipLaunchKernelGGL(k_l2_partial, dim3(blocks), dim3(kLossThreads), 0, st, (const v4f *)a, (const v4f *)b,
+                       n4, a, b, n, (double *)ws);
+    hipLaunchKernelGGL(k_l2_finish, dim3(1), dim3(kLossThreads), 0, st, (const double *)ws, blocks, loss);
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int launch_l2_loss_bwd(const float *a, const float *b, long long n, const float *grad_loss, float *ga, float *gb,
+                       hipStream_t st) {
+    const bool vec = ((((uintptr_t)a | (uintptr_t)b | (uintptr_t)ga | (uintptr_t)gb) & 15) == 0);
+    const long long n4 = vec ? n / 4 : 0;
+    const long long items = n4 > 4 ? n4 : 4;  // the last 4 threads also take the tail
+    const long long blocks = (items + kLossThreads - 1) / kLossThreads;
+    if (!vec) {
+        // scalar layout: every element is tail work of a one-element-per-thread grid
+        const long long nb = (n + kLossThreads - 1) / kLossThreads;
+        hipLaunchKernelGGL(k_l2_grad_scalar, dim3((unsigned)(nb < 1 ? 1 : nb)), dim3(kLossThreads), 0, st, a, b, n,
+                           grad_loss, ga, gb);
+        return hipGetLastError() == hipSuccess ? 0 : -2;
+    }
+    hipLaunchKernelGGL(k_l2_grad, dim3((unsigned)blocks), dim3(kLossThreads), 0, st, (const v4f *)a,
+                       (const v4f *)b, n4, a, b, n, grad_loss, (v4f *)ga, (v4f *)gb, ga, gb);
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+}  // namespace drp

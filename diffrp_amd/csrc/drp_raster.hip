@@ -798,6 +798,23 @@ int launch_rasterize(const RasterLaunch &L, hipStream_t st) {
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
+int launch_corner_slots(const int *inc, const int *inc_off, int V, int F, int *slot, hipStream_t st) {
+    long long nc = 3ll * F;
+    if (nc == 0) return 0;
+    hipLaunchKernelGGL(k_corner_slots_clear, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, st, nc, slot);
+    hipLaunchKernelGGL(k_corner_slots, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, st, inc, inc_off, V, slot);
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int launch_gather_pos(const float *partial, int n_img, int F, int V, int pos_batched, const int *inc_off,
+                      float *grad_pos, hipStream_t st) {
+    if (V == 0) return 0;
+    int n_out = pos_batched ? n_img : 1;
+    hipLaunchKernelGGL(k_gather_pos, dim3((V + 255) / 256, n_out), dim3(256), 0, st, (const float4 *)partial, n_img, F,
+                       V, pos_batched, inc_off, (float4 *)grad_pos);
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
 int launch_rasterize_bwd(const RasterBwdLaunch &L, hipStream_t st) {
     if (L.total_pix == 0) return 0;
     if (L.boxes && L.partial && L.records && L.inc_off && L.inc) {
@@ -815,7 +832,14 @@ int launch_rasterize_bwd(const RasterBwdLaunch &L, hipStream_t st) {
         T.H = L.b.H;
         T.W = L.b.W;
         T.views_per_block = 1;
-        tile_records_layout(T, L.records, L.b.n_img, L.b.H, L.b.W, 12);
+        if (L.um.n) {  // uneven batch: the items' tiles, one partial
+            T.um = L.um;
+            T.local_ids = L.local_ids;
+            T.n_img = 1;
+            tile_records_layout_tiles(T, L.records, L.um_tiles);
+        } else {
+            tile_records_layout(T, L.records, L.b.n_img, L.b.H, L.b.W, 12);
+        }
         long long nc = 3ll * L.b.F;
         hipLaunchKernelGGL(k_corner_slots_clear, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, st, nc,
                            L.corner_slot);
@@ -825,7 +849,7 @@ int launch_rasterize_bwd(const RasterBwdLaunch &L, hipStream_t st) {
         if (int rc = launch_tri_bwd_pos(T, st)) return rc;
         int n_out = L.b.pos_batched ? L.b.n_img : 1;
         dim3 gg((L.b.V + 255) / 256, n_out);
-        hipLaunchKernelGGL(k_gather_pos, gg, dim3(256), 0, st, (const float4 *)L.partial, L.b.n_img, L.b.F, L.b.V,
+        hipLaunchKernelGGL(k_gather_pos, gg, dim3(256), 0, st, (const float4 *)L.partial, T.n_img, L.b.F, L.b.V,
                            L.b.pos_batched, L.inc_off, (float4 *)L.grad_pos);
         return hipGetLastError() == hipSuccess ? 0 : -2;
     }

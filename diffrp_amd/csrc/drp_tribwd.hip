@@ -72,10 +72,16 @@ __device__ __forceinline__ int block_excl_scan256(int v, int *wsum, int &total) 
     return pre + x - v;
 }
 
-// MODE 0: POS (NV = 12), MODE 1: ATTR (NV = 3 C)
+// MODE 0: POS (NV = 12), MODE 1: ATTR (NV = 3 C), MODE 2: both from one pass
+// (NV = 3 C + 12: the attribute sums, then the position sums of the grad_rast
+// this pass writes -- what MODE 0 would read back -- so both come out of one
+// sort and one record search, with the same sums as the two passes)
+template <int MODE>
+constexpr int tile_nv(int C) { return MODE == 0 ? 12 : (MODE == 1 ? 3 * C : 3 * C + 12); }
+
 template <int MODE, int C, bool DB, typename K>
 __global__ void __launch_bounds__(256) k_tile_reduce(TriBwdLaunch L) {
-    constexpr int NV = MODE == 0 ? 12 : 3 * C;
+    constexpr int NV = tile_nv<MODE>(C);
     constexpr K kEmptyT = SortKey<K>::kEmptyT;
     __shared__ K key[kBTP];
     __shared__ union {
@@ -90,18 +96,32 @@ __global__ void __launch_bounds__(256) k_tile_reduce(TriBwdLaunch L) {
     const int tid = threadIdx.x;
     const int tile = xcd_remap(blockIdx.x, L.n_tiles_pad);
     if (tile >= L.n_tiles) return;  // uniform per block
-    const int img = tile / L.tiles_per_img, lt = tile - img * L.tiles_per_img;
-    const int r = (lt / L.tiles_x) * kBT + (tid >> 4), c = (lt % L.tiles_x) * kBT + (tid & 15);
-    const long long npx = (long long)L.H * L.W;
+    // the tile's image: arithmetic for a uniform batch, the item map for an uneven one
+    int img, lt, H = L.H, W = L.W, tiles_x = L.tiles_x;
+    long long pix0;
+    if (L.um.n) {
+        img = um_find(L.um.bt16, L.um.n, tile);
+        lt = tile - L.um.bt16[img];
+        H = L.um.hw[2 * img];
+        W = L.um.hw[2 * img + 1];
+        tiles_x = (W + kBT - 1) / kBT;
+        pix0 = L.um.pix[img];
+    } else {
+        img = tile / L.tiles_per_img;
+        lt = tile - img * L.tiles_per_img;
+        pix0 = (long long)img * L.H * L.W;
+    }
+    const int r = (lt / tiles_x) * kBT + (tid >> 4), c = (lt % tiles_x) * kBT + (tid & 15);
     float val[NV];
 #pragma unroll
     for (int j = 0; j < NV; ++j) val[j] = 0.f;
     int t = -1;
-    if (r < L.H && c < L.W) {
-        const long long pix = (long long)img * npx + (long long)r * L.W + c;
+    if (r < H && c < W) {
+        const long long pix = pix0 + (long long)r * W + c;
         float4 rv = reinterpret_cast<const float4 *>(L.rast)[pix];
         if (rv.w > 0.f) {
             int id = float_to_triidx(rv.w);
+            if (L.um.n && L.local_ids) id += L.um.tri[img];  // item-local id -> batch triangle + 1
             if (id >= 1 && id <= L.F) t = id - 1;
         }
         if (MODE == 0) {
@@ -111,9 +131,9 @@ __global__ void __launch_bounds__(256) k_tile_reduce(TriBwdLaunch L) {
                 if (load_tri(L.pos, L.tri, t, d, tv)) {
                     float4 g = reinterpret_cast<const float4 *>(L.grad_rast)[pix];
                     float4 gd = make_float4(0.f, 0.f, 0.f, 0.f);
-                    if (DB) gd = db_grad_scaled(reinterpret_cast<const float4 *>(L.grad_db)[pix], L.H, L.W);
+                    if (DB) gd = db_grad_scaled(reinterpret_cast<const float4 *>(L.grad_db)[pix], H, W);
                     float4 o[3];
-                    raster_pixel_grad<DB>(tv, c, r, L.H, L.W, g, gd, o);
+                    raster_pixel_grad<DB>(tv, c, r, H, W, g, gd, o);
 #pragma unroll
                     for (int q = 0; q < 3; ++q) {
                         val[4 * q] = o[q].x; val[4 * q + 1] = o[q].y; val[4 * q + 2] = o[q].z; val[4 * q + 3] = o[q].w;
@@ -168,6 +188,19 @@ __global__ void __launch_bounds__(256) k_tile_reduce(TriBwdLaunch L) {
                         val[2 * C + ch] = w2 * gc;
                     }
                     gr = make_float4(su, sv, 0.f, 0.f);
+                    if (MODE == 2) {  // the position chain of this grad_rast
+                        ImageDesc d = image_desc(L.b, img);
+                        TriVerts tv;
+                        if (load_tri(L.pos, L.tri, t, d, tv)) {
+                            float4 o[3];
+                            raster_pixel_grad<false>(tv, c, r, H, W, gr, make_float4(0.f, 0.f, 0.f, 0.f), o);
+#pragma unroll
+                            for (int q = 0; q < 3; ++q) {
+                                val[3 * C + 4 * q] = o[q].x; val[3 * C + 4 * q + 1] = o[q].y;
+                                val[3 * C + 4 * q + 2] = o[q].z; val[3 * C + 4 * q + 3] = o[q].w;
+                            }
+                        }
+                    }
                 } else {
                     t = -1;
                 }
@@ -248,6 +281,20 @@ constexpr int kSearch = 4;
 
 template <int NV>
 __device__ __forceinline__ void gather_records(const TriBwdLaunch &L, int t, int v0, int v1, float *acc) {
+    // uniform batch: images v0 .. v1, box of (image, t) at image * F + t, tile
+    // image * tiles_per_img + ty * tiles_x + tx; uneven batch: t's item only,
+    // box at t, tiles from the item's first
+    long long box_stride = L.F;
+    int tile_stride = L.tiles_per_img, tiles_x = L.tiles_x, tile_base = 0;
+    if (L.um.n) {
+        const int item = um_find(L.um.tri, L.um.n, t);
+        v0 = 0;
+        v1 = 1;
+        box_stride = 0;
+        tile_stride = 0;
+        tile_base = L.um.bt16[item];
+        tiles_x = (L.um.hw[2 * item + 1] + kBT - 1) / kBT;
+    }
     int img = v0 - 1, tx = 0, tx0 = 0, tx1 = -1, ty = 1, ty1 = 0;  // empty range: the first fill reads a box
     for (;;) {
         long long rb[kSearch];
@@ -264,7 +311,7 @@ __device__ __forceinline__ void gather_records(const TriBwdLaunch &L, int t, int
             if (ty > ty1) {  // next image where t won a pixel
                 done = true;
                 while (++img < v1) {
-                    ItemBox b = L.boxes[(long long)img * L.F + t];
+                    ItemBox b = L.boxes[(long long)img * box_stride + t];
                     int c0 = b.cc & 0xffff, c1 = (unsigned)b.cc >> 16, r0 = b.rr & 0xffff, r1 = (unsigned)b.rr >> 16;
                     if (!b.hit || c0 > c1 || r0 > r1) continue;
                     tx0 = tx = c0 / kBT;
@@ -276,7 +323,7 @@ __device__ __forceinline__ void gather_records(const TriBwdLaunch &L, int t, int
                 }
                 if (done) continue;
             }
-            const int tile = img * L.tiles_per_img + ty * L.tiles_x + tx;
+            const int tile = tile_base + img * tile_stride + ty * tiles_x + tx;
             rb[k] = (long long)tile * kBTP;
             hi[k] = tile;  // replaced by the record count below
             ++n;
@@ -320,9 +367,109 @@ __device__ __forceinline__ void gather_records(const TriBwdLaunch &L, int t, int
     }
 }
 
+// MODE 2: the attribute sums over the group's images (as MODE 1), and per
+// image the position sums written to that image's corner slots (as MODE 0);
+// records come in (image, tile) order, so an image's position sums are complete
+// when the next image's first record (or the end) arrives.
+template <int NA>
+__device__ __forceinline__ void gather_records_fused(const TriBwdLaunch &L, int t, int v0, int v1, float *acc) {
+    constexpr int NV = NA + 12;
+    const int *slot = L.corner_slot + 3 * (long long)t;
+    auto flush = [&](int img, const float *q12) {
+        float4 *P = reinterpret_cast<float4 *>(L.partial_pos) + (long long)img * 3 * L.F;
+        if (slot[0] >= 0) P[slot[0]] = make_float4(q12[0], q12[1], q12[2], q12[3]);  // -1: vertex out of range
+        if (slot[1] >= 0) P[slot[1]] = make_float4(q12[4], q12[5], q12[6], q12[7]);
+        if (slot[2] >= 0) P[slot[2]] = make_float4(q12[8], q12[9], q12[10], q12[11]);
+    };
+    const float zero12[12] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float pos[12];
+    int cur = v0 - 1;  // image whose position sums are in `pos`
+    int img = v0 - 1, tx = 0, tx0 = 0, tx1 = -1, ty = 1, ty1 = 0;
+    for (;;) {
+        long long rb[kSearch];
+        int lo[kSearch], hi[kSearch], hit[kSearch], im[kSearch];
+        int n = 0;
+        bool done = false;
+#pragma unroll
+        for (int k = 0; k < kSearch; ++k) {
+            rb[k] = 0;
+            lo[k] = 0;
+            hi[k] = -1;
+            hit[k] = -1;
+            im[k] = -1;
+            if (done) continue;
+            if (ty > ty1) {
+                done = true;
+                while (++img < v1) {
+                    ItemBox b = L.boxes[(long long)img * L.F + t];
+                    int c0 = b.cc & 0xffff, c1 = (unsigned)b.cc >> 16, r0 = b.rr & 0xffff, r1 = (unsigned)b.rr >> 16;
+                    if (!b.hit || c0 > c1 || r0 > r1) continue;
+                    tx0 = tx = c0 / kBT;
+                    tx1 = c1 / kBT;
+                    ty = r0 / kBT;
+                    ty1 = r1 / kBT;
+                    done = false;
+                    break;
+                }
+                if (done) continue;
+            }
+            const int tile = img * L.tiles_per_img + ty * L.tiles_x + tx;
+            rb[k] = (long long)tile * kBTP;
+            hi[k] = tile;
+            im[k] = img;
+            ++n;
+            if (++tx > tx1) {
+                tx = tx0;
+                ++ty;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kSearch; ++k)
+            if (k < n) hi[k] = L.rec_count[hi[k]] - 1;
+        for (bool more = true; more;) {
+            more = false;
+#pragma unroll
+            for (int k = 0; k < kSearch; ++k) {
+                if (hit[k] >= 0 || lo[k] > hi[k]) continue;
+                const int mid = (lo[k] + hi[k]) >> 1;
+                const int tm = L.rec_t[rb[k] + mid];
+                if (tm < t) lo[k] = mid + 1;
+                else if (tm > t) hi[k] = mid - 1;
+                else hit[k] = mid;
+                more = more || (hit[k] < 0 && lo[k] <= hi[k]);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kSearch; ++k) {
+            if (k >= n) continue;
+            if (im[k] != cur) {  // a new image: the previous one's sums are complete
+                if (cur >= v0) flush(cur, pos);
+                for (int e = cur + 1; e < im[k]; ++e) flush(e, zero12);
+                cur = im[k];
+#pragma unroll
+                for (int j = 0; j < 12; ++j) pos[j] = 0.f;
+            }
+            if (hit[k] < 0) continue;
+            const float *q = L.rec_v + (rb[k] + hit[k]) * NV;
+#pragma unroll
+            for (int j = 0; j < NV; j += 4) {
+                const float4 x = *reinterpret_cast<const float4 *>(q + j);
+                if (j < NA) {
+                    acc[j] += x.x; acc[j + 1] += x.y; acc[j + 2] += x.z; acc[j + 3] += x.w;
+                } else {
+                    pos[j - NA] += x.x; pos[j - NA + 1] += x.y; pos[j - NA + 2] += x.z; pos[j - NA + 3] += x.w;
+                }
+            }
+        }
+        if (n < kSearch) break;
+    }
+    if (cur >= v0) flush(cur, pos);
+    for (int e = cur + 1; e < v1; ++e) flush(e, zero12);
+}
+
 template <int MODE, int C>
 __global__ void __launch_bounds__(256, 6) k_tile_gather(TriBwdLaunch L) {  // 6 waves per SIMD: latency-bound searches
-    constexpr int NV = MODE == 0 ? 12 : 3 * C;
+    constexpr int NV = tile_nv<MODE>(C);
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int grp = blockIdx.y;
     if (t >= L.F) return;
@@ -331,7 +478,8 @@ __global__ void __launch_bounds__(256, 6) k_tile_gather(TriBwdLaunch L) {  // 6 
     for (int j = 0; j < NV; ++j) acc[j] = 0.f;
     const int VB = L.views_per_block;
     const int v0 = grp * VB, v1 = min(v0 + VB, L.n_img);
-    gather_records<NV>(L, t, v0, v1, acc);
+    if (MODE == 2) gather_records_fused<3 * C>(L, t, v0, v1, acc);
+    else gather_records<NV>(L, t, v0, v1, acc);
     if (MODE == 0) {
         // corners go to their vertex's slots (CSR order), so the vertex gather reads contiguously
         float4 *P = reinterpret_cast<float4 *>(L.partial) + (long long)grp * 3 * L.F;
@@ -340,14 +488,15 @@ __global__ void __launch_bounds__(256, 6) k_tile_gather(TriBwdLaunch L) {  // 6 
         if (slot[1] >= 0) P[slot[1]] = make_float4(acc[4], acc[5], acc[6], acc[7]);
         if (slot[2] >= 0) P[slot[2]] = make_float4(acc[8], acc[9], acc[10], acc[11]);
     } else {
-        float *P = L.partial + ((long long)grp * L.F + t) * NV;
-        if (NV % 4 == 0) {
+        constexpr int NA = 3 * C;  // attribute part (MODE 1: all of the record)
+        float *P = L.partial + ((long long)grp * L.F + t) * NA;
+        if (NA % 4 == 0) {
 #pragma unroll
-            for (int j = 0; j < NV; j += 4)
+            for (int j = 0; j < NA; j += 4)
                 *reinterpret_cast<float4 *>(P + j) = make_float4(acc[j], acc[j + 1], acc[j + 2], acc[j + 3]);
         } else {
 #pragma unroll
-            for (int j = 0; j < NV; ++j) P[j] = acc[j];
+            for (int j = 0; j < NA; ++j) P[j] = acc[j];
         }
     }
 }
@@ -383,6 +532,27 @@ static void launch_attr_c(const TriBwdLaunch &L, hipStream_t st) {
     hipLaunchKernelGGL((k_tile_gather<1, C>), dim3((L.F + 255) / 256, groups), dim3(256), 0, st, L);
 }
 
+template <int C>
+static void launch_fused_c(const TriBwdLaunch &L, hipStream_t st) {
+    dim3 grid;
+    tile_grid(L, grid);
+    if (L.F < (int)SortKey<unsigned>::kEmptyT)
+        hipLaunchKernelGGL((k_tile_reduce<2, C, false, unsigned>), grid, dim3(kBTP), 0, st, L);
+    else
+        hipLaunchKernelGGL((k_tile_reduce<2, C, false, unsigned long long>), grid, dim3(kBTP), 0, st, L);
+    int groups = (L.n_img + L.views_per_block - 1) / L.views_per_block;
+    hipLaunchKernelGGL((k_tile_gather<2, C>), dim3((L.F + 255) / 256, groups), dim3(256), 0, st, L);
+}
+
+int launch_tri_bwd_fused(const TriBwdLaunch &L, int C, hipStream_t st) {
+    switch (C) {
+        case 4: launch_fused_c<4>(L, st); break;
+        case 8: launch_fused_c<8>(L, st); break;
+        default: return -1;
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
 int launch_tri_bwd_attr(const TriBwdLaunch &L, int C, hipStream_t st) {
     switch (C) {
         case 1: launch_attr_c<1>(L, st); break;
@@ -396,10 +566,25 @@ int launch_tri_bwd_attr(const TriBwdLaunch &L, int C, hipStream_t st) {
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
-size_t tile_records_bytes(int n_img, int H, int W, int nv) {
-    long long tiles = (long long)n_img * ((W + kBT - 1) / kBT) * ((H + kBT - 1) / kBT);
+size_t tile_records_bytes_tiles(long long tiles, int nv) {
     auto a256 = [](size_t x) { return (x + 255) & ~(size_t)255; };
     return a256((size_t)tiles * 4) + a256((size_t)tiles * kBTP * 4) + a256((size_t)tiles * kBTP * nv * 4);
+}
+
+size_t tile_records_bytes(int n_img, int H, int W, int nv) {
+    return tile_records_bytes_tiles((long long)n_img * ((W + kBT - 1) / kBT) * ((H + kBT - 1) / kBT), nv);
+}
+
+void tile_records_layout_tiles(TriBwdLaunch &T, void *ws, long long n_tiles) {
+    T.n_tiles = (int)n_tiles;
+    T.n_tiles_pad = (T.n_tiles + 7) & ~7;
+    auto a256 = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    char *b = (char *)ws;
+    T.rec_count = (int *)b;
+    b += a256((size_t)T.n_tiles * 4);
+    T.rec_t = (int *)b;
+    b += a256((size_t)T.n_tiles * kBTP * 4);
+    T.rec_v = (float *)b;
 }
 
 void tile_records_layout(TriBwdLaunch &T, void *ws, int n_img, int H, int W, int nv) {

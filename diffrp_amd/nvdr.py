@@ -22,8 +22,10 @@ kernels through the C-ABI of ``libdrp_hip.so``; gradients flow through
 diffrp itself can import it unchanged.
 """
 import sys
+import threading
 import types
 import weakref
+from collections import OrderedDict
 from typing import Optional, Sequence, Tuple, Union
 
 import numpy as np
@@ -62,13 +64,25 @@ class RasterizeCudaContext:
             raise ValueError(f'tile must be one of {sorted(_TILES)}')
         self.tile = None if tile is None else tuple(tile)
         self.bin_capacity = bin_capacity
-        self._tuned = {}
-        self._pending = {}  # shape -> (tile, capacity, pinned peak bin count, event) of its first eager call
+        self._tuned = OrderedDict()    # shape -> (tile, capacity), least recently used first
+        self._pending = OrderedDict()  # shape -> (tile, capacity, pinned peak bin count, event, tiles, images)
+        self._lock = threading.Lock()
         _lib.lib()
+
+    MAX_TUNED = 1024    # remembered shapes (least recently used are forgotten)
+    MAX_PENDING = 64    # shapes waiting for their first read-back
 
     def set_tuning(self, key, tile, bin_capacity):
         """Install a tuned (tile, bin capacity) for shapes keyed by ``tuning_key``."""
-        self._tuned[key] = (tuple(tile), int(bin_capacity))
+        with self._lock:
+            self._pending.pop(key, None)  # an installed tuning replaces what the first call would teach
+            self._remember(key, (tuple(tile), int(bin_capacity)))
+
+    def _remember(self, key, val):
+        self._tuned[key] = val
+        self._tuned.move_to_end(key)
+        while len(self._tuned) > self.MAX_TUNED:
+            self._tuned.popitem(last=False)
 
     @staticmethod
     def tuning_key(n_img, H, W, F):
@@ -77,10 +91,12 @@ class RasterizeCudaContext:
     def config(self, n_img, H, W, F) -> Tuple[Tuple[int, int], int]:
         key = self.tuning_key(n_img, H, W, F)
         if self.tile is None and self.bin_capacity is None:
-            if key not in self._tuned:
-                self._settle(key)
-            if key in self._tuned:
-                return self._tuned[key]
+            with self._lock:
+                self._settle_all()
+                ent = self._tuned.get(key)
+                if ent is not None:
+                    self._tuned.move_to_end(key)
+                    return ent
         tile = self.tile or default_tile(H, W, F)
         cap = self.bin_capacity or default_bin_capacity(tile, H, W, F)
         return tile, cap
@@ -89,30 +105,51 @@ class RasterizeCudaContext:
     # counts are exact whatever the capacity, so their maximum, read back
     # without a host wait at a later call, sizes the bins of the calls that
     # follow (a crumpled mesh can load its tiles far beyond the heuristic, and
-    # spilled triangles are scanned by every tile of the image).
-    def _watch(self, key, tile, cap, counts: torch.Tensor):
-        if (self.tile is not None or self.bin_capacity is not None or key in self._tuned or key in self._pending
-                or len(self._pending) >= 64 or counts.numel() == 0 or torch.cuda.is_current_stream_capturing()):
-            return  # (at most 64 shapes wait for their first read-back)
-        st = torch.cuda.current_stream(counts.device)
-        with torch.cuda.stream(st):
-            host = torch.empty(1, dtype=torch.int32, pin_memory=True)
-            host.copy_(counts.max().view(1), non_blocking=True)
-            ev = torch.cuda.Event()
-            ev.record(st)
-        self._pending[key] = (tile, cap, host, ev)
-
-    def _settle(self, key):
-        ent = self._pending.get(key)
-        if ent is None or torch.cuda.is_current_stream_capturing() or not ent[3].query():
+    # spilled triangles are scanned by every tile of the image).  The learnt
+    # capacity is bounded by a memory budget (learnt_bin_capacity).
+    def _watch(self, key, tile, cap, counts: torch.Tensor, n_img, F):
+        if (self.tile is not None or self.bin_capacity is not None or counts.numel() == 0
+                or torch.cuda.is_current_stream_capturing()):
             return
-        tile, cap, host, _ = self._pending.pop(key)
-        peak = int(host[0])
-        if peak > cap:
-            cap = 32
-            while cap < int(peak * 1.15) + 1 and cap < (1 << 16):
-                cap *= 2
-        self._tuned[key] = (tile, cap)
+        with self._lock:
+            if key in self._tuned or key in self._pending:
+                return
+            while len(self._pending) >= self.MAX_PENDING:
+                self._pending.popitem(last=False)  # the oldest shape waits for a later first call
+            st = torch.cuda.current_stream(counts.device)
+            with torch.cuda.stream(st):
+                host = torch.empty(1, dtype=torch.int32, pin_memory=True)
+                host.copy_(counts.max().view(1), non_blocking=True)
+                ev = torch.cuda.Event()
+                ev.record(st)
+            self._pending[key] = (tile, cap, host, ev, counts.numel(), n_img * F)
+
+    def _settle_all(self):
+        """Settle every pending shape whose read-back has completed (caller holds the lock)."""
+        if not self._pending or torch.cuda.is_current_stream_capturing():
+            return
+        for key in [k for k, ent in self._pending.items() if ent[3].query()]:
+            tile, cap, host, _, n_tiles, items = self._pending.pop(key)
+            if key not in self._tuned:  # a tuning installed meanwhile wins
+                self._remember(key, (tile, learnt_bin_capacity(int(host[0]), cap, n_tiles, items)))
+
+
+def learnt_bin_capacity(peak: int, cap: int, n_tiles: int, items: int) -> int:
+    """Bin capacity for the calls after a first call with capacity ``cap`` whose
+    busiest tile held ``peak`` entries: the next power of two above peak + 15 %,
+    at most 65536, and at most what a budget of 256 MiB plus 64 bytes per
+    (image, triangle) gives every tile (all tiles share one capacity, so one
+    dense tile must not size the bins of a large image); never below ``cap``."""
+    if peak <= cap:
+        return cap
+    want = 32
+    while want < int(peak * 1.15) + 1 and want < (1 << 16):
+        want *= 2
+    budget = (256 << 20) + 64 * int(items)
+    limit = 32
+    while limit * 2 * 4 * max(int(n_tiles), 1) <= budget and limit < (1 << 16):
+        limit *= 2
+    return max(cap, min(want, limit))
 
 
 RasterizeHipContext = RasterizeCudaContext
@@ -220,7 +257,7 @@ def _rasterize_launch(ctx, pos, tri, H, W, rng, prev, want_db):
                              ptr(ws), ws_bytes, tw, th, cap, list_total, max_range, ptr(boxes))
     check(rc, 'rasterize')
     n_tiles = n * ((W + tw - 1) // tw) * ((H + th - 1) // th)
-    ctx._watch(ctx.tuning_key(n, H, W, F), (tw, th), cap, ws[:4 * n_tiles].view(torch.int32))  # bin counts
+    ctx._watch(ctx.tuning_key(n, H, W, F), (tw, th), cap, ws[:4 * n_tiles].view(torch.int32), n, F)  # bin counts
     return rast, rast_db, boxes
 
 
@@ -253,6 +290,7 @@ class _RasterizeFunc(torch.autograd.Function):
         fctx.rng = rng
         fctx.HW = (H, W)
         fctx.want_db = want_db
+        fctx.set_materialize_grads(False)  # an unused rast_db brings no gradient
         if rast_db is None:
             rast_db = torch.empty((rast.shape[0], H, W, 0), dtype=torch.float32, device=pos.device)
         fctx.mark_non_differentiable(*(() if want_db else (rast_db,)))
@@ -264,9 +302,17 @@ class _RasterizeFunc(torch.autograd.Function):
         pos, tri, rast, boxes = fctx.saved_tensors
         H, W = fctx.HW
         rng = fctx.rng
+        gdb_used = fctx.want_db and ddb is not None and ddb.numel() > 0
+        if dy is None and not gdb_used:
+            return None, None, None, None, None, None, None, None, None
+        # the interpolate backward that produced dy may have computed this pass
+        # already (dy reached here unchanged: interpolate was its only source)
+        pre = getattr(dy, '_drp_pos_grad', None) if dy is not None else None
+        if pre is not None and not gdb_used and pre[1] == rast.data_ptr() and pre[2] == dy._version:
+            return None, pre[0], None, None, None, None, None, None, None
         gpos = torch.zeros_like(pos)
-        dy = _as_f32(dy, 'grad_rast')
-        gdb = _as_f32(ddb, 'grad_db') if (fctx.want_db and ddb is not None and ddb.numel()) else None
+        dy = _as_f32(dy, 'grad_rast') if dy is not None else torch.zeros_like(rast)
+        gdb = _as_f32(ddb, 'grad_db') if gdb_used else None
         n, V, F = rast.shape[0], pos.shape[-2], tri.shape[0]
         L = _lib.lib()
         off = inc = ws = None
@@ -303,8 +349,10 @@ def _rasterize_impl(glctx, pos, tri, resolution, ranges, grad_db, prev):
     holder = []
     rast, rast_db = _RasterizeFunc.apply(glctx, pos, tri, H, W, rng, prev, bool(grad_db), holder)
     if holder and holder[0] is not None:
-        # pixel boxes of this rasterisation, for the triangle-centric interpolate backward
-        rast._drp_boxes = (holder[0], tri.shape[0], rast._version)
+        # pixel boxes of this rasterisation, for the triangle-centric interpolate
+        # backward, and the positions it rasterised (when differentiable: the
+        # interpolate backward can then run the rasterize backward in the same pass)
+        rast._drp_boxes = (holder[0], tri.shape[0], rast._version, pos if rast.requires_grad else None)
     return rast, rast_db
 
 
@@ -379,9 +427,12 @@ def _views_per_group(n, F):
     return p
 
 
+_FUSED_C = (4, 8)
+
+
 class _InterpolateFunc(torch.autograd.Function):
     @staticmethod
-    def forward(fctx, attr, rast, tri, rast_db, diff_idx, boxes):
+    def forward(fctx, attr, rast, tri, rast_db, diff_idx, boxes, src_pos):
         batched, V, C = _attr_layout(attr, rast)
         n, H, W, _ = rast.shape
         out = torch.empty((n, H, W, C), dtype=torch.float32, device=attr.device)
@@ -391,7 +442,10 @@ class _InterpolateFunc(torch.autograd.Function):
                                             tri.shape[0], ptr(rast_db) if nd else None,
                                             ptr(diff_idx) if nd else None, nd, ptr(out), ptr(out_da) if nd else None)
         check(rc, 'interpolate')
-        fctx.save_for_backward(attr, rast, tri, rast_db if nd else None, diff_idx if nd else None, boxes)
+        fused = (src_pos is not None and boxes is not None and nd == 0 and not batched and C in _FUSED_C
+                 and src_pos.shape[-2] == V and tri.shape[0] > 0)
+        fctx.save_for_backward(attr, rast, tri, rast_db if nd else None, diff_idx if nd else None, boxes,
+                               src_pos if fused else None)
         fctx.nd = nd
         if not nd:
             fctx.mark_non_differentiable(out_da)
@@ -399,7 +453,7 @@ class _InterpolateFunc(torch.autograd.Function):
 
     @staticmethod
     def backward(fctx, dy, dda):
-        attr, rast, tri, rast_db, diff_idx, boxes = fctx.saved_tensors
+        attr, rast, tri, rast_db, diff_idx, boxes, src_pos = fctx.saved_tensors
         batched, V, C = _attr_layout(attr, rast)
         n, H, W, _ = rast.shape
         F = tri.shape[0]
@@ -407,11 +461,27 @@ class _InterpolateFunc(torch.autograd.Function):
         dy = _as_f32(dy, 'grad_out')
         gattr = torch.zeros_like(attr)
         grast = torch.empty_like(rast)
+        L = _lib.lib()
+        if src_pos is not None and fctx.needs_input_grad[1]:
+            # the rasterize backward of this grad_rast in the same pass; handed to
+            # that backward through the grad_rast tensor (used only when it arrives
+            # there unchanged)
+            gpos = torch.zeros_like(src_pos)
+            off, inc = incidence(tri, V)
+            vpg = _views_per_group(n, F)
+            ws_bytes = L.drp_interpolate_rasterize_bwd_workspace_size(n, H, W, F, C, vpg)
+            ws = torch.empty(ws_bytes, dtype=torch.uint8, device=attr.device)
+            rc = L.drp_interpolate_rasterize_bwd(stream_of(attr), ptr(attr), n, V, C, ptr(rast), H, W, ptr(tri), F,
+                                                 ptr(src_pos), int(src_pos.dim() == 3), ptr(dy), ptr(gattr),
+                                                 ptr(grast), ptr(gpos), ptr(boxes), vpg, ptr(off), ptr(inc), ptr(ws),
+                                                 ws_bytes)
+            check(rc, 'interpolate + rasterize backward')
+            grast._drp_pos_grad = (gpos, rast.data_ptr(), grast._version)
+            return gattr, grast, None, None, None, None, None
         gdb = torch.empty_like(rast) if nd else None
         dda = _as_f32(dda, 'grad_da') if (nd and dda is not None) else None
         if nd and dda is None:
             dda = torch.zeros((n, H, W, 2 * nd), dtype=torch.float32, device=attr.device)
-        L = _lib.lib()
         off = inc = ws = None
         ws_bytes = vpg = 0
         if boxes is not None and nd == 0 and C in _TRI_BWD_C:
@@ -425,7 +495,7 @@ class _InterpolateFunc(torch.autograd.Function):
                                    ptr(rast_db), ptr(diff_idx), nd, ptr(dy), ptr(dda), ptr(gattr), ptr(grast),
                                    ptr(gdb), ptr(boxes), vpg, ptr(off), ptr(inc), ptr(ws), ws_bytes)
         check(rc, 'interpolate backward')
-        return gattr, grast, None, gdb, None, None
+        return gattr, grast, None, gdb, None, None, None
 
 
 def interpolate(attr, rast, tri, rast_db=None, diff_attrs=None):
@@ -452,13 +522,15 @@ def interpolate(attr, rast, tri, rast_db=None, diff_attrs=None):
         if len(idx):
             diff_idx = torch.tensor(idx, dtype=torch.int32, device=attr.device)
             rast_db = _as_f32(rast_db, 'rast_db')
-    boxes = None
+    boxes = src_pos = None
     meta = getattr(rast, '_drp_boxes', None)
     # boxes describe where each triangle's pixels lie; valid while rast is unmodified
     if meta is not None and meta[1] == tri.shape[0] and meta[2] == rast._version and \
             meta[0].shape[0] == rast.shape[0]:
         boxes = meta[0]
-    return _InterpolateFunc.apply(attr, rast, tri, rast_db if diff_idx is not None else None, diff_idx, boxes)
+        src_pos = meta[3]
+    return _InterpolateFunc.apply(attr, rast, tri, rast_db if diff_idx is not None else None, diff_idx, boxes,
+                                  src_pos)
 
 
 # ----------------------------------------------------------------------------
@@ -533,11 +605,16 @@ class _AntialiasFunc(torch.autograd.Function):
         gcol = torch.empty_like(color)
         gpos = torch.zeros_like(pos) if fctx.needs_input_grad[2] else None
         L = _lib.lib()
-        ws = fctx.ws
+        ws, reuse = fctx.ws, 1
+        if ws is None:
+            # a second backward through the same graph: the forward's workspace was
+            # released by the first one, so the pair decisions are recomputed
+            ws_bytes = L.drp_antialias_workspace_size(n, H, W, V, F, batched)
+            ws, reuse = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=color.device), 0
         rc = L.drp_antialias_bwd(stream_of(color), ptr(color), n, H, W, C, ptr(rast), ptr(pos), V, batched, ptr(tri),
-                                 F, ptr(opp), ptr(dy), ptr(gcol), ptr(gpos), float(fctx.boost), 1, ptr(ws),
+                                 F, ptr(opp), ptr(dy), ptr(gcol), ptr(gpos), float(fctx.boost), reuse, ptr(ws),
                                  ws.numel())
-        fctx.ws = None
+        fctx.ws = None  # released after its one reuse
         check(rc, 'antialias backward')
         return gcol, None, gpos, None, None, None
 

@@ -76,19 +76,59 @@ int drp_rasterize_fwd(void *stream, const float *pos, int n_img, int V, int pos_
  *   shape; the caller copies it unchanged to 16-byte aligned device memory
  *   (plan_dev) and passes both copies.
  *   rast [n_pix,4]: item i's H_i x W_i image at the sum of the previous items'
- *   pixel counts, bottom row first; ids are item-local (triangle - first
- *   triangle of the item + 1), so each image is exactly what rasterizing the
- *   item alone gives, and the per-item interpolate / antialias / backward
- *   calls apply to its slices.  prev (nullable) is the previous layer in the
- *   same layout (depth peeling).  boxes (nullable) int32 [n_tri,3] as for
- *   drp_rasterize_fwd, indexed by the batch triangle. */
+ *   pixel counts, bottom row first; with global_ids = 0 the ids are item-local
+ *   (triangle - first triangle of the item + 1), so each image is exactly what
+ *   rasterizing the item alone gives, and the per-item interpolate /
+ *   antialias / backward calls apply to its slices; with global_ids = 1 they
+ *   are the batch triangle + 1, the layout the drp_batch_* passes below take.
+ *   prev (nullable) is the previous layer in the same layout (depth peeling).
+ *   boxes (nullable) int32 [n_tri,3] as for drp_rasterize_fwd, indexed by the
+ *   batch triangle. */
 size_t drp_batch_plan_size(const int32_t *items, int n_img, int tile_w, int tile_h);
 int drp_batch_plan(const int32_t *items, int n_img, int tile_w, int tile_h, void *plan, size_t plan_bytes);
 int drp_batch_plan_info(const void *plan, long long *n_pix, long long *n_tiles, long long *n_tri, long long *n_vert);
 size_t drp_rasterize_batch_workspace_size(const void *plan);
 int drp_rasterize_batch_fwd(void *stream, const float *pos, const int32_t *tri, const void *plan,
                             const void *plan_dev, const float *prev, float *rast, float *rast_db, void *workspace,
-                            size_t workspace_bytes, int32_t *boxes);
+                            size_t workspace_bytes, int32_t *boxes, int global_ids);
+
+/* The rest of an uneven batch's forward and backward in one launch sequence
+ * per pass (the batched form of diffrp's per-object interpolate at
+ * diffrp/rendering/interpolator.py:83 and antialias at surface_deferred.py:775
+ * and of their autograd, for meshes of different sizes at different
+ * resolutions).  Buffers are the items' packed back to back as in the plan:
+ * rast [n_pix,4] from drp_rasterize_batch_fwd with global_ids = 1, colours and
+ * their gradients [n_pix,C], pos / grad_pos [n_vert,4] and attr / grad_attr
+ * [n_vert,C] concatenated, tri [n_tri,3] with batch (global) vertex indices
+ * (each item's triangles offset by the vertices before it), opp its topology
+ * table (drp_topology_build), boxes from the rasterize call, inc_off / inc the
+ * incidence table of tri (drp_incidence_build).  Gradients are summed in fixed
+ * orders without atomics, except the antialias position gradient (one float
+ * atomic per vertex component and 32 x 32 tile).  grad_attr, grad_pos are
+ * accumulated (caller zero-fills); grad_rast, grad_color written.  The
+ * interpolate backward takes C in {1, 2, 3, 4, 8, 16}.  The rasterize
+ * backward also takes the item-local ids of global_ids = 0 (local_ids = 1),
+ * and the gradient of rast_db (grad_db, nullable). */
+int drp_batch_interpolate_fwd(void *stream, const void *plan, const float *attr, int C, const float *rast,
+                              const int32_t *tri, float *out);
+size_t drp_batch_interpolate_bwd_workspace_size(const void *plan, int C);
+int drp_batch_interpolate_bwd(void *stream, const void *plan, const void *plan_dev, const float *attr, int C,
+                              const float *rast, const int32_t *tri, const float *grad_out, float *grad_attr,
+                              float *grad_rast, const int32_t *boxes, const int32_t *inc_off, const int32_t *inc,
+                              void *workspace, size_t workspace_bytes);
+size_t drp_batch_rasterize_bwd_workspace_size(const void *plan);
+int drp_batch_rasterize_bwd(void *stream, const void *plan, const void *plan_dev, const float *pos, const int32_t *tri,
+                            const float *rast, int local_ids, const float *grad_rast, const float *grad_db,
+                            float *grad_pos, const int32_t *boxes, const int32_t *inc_off, const int32_t *inc,
+                            void *workspace, size_t workspace_bytes);
+size_t drp_batch_antialias_workspace_size(const void *plan);
+int drp_batch_antialias_fwd(void *stream, const void *plan, const void *plan_dev, const float *color, int C,
+                            const float *rast, const float *pos, const int32_t *tri, const int32_t *opp, float *out,
+                            void *workspace, size_t workspace_bytes);
+int drp_batch_antialias_bwd(void *stream, const void *plan, const void *plan_dev, const float *color, int C,
+                            const float *rast, const float *pos, const int32_t *tri, const int32_t *opp,
+                            const float *grad_out, float *grad_color, float *grad_pos, float pos_gradient_boost,
+                            int reuse_forward, void *workspace, size_t workspace_bytes);
 
 /* Backward of rasterize: autograd of `rast` as diffrp consumes it
  * (u, v through diffrp/rendering/interpolator.py:42 and :92).  Accumulates
@@ -175,6 +215,23 @@ int drp_interpolate_bwd(void *stream, const float *attr, int n_img, int V, int a
                         int views_per_group, const int32_t *inc_off, const int32_t *inc, void *workspace,
                         size_t workspace_bytes);
 
+/* Interpolate backward together with the backward of the rasterize call that
+ * produced rast (the chain attr -> interpolate -> rast -> rasterize -> pos of
+ * diffrp/rendering/interpolator.py:42-45,83 when interpolate is the only
+ * consumer of rast's gradient): one pass over the pixels computes the
+ * attribute contributions, grad_rast (written, as drp_interpolate_bwd writes
+ * it) and that grad_rast's position contributions; one record search per
+ * (image group, triangle) sums both.  grad_attr and grad_pos are accumulated
+ * (caller zero-fills) and equal, bit for bit, drp_interpolate_bwd followed by
+ * drp_rasterize_bwd on the grad_rast it wrote.  attr [V,C] (not batched), C in
+ * {4, 8}; pos [V,4] or [n,V,4] (pos_batched); boxes from the rasterize call. */
+size_t drp_interpolate_rasterize_bwd_workspace_size(int n_img, int H, int W, int F, int C, int views_per_group);
+int drp_interpolate_rasterize_bwd(void *stream, const float *attr, int n_img, int V, int C, const float *rast, int H,
+                                  int W, const int32_t *tri, int F, const float *pos, int pos_batched,
+                                  const float *grad_out, float *grad_attr, float *grad_rast, float *grad_pos,
+                                  const int32_t *boxes, int views_per_group, const int32_t *inc_off,
+                                  const int32_t *inc, void *workspace, size_t workspace_bytes);
+
 /* ---- antialias -----------------------------------------------------------
  * Topology table for antialias: replaces dr.antialias_construct_topology_hash
  * (built implicitly by dr.antialias at diffrp/rendering/surface_deferred.py:775,
@@ -211,6 +268,28 @@ int drp_antialias_bwd(void *stream, const float *color, int n_img, int H, int W,
  * writes grad_frag_xy [H,W,2].  The colour gradient is grad_out unchanged. */
 int drp_edge_grad_bwd(void *stream, const float *grad_out, const float *rgb, int H, int W, int C, const float *rast,
                       const float *frag_xy, const float *tris_xy, int F, float *grad_frag_xy);
+
+/* ---- clip-space transform ------------------------------------------------
+ * Replaces SurfaceDeferredRenderSession.clip_space (diffrp/rendering/
+ * surface_deferred.py:229-238, transform_point of diffrp/utils/shader_ops.py:
+ * 123-136) for n views at once: xyz [V,3] world positions, VP [n,4,4]
+ * row-major view-projection matrices, pos [n,V,4] = VP_b [x, y, z, 1]^T
+ * (each row one fma chain m0 x + (m1 y + (m2 z + m3))).  Backward: grad_xyz
+ * [V,3] written, the views summed in view order (deterministic). */
+int drp_clip_transform_fwd(void *stream, const float *xyz, int V, const float *VP, int n, float *pos);
+int drp_clip_transform_bwd(void *stream, const float *grad_pos, int V, const float *VP, int n, float *grad_xyz);
+
+/* ---- image L2 loss ---------------------------------------------------------
+ * No counterpart on the reference path: the loss of a fitting step,
+ * loss = sum (a - b)^2 over n floats (one read of each image; the sum is
+ * deterministic, accumulated in double and rounded to float), and its
+ * backward grad_a = 2 g (a - b), grad_b = -grad_a (either may be null) with g
+ * the device scalar *grad_loss. */
+size_t drp_l2_loss_workspace_size(void);
+int drp_l2_loss_fwd(void *stream, const float *a, const float *b, long long n, float *loss, void *workspace,
+                    size_t workspace_bytes);
+int drp_l2_loss_bwd(void *stream, const float *a, const float *b, long long n, const float *grad_loss, float *grad_a,
+                    float *grad_b);
 
 #ifdef __cplusplus
 }

@@ -93,6 +93,53 @@ def gbuffer_collect_single(rast, tris, stencils, values_fn: Callable, default: L
     return buf.index_put(interp.indices, values_fn(interp)), interp
 
 
+def vertex_array_object(objects):
+    """diffrp/rendering/mixin.py:74-113 for objects given as dicts with 'verts'
+    [V,3], 'tris' [F,3], 'M' [4,4] and per-vertex attributes: world positions
+    (transform_point4x3 with each object's M), triangles offset by the vertices
+    of the objects before, stencils [0] ++ (object index + 1 per triangle), and
+    every other attribute concatenated."""
+    world, tris, sts, offset = [], [], [torch.zeros(1, dtype=torch.int64)], 0
+    for s, o in enumerate(objects):
+        world.append(transform_point4x3(o['verts'], o['M']))
+        tris.append(o['tris'].long() + offset)
+        sts.append(torch.full([len(o['tris'])], s + 1, dtype=torch.int64))
+        offset += len(o['verts'])
+    keys = [k for k in objects[0] if k not in ('verts', 'tris', 'M')]
+    out = {'world_pos': torch.cat(world), 'tris': torch.cat(tris).int(), 'stencils': torch.cat(sts).int(),
+           'verts': torch.cat([o['verts'] for o in objects])}
+    for k in keys:
+        out[k] = torch.cat([o[k] for o in objects])
+    return out
+
+
+def gbuffer_collect_layer(rast, tris, stencils, n_obj, operator: Callable, default: List[float]):
+    """One layer of the stencil-masked G-buffer path with any number of objects
+    (surface_deferred.py:283-293 and :324-332, mixin.py:130-155): per object an
+    interpolator over the pixels whose stencil is its index + 1, skipped when
+    empty; ``operator(object index, interpolator)`` gives its [P,C] values or
+    None; the default-filled [1,H,W,C] buffer receives one index_put (the
+    objects' indices and values concatenated when more than one returns values)."""
+    stencil_buf = stencils[float_to_triidx(rast[..., -1])]
+    H, W = rast.shape[1:3]
+    buf = torch.tensor(default, dtype=rast.dtype).expand(1, H, W, len(default))
+    indices, values, nind = [], [], 0
+    for pi in range(n_obj):
+        interp = MaskedSparseInterpolator(rast, tris, stencil_buf == pi + 1)
+        if len(interp.indices[0]) == 0:
+            continue
+        op = operator(pi, interp)
+        if op is not None:
+            nind = len(interp.indices)
+            indices.append(interp.indices)
+            values.append(op)
+    if len(indices) == 1:
+        buf = buf.index_put(indices[0], values[0])
+    elif nind != 0:
+        buf = buf.index_put(tuple(torch.cat([x[i] for x in indices]) for i in range(nind)), torch.cat(values))
+    return buf
+
+
 def alpha_blend(bgc, bga, fgc, fga):
     """diffrp/utils/composite.py:28-29."""
     return torch.lerp(bgc, fgc, fga), torch.addcmul(fga, bga, 1 - fga)
