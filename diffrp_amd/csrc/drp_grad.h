@@ -17,10 +17,13 @@ namespace drp {
 // ----------------------------------------------------------------------------
 __device__ __forceinline__ double dfma(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
+// NDC centre of pixel column c (row r) in double: (2c + 1 - W) / W
+__device__ __forceinline__ double pix_ndc_d(int c, int W) { return (double)(2 * c + 1 - W) / (double)W; }
+
+// px, py: the pixel's NDC centre, pix_ndc_d(c, W), pix_ndc_d(r, H)
 template <bool DB>
-__device__ __forceinline__ void raster_pixel_grad(const TriVerts &v, int c, int r, int H, int W, float4 g, float4 gd,
-                                                  float4 out[3]) {
-    const double px = (double)(2 * c + 1 - W) / (double)W, py = (double)(2 * r + 1 - H) / (double)H;
+__device__ __forceinline__ void raster_pixel_grad_at(const TriVerts &v, double px, double py, float4 g, float4 gd,
+                                                     float4 out[3]) {
     double X[3], Y[3], wk[3], zk[3], e[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -79,6 +82,12 @@ __device__ __forceinline__ void raster_pixel_grad(const TriVerts &v, int c, int 
         out[k] = make_float4((float)gX[k], (float)gY[k], (float)(gzw * ek),
                              (float)(gw[k] - px * gX[k] - py * gY[k] - gzw * zw * ek));
     }
+}
+
+template <bool DB>
+__device__ __forceinline__ void raster_pixel_grad(const TriVerts &v, int c, int r, int H, int W, float4 g, float4 gd,
+                                                  float4 out[3]) {
+    raster_pixel_grad_at<DB>(v, pix_ndc_d(c, W), pix_ndc_d(r, H), g, gd, out);
 }
 
 __device__ __forceinline__ float4 db_grad_scaled(float4 gd, int H, int W) {

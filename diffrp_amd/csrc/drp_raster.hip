@@ -195,6 +195,7 @@ __device__ __forceinline__ int rec_cls(int box) { return (box >> 18) & 3; }
 // column) and settled by one exact 64-bit step either way, so the span is
 // exactly the set of pixels whose three edge functions are non-negative.
 __device__ __forceinline__ void row_span(const FineRec &R, int box, int y, int &xl, int &xr) {
+    // branch-free per edge: the lanes of a wave take edges of either direction
 #pragma unroll
     for (int e = 0; e < 3; ++e) {
         const int j = e == 2 ? 0 : e + 1;
@@ -202,23 +203,18 @@ __device__ __forceinline__ void row_span(const FineRec &R, int box, int y, int &
         const long long a = -(long long)dy * kSubpix;
         const long long cc = (long long)dx * (kSubpix * y - R.Y[e]) + (long long)dy * R.X[e] -
                              (((box >> (20 + e)) & 1) ? 0 : 1);
-        if (a == 0) {
-            if (cc < 0) xr = xl - 1;
-            continue;
-        }
-        double q = -(double)cc * (double)__builtin_amdgcn_rcpf((float)(-dy) * (float)kSubpix);
+        const bool flat = a == 0, rising = a > 0;  // rising: covered for x >= crossing, else x <= crossing
+        const float den = flat ? 1.0f : (float)(-dy) * (float)kSubpix;
+        double q = -(double)cc * (double)__builtin_amdgcn_rcpf(den);
         q = fmin(fmax(q, (double)(xl - 1)), (double)(xr + 1));
-        if (a > 0) {  // covered for x >= crossing
-            int x0 = (int)ceil(q);
-            if (cc + a * (x0 - 1) >= 0) --x0;
-            else if (cc + a * x0 < 0) ++x0;
-            xl = max(xl, x0);
-        } else {      // covered for x <= crossing
-            int x0 = (int)floor(q);
-            if (cc + a * (x0 + 1) >= 0) ++x0;
-            else if (cc + a * x0 < 0) --x0;
-            xr = min(xr, x0);
-        }
+        int x0 = (int)(rising ? ceil(q) : floor(q));
+        const int d = rising ? -1 : 1;  // the step towards the uncovered side
+        if (cc + a * (x0 + d) >= 0) x0 += d;
+        else if (cc + a * x0 < 0) x0 -= d;
+        const int nxl = flat ? xl : (rising ? max(xl, x0) : xl);
+        const int nxr = flat ? (cc < 0 ? xl - 1 : xr) : (rising ? xr : min(xr, x0));
+        xl = nxl;
+        xr = nxr;
     }
 }
 
@@ -233,29 +229,25 @@ __device__ __forceinline__ void row_span(const FineRec &R, int box, int y, int &
 // coordinate limit included, the reciprocal perturbed by one ulp).
 constexpr int kSmallRec = 1 << 14;
 __device__ __forceinline__ void row_span32(const FineRec &R, int box, int y, int &xl, int &xr) {
+    // branch-free per edge, the same decisions as row_span in 32-bit arithmetic
 #pragma unroll
     for (int e = 0; e < 3; ++e) {
         const int j = e == 2 ? 0 : e + 1;
         const int dx = R.X[j] - R.X[e], dy = R.Y[j] - R.Y[e];
         const int a = -dy * kSubpix;
         const int cc = dx * (kSubpix * y - R.Y[e]) + dy * R.X[e] - (((box >> (20 + e)) & 1) ? 0 : 1);
-        if (a == 0) {
-            if (cc < 0) xr = xl - 1;
-            continue;
-        }
-        float q = -(float)cc * __builtin_amdgcn_rcpf((float)(-dy) * (float)kSubpix);
+        const bool flat = a == 0, rising = a > 0;
+        const float den = flat ? 1.0f : (float)(-dy) * (float)kSubpix;
+        float q = -(float)cc * __builtin_amdgcn_rcpf(den);
         q = fminf(fmaxf(q, (float)(xl - 1)), (float)(xr + 1));
-        if (a > 0) {
-            int x0 = (int)ceilf(q);
-            if (cc + a * (x0 - 1) >= 0) --x0;
-            else if (cc + a * x0 < 0) ++x0;
-            xl = max(xl, x0);
-        } else {
-            int x0 = (int)floorf(q);
-            if (cc + a * (x0 + 1) >= 0) ++x0;
-            else if (cc + a * x0 < 0) --x0;
-            xr = min(xr, x0);
-        }
+        int x0 = (int)(rising ? ceilf(q) : floorf(q));
+        const int d = rising ? -1 : 1;
+        if (cc + a * (x0 + d) >= 0) x0 += d;
+        else if (cc + a * x0 < 0) x0 -= d;
+        const int nxl = flat ? xl : (rising ? max(xl, x0) : xl);
+        const int nxr = flat ? (cc < 0 ? xl - 1 : xr) : (rising ? xr : min(xr, x0));
+        xl = nxl;
+        xr = nxr;
     }
 }
 
@@ -295,6 +287,7 @@ struct FineLds {
     int queue[kQueue];                  // covered pixels: record | x << 8 | y << 16 (tile-local)
     unsigned long long zbuf[NT];        // depth-test keys
     unsigned long long pkey[PEEL ? NT : 1];
+    float pxs[TW], pys[TH];             // NDC centres of the tile's columns and rows (one division each per tile)
     int wsum[3 * (NT / 64)];            // scan partials: records; row spans (two, alternating)
     int bcast;                          // block-wide broadcast slot
 };
@@ -348,6 +341,8 @@ __device__ __forceinline__ FineTile fine_begin(FineLds<TW, TH, PEEL> &S, const B
     T.tc0 = tx * TW;
     T.tr0 = ty * TH;
     S.zbuf[tid] = ~0ull;
+    if (tid < TW) S.pxs[tid] = pix_ndc(T.tc0 + tid, d.W);
+    if (tid < TH) S.pys[tid] = pix_ndc(T.tr0 + tid, d.H);
     if (PEEL) {
         unsigned long long pk = ~0ull;  // empty previous layer: nothing can follow
         const int c = T.tc0 + (tid % TW), r = T.tr0 + (tid / TW);
@@ -466,7 +461,7 @@ __device__ __forceinline__ void fine_entries(FineLds<TW, TH, PEEL> &S, const Fin
                     const FineRec &R = S.rec[en & 255];
                     int lx = (en >> 8) & 255, ly = en >> 16;
                     TriVerts v = R.v;
-                    float px = pix_ndc(tc0 + lx, d.W), py = pix_ndc(tr0 + ly, d.H);
+                    float px = S.pxs[lx], py = S.pys[ly];
                     PixEval p;
                     eval_pixel(v, px, py, p);
                     if ((rec_cls(R.box) != kHomog || homog_inside(p)) && p.s != 0.f) {
@@ -490,7 +485,7 @@ template <int TW, bool DB>
 __device__ __forceinline__ void fine_output(const FineTile &T, unsigned long long key, const Batch &b,
                                             const float *__restrict__ pos, const int *__restrict__ tri,
                                             float4 *__restrict__ rast, float4 *__restrict__ rast_db,
-                                            ItemBox *__restrict__ boxes) {
+                                            ItemBox *__restrict__ boxes, float px, float py) {
     const ImageDesc &d = T.d;
     const int c = T.tc0 + ((int)threadIdx.x % TW), r = T.tr0 + ((int)threadIdx.x / TW);
     if (c >= d.W || r >= d.H) return;
@@ -504,7 +499,6 @@ __device__ __forceinline__ void fine_output(const FineTile &T, unsigned long lon
     float zw = inv_ord_f32((uint32_t)(key >> 32));
     TriVerts v;
     load_tri(pos, tri, t, d, v);
-    float px = pix_ndc(c, d.W), py = pix_ndc(r, d.H);
     PixEval p;
     eval_pixel(v, px, py, p);
     float u = p.e[0] / p.s, vv = p.e[1] / p.s;
@@ -588,7 +582,8 @@ __global__ void __launch_bounds__(TW *TH, kFineWavesPerEU) k_fine(Batch b, const
         sp.gkey[(long long)slot * kMaxFineNT + threadIdx.x] = S.zbuf[threadIdx.x];
         return;
     }
-    fine_output<TW, DB>(T, S.zbuf[threadIdx.x], b, pos, tri, rast, rast_db, boxes);
+    fine_output<TW, DB>(T, S.zbuf[threadIdx.x], b, pos, tri, rast, rast_db, boxes, S.pxs[threadIdx.x % TW],
+                        S.pys[threadIdx.x / TW]);
 }
 
 // chunks 1.. of split tiles: a fixed grid drains the queue k_fine filled
@@ -631,7 +626,7 @@ __global__ void __launch_bounds__(TW *TH, kFineWavesPerEU) k_fine_split(Batch b,
         if (last) {
             __threadfence();
             const unsigned long long key = atomicMin(g, ~0ull);  // read at the point of coherence
-            fine_output<TW, DB>(T, key, b, pos, tri, rast, rast_db, boxes);
+            fine_output<TW, DB>(T, key, b, pos, tri, rast, rast_db, boxes, S.pxs[tid % TW], S.pys[tid / TW]);
         }
         __syncthreads();  // S is reused by the next item
     }

@@ -93,6 +93,7 @@ __global__ void __launch_bounds__(256) k_tile_reduce(TriBwdLaunch L) {
     constexpr bool QUAD = NV % 4 == 0;
     __shared__ __align__(16) float contrib[QUAD ? NV * kBTP : NV * kRowStride];
     __shared__ int wsum[4];
+    __shared__ double spx[kBT], spy[kBT];  // NDC centres of the tile's columns and rows (position chain)
     const int tid = threadIdx.x;
     const int tile = xcd_remap(blockIdx.x, L.n_tiles_pad);
     if (tile >= L.n_tiles) return;  // uniform per block
@@ -112,6 +113,11 @@ __global__ void __launch_bounds__(256) k_tile_reduce(TriBwdLaunch L) {
         pix0 = (long long)img * L.H * L.W;
     }
     const int r = (lt / tiles_x) * kBT + (tid >> 4), c = (lt % tiles_x) * kBT + (tid & 15);
+    if (MODE != 1) {  // one double division per column and row of the tile, not two per pixel
+        if (tid < kBT) spx[tid] = pix_ndc_d((lt % tiles_x) * kBT + tid, W);
+        else if (tid < 2 * kBT) spy[tid - kBT] = pix_ndc_d((lt / tiles_x) * kBT + tid - kBT, H);
+        __syncthreads();
+    }
     float val[NV];
 #pragma unroll
     for (int j = 0; j < NV; ++j) val[j] = 0.f;
@@ -133,7 +139,7 @@ __global__ void __launch_bounds__(256) k_tile_reduce(TriBwdLaunch L) {
                     float4 gd = make_float4(0.f, 0.f, 0.f, 0.f);
                     if (DB) gd = db_grad_scaled(reinterpret_cast<const float4 *>(L.grad_db)[pix], H, W);
                     float4 o[3];
-                    raster_pixel_grad<DB>(tv, c, r, H, W, g, gd, o);
+                    raster_pixel_grad_at<DB>(tv, spx[tid & 15], spy[tid >> 4], g, gd, o);
 #pragma unroll
                     for (int q = 0; q < 3; ++q) {
                         val[4 * q] = o[q].x; val[4 * q + 1] = o[q].y; val[4 * q + 2] = o[q].z; val[4 * q + 3] = o[q].w;
@@ -193,7 +199,8 @@ __global__ void __launch_bounds__(256) k_tile_reduce(TriBwdLaunch L) {
                         TriVerts tv;
                         if (load_tri(L.pos, L.tri, t, d, tv)) {
                             float4 o[3];
-                            raster_pixel_grad<false>(tv, c, r, H, W, gr, make_float4(0.f, 0.f, 0.f, 0.f), o);
+                            raster_pixel_grad_at<false>(tv, spx[tid & 15], spy[tid >> 4], gr,
+                                                        make_float4(0.f, 0.f, 0.f, 0.f), o);
 #pragma unroll
                             for (int q = 0; q < 3; ++q) {
                                 val[3 * C + 4 * q] = o[q].x; val[3 * C + 4 * q + 1] = o[q].y;

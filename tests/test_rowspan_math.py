@@ -45,19 +45,17 @@ def _span32(X, Y, edges, y, xl, xr, rng):
             continue
         q = F32(F32(-F32(cc)) * _rcp(F32(F32(-dy) * F32(16)), rng))
         q = min(max(q, F32(xl - 1)), F32(xr + 1))
-        if a > 0:
-            x0 = int(math.ceil(q))
-            if _i32(cc + _i32(a * (x0 - 1))) >= 0:
-                x0 -= 1
-            elif _i32(cc + _i32(a * x0)) < 0:
-                x0 += 1
+        # one step d towards the uncovered side, the same for both directions
+        rising = a > 0
+        x0 = int(math.ceil(q)) if rising else int(math.floor(q))
+        d = -1 if rising else 1
+        if _i32(cc + _i32(a * (x0 + d))) >= 0:
+            x0 += d
+        elif _i32(cc + _i32(a * x0)) < 0:
+            x0 -= d
+        if rising:
             xl = max(xl, x0)
         else:
-            x0 = int(math.floor(q))
-            if _i32(cc + _i32(a * (x0 + 1))) >= 0:
-                x0 += 1
-            elif _i32(cc + _i32(a * x0)) < 0:
-                x0 -= 1
             xr = min(xr, x0)
     return xl, xr
 
