@@ -29,6 +29,9 @@ _SIGS = {
     'drp_rasterize_fwd': (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p, c_int, c_void_p, c_int, c_int,
                                   c_void_p, c_void_p, c_void_p, c_void_p, c_size, c_int, c_int, c_int, c_ll, c_int,
                                   c_void_p]),
+    'drp_rasterize_fwd_noalloc': (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p, c_int, c_void_p, c_int,
+                                          c_int, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_ll, c_int,
+                                          c_void_p]),
     'drp_rasterize_bwd_workspace_size': (c_size, [c_int, c_int, c_int, c_int]),
     'drp_rasterize_bwd': (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p, c_int, c_void_p, c_int, c_int,
                                   c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,

@@ -8,6 +8,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <mutex>
 #include <string>
 
 #include "../../include/drp.h"
@@ -156,6 +157,58 @@ int drp_rasterize_fwd(void *stream, const float *pos, int n_img, int V, int pos_
     int rc = launch_rasterize(L, (hipStream_t)stream);
     if (rc) return fail(rc, "rasterize launch failed: %s", hipGetErrorString(hipGetLastError()));
     return 0;
+}
+
+// ---- allocation-free rasterize ---------------------------------------------
+// The workspace of drp_rasterize_fwd_noalloc is one file-scope device array,
+// `scratch`, of kNoAllocScratch bytes: each GPU the library runs on holds its
+// own copy until the process ends.  Host code never passes the array's name
+// to a kernel (on the host it is only a stand-in symbol): its device address
+// is looked up once per device, with that device current, and kept here.
+namespace {
+constexpr size_t kNoAllocScratch = DRP_NOALLOC_SCRATCH_BYTES;
+constexpr int kMaxScratchDevices = 64;
+__device__ __align__(256) unsigned char scratch[kNoAllocScratch];
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxScratchDevices];
+
+int scratch_address(void **out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return fail(-2, "hipGetDevice failed: %s", hipGetErrorString(e));
+    if (dev < 0 || dev >= kMaxScratchDevices) return fail(-1, "device %d beyond the %d supported", dev, kMaxScratchDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(-2, "hipGetSymbolAddress(scratch) failed on device %d: %s", dev, hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+}  // namespace
+
+int drp_rasterize_fwd_noalloc(void *stream, const float *pos, int n_img, int V, int pos_batched,
+                              const int32_t *tri, int F, const int32_t *ranges_dev, int H, int W, const float *prev,
+                              float *rast, float *rast_db, int tile_w, int tile_h, int bin_cap, long long list_total,
+                              int max_range, int32_t *boxes) {
+    if (int e = check_dims(n_img, H, W)) return e;
+    if (n_img == 0) return 0;
+    if (!tile_ok(tile_w, tile_h)) return fail(-1, "unsupported tile shape %dx%d", tile_w, tile_h);
+    if (bin_cap < 1) return fail(-1, "bin capacity must be >= 1");
+    if (F < 0) return fail(-1, "negative F (%d)", F);
+    long long lt = ranges_dev ? list_total : (long long)n_img * F;
+    size_t need = drp_rasterize_workspace_size(n_img, H, W, lt, tile_w, tile_h, bin_cap);
+    if (need > kNoAllocScratch)
+        return fail(-3, "rasterize needs %zu bytes of scratch; the allocation-free entry point has %zu "
+                        "(use drp_rasterize_fwd with a workspace, or fewer images / a smaller bin capacity)",
+                    need, kNoAllocScratch);
+    void *ws = nullptr;
+    if (int e = scratch_address(&ws)) return e;
+    return drp_rasterize_fwd(stream, pos, n_img, V, pos_batched, tri, F, ranges_dev, H, W, prev, rast, rast_db, ws,
+                             kNoAllocScratch, tile_w, tile_h, bin_cap, list_total, max_range, boxes);
 }
 
 // ---- uneven batches --------------------------------------------------------

@@ -63,6 +63,23 @@ int drp_rasterize_fwd(void *stream, const float *pos, int n_img, int V, int pos_
                       void *workspace, size_t workspace_bytes, int tile_w, int tile_h, int bin_cap,
                       long long list_total, int max_range, int32_t *boxes);
 
+/* ---- rasterize without allocation --------------------------------------
+ * No counterpart in the reference: for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  drp_rasterize_fwd with the same arguments except the
+ * workspace: the scratch is a device array of DRP_NOALLOC_SCRATCH_BYTES
+ * inside the library, one per GPU, held until the process ends; calls whose
+ * drp_rasterize_workspace_size exceeds it fail (-3) with a message.  Every
+ * call on a GPU shares that GPU's array: calls that overlap on one GPU (two
+ * streams, two threads) race.  The first call on a device looks the array's
+ * address up (make that first call outside graph capture); the device is the
+ * current one.  Results equal drp_rasterize_fwd's bit for bit. */
+#define DRP_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int drp_rasterize_fwd_noalloc(void *stream, const float *pos, int n_img, int V, int pos_batched,
+                              const int32_t *tri, int F, const int32_t *ranges_dev, int H, int W, const float *prev,
+                              float *rast, float *rast_db, int tile_w, int tile_h, int bin_cap, long long list_total,
+                              int max_range, int32_t *boxes);
+
 /* ---- uneven batches ------------------------------------------------------
  * Many meshes of different triangle counts, each at its own resolution, in
  * one rasterizer launch (the batched form of the DepthPeeler calls at
