@@ -53,6 +53,8 @@ def algorithmic_bytes(n_views, H, W, V, F, C, layers=1):
 
 # per-operator algorithmic bytes for one launch over n views (fwd or bwd half)
 def op_bytes(op, n, N, V, F, C):
+    if op == 'interpolate_rasterize_bwd':  # the two backward operators' bytes: the fused pass replaces both calls
+        return op_bytes('interpolate_bwd', n, N, V, F, C) + op_bytes('rasterize_bwd', n, N, V, F, C)
     return {
         'rasterize_fwd': n * (16 * N + 16 * V + 12 * F),
         'rasterize_bwd': n * (32 * N + 32 * V + 12 * F),
@@ -197,6 +199,8 @@ def per_op_times(S, iters=10):
     iws_t = torch.empty(iws, dtype=torch.uint8, device=pos.device)
     pws = L.drp_rasterize_bwd_workspace_size(n, H, W, F)
     pws_t = torch.empty(pws, dtype=torch.uint8, device=pos.device)
+    fws = L.drp_interpolate_rasterize_bwd_workspace_size(n, H, W, F, C, vpg)
+    fws_t = torch.empty(fws, dtype=torch.uint8, device=pos.device)
     s = stream_of(pos)
     fns = {
         'rasterize_fwd': lambda: L.drp_rasterize_fwd(s, ptr(pos), n, V, 1, ptr(S.tri), F, None, H, W, None, ptr(rout),
@@ -214,6 +218,10 @@ def per_op_times(S, iters=10):
         'rasterize_bwd': lambda: L.drp_rasterize_bwd(s, ptr(pos), n, V, 1, ptr(S.tri), F, None, H, W, ptr(rast),
                                                      ptr(grast), None, ptr(gpos), ptr(boxes), ptr(off), ptr(inc),
                                                      ptr(pws_t), pws),
+        # what the step runs for the two backward passes above: one pass over the pixels
+        'interpolate_rasterize_bwd': lambda: L.drp_interpolate_rasterize_bwd(
+            s, ptr(attr), n, V, C, ptr(rast), H, W, ptr(S.tri), F, ptr(pos), 1, ptr(gout), ptr(gattr), ptr(grast),
+            ptr(gpos), ptr(boxes), vpg, ptr(off), ptr(inc), ptr(fws_t), fws),
     }
     res = {}
     for k, f in fns.items():
@@ -221,6 +229,10 @@ def per_op_times(S, iters=10):
         res[k] = {'ms': t * 1e3, 'bytes': op_bytes(k, n, H * W, V, F, C)}
         res[k]['gbs'] = res[k]['bytes'] / t / 1e9
     return res
+
+
+# the operator calls of the captured step (interpolate and rasterize backward run as one pass there)
+STEP_OPS = ('rasterize_fwd', 'interpolate_fwd', 'antialias_fwd', 'antialias_bwd', 'interpolate_rasterize_bwd')
 
 
 # kernels each operator call launches (the unit of the per-operator timings)
@@ -231,6 +243,8 @@ OP_KERNELS = {
     'antialias_bwd': ['k_aa_pairs_bwd', 'k_aa_gather_bwd'],
     'interpolate_bwd': ['k_tile_reduce<1,C>', 'k_tile_gather<1,C>', 'k_gather_attr'],
     'rasterize_bwd': ['k_tile_reduce<0>', 'k_tile_gather<0>', 'k_gather_pos'],
+    'interpolate_rasterize_bwd': ['k_corner_slots_clear', 'k_corner_slots', 'k_tile_reduce<2,C>',
+                                  'k_tile_gather<2,C>', 'k_gather_attr', 'k_gather_pos'],
 }
 
 TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'profiles', 'traffic_64view_512.txt')
@@ -601,7 +615,7 @@ def main():
     progress('operators timed')
     roof = None
     if ops:
-        dom = max(ops, key=lambda k: ops[k]['ms'])
+        dom = max(STEP_OPS, key=lambda k: ops[k]['ms'])
         roof = {'bound': 'hbm', 'kernel': dom, 'kernels': OP_KERNELS[dom], 'achieved': round(ops[dom]['gbs'], 1),
                 'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': round(ops[dom]['gbs'] / HBM_PEAK_GBS, 4),
                 'traffic': measured_traffic(dom, S, args.subdiv),

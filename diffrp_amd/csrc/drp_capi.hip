@@ -471,7 +471,7 @@ size_t drp_batch_interpolate_bwd_workspace_size(const void *plan, int C) {
     if (!plan || C <= 0) return 0;
     memcpy(&v.h, plan, sizeof(v.h));
     long long tiles = plan_map(plan, v.h.n_img, v.h.n_tiles).bt16[v.h.n_img];
-    return batch_interp_partial_bytes(v.h, C) + tile_records_bytes_tiles(tiles, 3 * C);
+    return batch_interp_partial_bytes(v.h, C) + tile_records_bytes_tiles(tiles, 3 * C, v.h.n_tri);
 }
 
 int drp_batch_interpolate_bwd(void *stream, const void *plan, const void *plan_dev, const float *attr, int C,
@@ -527,7 +527,8 @@ size_t drp_batch_rasterize_bwd_workspace_size(const void *plan) {
     memcpy(&h, plan, sizeof(h));
     if (h.n_tri <= 0) return 0;
     long long tiles = plan_map(plan, h.n_img, h.n_tiles).bt16[h.n_img];
-    return align256((size_t)h.n_tri * 3 * 16) + tile_records_bytes_tiles(tiles, 12) + align256((size_t)h.n_tri * 3 * 4);
+    return align256((size_t)h.n_tri * 3 * 16) + tile_records_bytes_tiles(tiles, 12, h.n_tri) +
+           align256((size_t)h.n_tri * 3 * 4);
 }
 
 int drp_batch_rasterize_bwd(void *stream, const void *plan, const void *plan_dev, const float *pos, const int32_t *tri,
@@ -565,7 +566,7 @@ int drp_batch_rasterize_bwd(void *stream, const void *plan, const void *plan_dev
     L.boxes = (const ItemBox *)boxes;
     L.partial = (float *)workspace;
     L.records = (char *)workspace + align256((size_t)v.h.n_tri * 3 * 16);
-    L.corner_slot = (int *)((char *)L.records + tile_records_bytes_tiles(v.bt16, 12));
+    L.corner_slot = (int *)((char *)L.records + tile_records_bytes_tiles(v.bt16, 12, v.h.n_tri));
     L.inc_off = inc_off;
     L.inc = inc;
     L.um = v.um;
@@ -663,7 +664,8 @@ int drp_batch_antialias_bwd(void *stream, const void *plan, const void *plan_dev
 
 size_t drp_rasterize_bwd_workspace_size(int n_img, int H, int W, int F) {
     if (n_img <= 0 || F <= 0 || H <= 0 || W <= 0) return 0;
-    return align256((size_t)n_img * (size_t)F * 3 * 16) + tile_records_bytes(n_img, H, W, 12) +
+    return align256((size_t)n_img * (size_t)F * 3 * 16) +
+           tile_records_bytes(n_img, H, W, 12, (long long)n_img * F) +
            align256((size_t)F * 3 * 4);
 }
 
@@ -689,7 +691,7 @@ int drp_rasterize_bwd(void *stream, const float *pos, int n_img, int V, int pos_
         L.boxes = (const ItemBox *)boxes;
         L.partial = (float *)workspace;
         L.records = (char *)workspace + align256((size_t)n_img * (size_t)F * 3 * 16);
-        L.corner_slot = (int *)((char *)L.records + tile_records_bytes(n_img, H, W, 12));
+        L.corner_slot = (int *)((char *)L.records + tile_records_bytes(n_img, H, W, 12, (long long)n_img * F));
         L.inc_off = inc_off;
         L.inc = inc;
     }
@@ -755,7 +757,8 @@ static size_t interp_partial_bytes(int n_img, int F, int C, int attr_batched, in
 size_t drp_interpolate_bwd_workspace_size(int n_img, int H, int W, int F, int C, int attr_batched,
                                           int views_per_group) {
     if (n_img <= 0 || F <= 0 || C <= 0 || H <= 0 || W <= 0) return 0;
-    return interp_partial_bytes(n_img, F, C, attr_batched, views_per_group) + tile_records_bytes(n_img, H, W, 3 * C);
+    return interp_partial_bytes(n_img, F, C, attr_batched, views_per_group) +
+           tile_records_bytes(n_img, H, W, 3 * C, (long long)n_img * F);
 }
 
 int drp_interpolate_bwd(void *stream, const float *attr, int n_img, int V, int attr_batched, int C,
@@ -820,7 +823,7 @@ size_t drp_interpolate_rasterize_bwd_workspace_size(int n_img, int H, int W, int
     const int vpg = views_per_block(views_per_group);
     const size_t groups = (size_t)((n_img + vpg - 1) / vpg);
     return align256(groups * (size_t)F * 3 * (size_t)C * 4) + align256((size_t)n_img * (size_t)F * 3 * 16) +
-           tile_records_bytes(n_img, H, W, 3 * C + 12) + align256((size_t)F * 3 * 4);
+           tile_records_bytes(n_img, H, W, 3 * C + 12, (long long)n_img * F) + align256((size_t)F * 3 * 4);
 }
 
 int drp_interpolate_rasterize_bwd(void *stream, const float *attr, int n_img, int V, int C, const float *rast, int H,
@@ -849,7 +852,7 @@ int drp_interpolate_rasterize_bwd(void *stream, const float *attr, int n_img, in
     float *partial_pos = (float *)b;
     b += align256((size_t)n_img * (size_t)F * 3 * 16);
     void *records = b;
-    b += tile_records_bytes(n_img, H, W, 3 * C + 12);
+    b += tile_records_bytes(n_img, H, W, 3 * C + 12, (long long)n_img * F);
     int *slot = (int *)b;
     hipStream_t st = (hipStream_t)stream;
     if (int rc = launch_corner_slots(inc, inc_off, V, F, slot, st))
@@ -878,7 +881,7 @@ int drp_interpolate_rasterize_bwd(void *stream, const float *attr, int n_img, in
     T.H = H;
     T.W = W;
     T.views_per_block = vpg;
-    tile_records_layout(T, records, n_img, H, W, 3 * C + 12);
+    tile_records_layout(T, records, n_img, H, W, (long long)n_img * F);
     int rc = launch_tri_bwd_fused(T, C, st);
     if (!rc) rc = launch_gather_attr(partial_attr, groups, F, V, C, inc_off, inc, grad_attr, st);
     if (!rc) rc = launch_gather_pos(partial_pos, n_img, F, V, pos_batched, inc_off, grad_pos, st);

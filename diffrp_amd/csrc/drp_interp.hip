@@ -245,9 +245,9 @@ static void launch_interp_bwd_tri_t(const InterpLaunch &L, hipStream_t st) {
     if (L.um.n) {
         T.um = L.um;
         T.views_per_block = 1;
-        tile_records_layout_tiles(T, L.records, L.um_tiles);
+        tile_records_layout_tiles(T, L.records, L.um_tiles, L.F);
     } else {
-        tile_records_layout(T, L.records, n_img, L.H, L.W, 3 * NC);
+        tile_records_layout(T, L.records, n_img, L.H, L.W, (long long)n_img * L.F);
     }
     launch_tri_bwd_attr(T, NC, st);
     int outs = L.attr_batched ? n_img : 1;

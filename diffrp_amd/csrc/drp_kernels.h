@@ -78,7 +78,7 @@ struct RasterBwdLaunch {
     // triangle-centric path (all set) or per-pixel atomics (any null)
     const ItemBox *boxes;
     float *partial;          // [n_img, 3F] float4, corners in vertex (CSR) order
-    void *records;           // tile records (tile_records_bytes(n_img, H, W, 12))
+    void *records;           // tile records (tile_records_bytes(n_img, H, W, 12, n_img * F))
     int *corner_slot;        // [3F]: position of corner 3t+k in the incidence list
     const int *inc_off, *inc;
     UnevenMap um;            // uneven batch (um.n > 0): packed items, global vertices
@@ -104,7 +104,7 @@ struct InterpLaunch {
     int H, W;
     const ItemBox *boxes;
     float *partial;          // [groups, F, 3, C]
-    void *records;           // tile records (tile_records_bytes(n_img, H, W, 3 C))
+    void *records;           // tile records (tile_records_bytes(n_img, H, W, 3 C, n_img * F))
     const int *inc_off, *inc;
     int views_per_group;
     UnevenMap um;            // uneven batch (um.n > 0): packed items, global ids and vertices, one group
@@ -141,6 +141,9 @@ struct AALaunch {
     int total_blk, total_t32;    // uneven batch: pair blocks and backward tiles of all items
 };
 
+// tiles of a triangle's pixel box whose record index the tile pass stores
+constexpr int kBoxSlots = 4;
+
 struct TriBwdLaunch {
     Batch b;
     const float *pos;
@@ -158,6 +161,7 @@ struct TriBwdLaunch {
     // per-tile records (16 x 16 tiles): count, triangle, sums
     int *rec_count, *rec_t;
     float *rec_v;
+    int *box_slot;           // [boxes, kBoxSlots] record index of the box's first tiles (nullptr: search)
     int tiles_x, tiles_per_img, n_tiles, n_tiles_pad;
     UnevenMap um;            // uneven batch (um.n > 0): tiles of all items, one group
     int local_ids;           // uneven batch: rast ids count from each item's first triangle (else global)
@@ -171,12 +175,14 @@ int launch_gather_pos(const float *partial, int n_img, int F, int V, int pos_bat
                       float *grad_pos, hipStream_t st);
 int launch_gather_attr(const float *partial, int groups, int F, int V, int C, const int *inc_off, const int *inc,
                        float *grad_attr, hipStream_t st);
-// workspace of the tile records for nv floats per record, and its layout
-size_t tile_records_bytes(int n_img, int H, int W, int nv);
-void tile_records_layout(TriBwdLaunch &T, void *ws, int n_img, int H, int W, int nv);
+// workspace of the tile records for nv floats per record and n_boxes pixel
+// boxes (n_img * F for a uniform batch, the triangles of an uneven one), and
+// its layout
+size_t tile_records_bytes(int n_img, int H, int W, int nv, long long n_boxes);
+void tile_records_layout(TriBwdLaunch &T, void *ws, int n_img, int H, int W, long long n_boxes);
 // the same for an uneven batch of n_tiles 16 x 16 tiles in all
-size_t tile_records_bytes_tiles(long long n_tiles, int nv);
-void tile_records_layout_tiles(TriBwdLaunch &T, void *ws, long long n_tiles);
+size_t tile_records_bytes_tiles(long long n_tiles, int nv, long long n_boxes);
+void tile_records_layout_tiles(TriBwdLaunch &T, void *ws, long long n_tiles, long long n_boxes);
 
 struct EdgeLaunch {
     const float *grad_out, *rgb, *rast, *frag_xy, *tris_xy;

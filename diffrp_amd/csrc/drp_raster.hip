@@ -831,9 +831,9 @@ int launch_rasterize_bwd(const RasterBwdLaunch &L, hipStream_t st) {
             T.um = L.um;
             T.local_ids = L.local_ids;
             T.n_img = 1;
-            tile_records_layout_tiles(T, L.records, L.um_tiles);
+            tile_records_layout_tiles(T, L.records, L.um_tiles, L.b.F);
         } else {
-            tile_records_layout(T, L.records, L.b.n_img, L.b.H, L.b.W, 12);
+            tile_records_layout(T, L.records, L.b.n_img, L.b.H, L.b.W, (long long)L.b.n_img * L.b.F);
         }
         long long nc = 3ll * L.b.F;
         hipLaunchKernelGGL(k_corner_slots_clear, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, st, nc,

@@ -250,7 +250,18 @@ __global__ void __launch_bounds__(256) k_tile_reduce(TriBwdLaunch L) {
     if (tid == 0) U.seg_start[nseg] = nvalid;
     __syncthreads();
     const long long rbase = (long long)tile * kBTP;
-    if (head) L.rec_t[rbase + seg] = (int)(sk >> 8);
+    if (head) {
+        const int st = (int)(sk >> 8);
+        L.rec_t[rbase + seg] = st;
+        if (L.box_slot) {  // the record's place among the first tiles of st's box
+            const long long bi = (L.um.n ? 0ll : (long long)img * L.F) + st;
+            const ItemBox b = L.boxes[bi];
+            const int tx = lt % tiles_x, ty = lt / tiles_x;
+            const int tx0 = (b.cc & 0xffff) / kBT, tx1 = ((unsigned)b.cc >> 16) / kBT, ty0 = (b.rr & 0xffff) / kBT;
+            const int k = (ty - ty0) * (tx1 - tx0 + 1) + (tx - tx0);
+            if (tx >= tx0 && tx <= tx1 && ty >= ty0 && k < kBoxSlots) L.box_slot[kBoxSlots * bi + k] = (int)(rbase + seg);
+        }
+    }
     if (tid == 0) L.rec_count[tile] = nseg;
     if (QUAD) {
         // one thread per (segment, four components): each key read serves four sums (pixel order)
@@ -280,17 +291,100 @@ __global__ void __launch_bounds__(256) k_tile_reduce(TriBwdLaunch L) {
     }
 }
 
-// Add the records of triangle t over images [v0, v1): for each image where t
-// won a pixel, the tiles of its pixel box, in (image, tile) order.  The
-// searches (records sorted by triangle) run kSearch at a time in lockstep, so
-// their dependent loads overlap instead of queueing one after another.
+// Records of triangle t in one image, in tile order over its pixel box (box
+// index bi).  The first kBoxSlots tiles of the box are looked up through
+// box_slot, which the tile pass filled with the record index of every
+// (box, tile) it wrote; a slot is trusted only when it points into that tile's
+// records and names t there (the workspace is not cleared between calls, so a
+// slot of a tile where t won no pixel holds whatever an earlier call left).
+// Further tiles, and every tile without box_slot, are binary-searched
+// (records sorted by triangle), kSearch at a time in lockstep so that their
+// dependent loads overlap.
 constexpr int kSearch = 4;
 
-template <int NV>
-__device__ __forceinline__ void gather_records(const TriBwdLaunch &L, int t, int v0, int v1, float *acc) {
-    // uniform batch: images v0 .. v1, box of (image, t) at image * F + t, tile
-    // image * tiles_per_img + ty * tiles_x + tx; uneven batch: t's item only,
-    // box at t, tiles from the item's first
+template <int NV, typename Add>
+__device__ __forceinline__ void image_records(const TriBwdLaunch &L, int t, long long bi, int tile_base, int tiles_x,
+                                              Add &&add) {
+    const ItemBox b = L.boxes[bi];
+    const int c0 = b.cc & 0xffff, c1 = (unsigned)b.cc >> 16, r0 = b.rr & 0xffff, r1 = (unsigned)b.rr >> 16;
+    if (!b.hit || c0 > c1 || r0 > r1) return;
+    const int tx0 = c0 / kBT, tx1 = c1 / kBT, ty0 = r0 / kBT, bw = tx1 - tx0 + 1;
+    const int nb = bw * (r1 / kBT - ty0 + 1);
+    int k0 = 0;
+    if (L.box_slot) {
+        const int4 s4 = *reinterpret_cast<const int4 *>(L.box_slot + kBoxSlots * bi);
+        const int sv[kBoxSlots] = {s4.x, s4.y, s4.z, s4.w};
+        int idx[kBoxSlots], cnt[kBoxSlots], rt[kBoxSlots], seg[kBoxSlots];
+        int tx = tx0, ty = ty0;
+#pragma unroll
+        for (int j = 0; j < kBoxSlots; ++j) {
+            idx[j] = -1;
+            cnt[j] = 0;
+            rt[j] = -1;
+            seg[j] = -1;
+            if (j < nb) {
+                const int tile = tile_base + ty * tiles_x + tx;
+                seg[j] = sv[j] - tile * kBTP;
+                if (seg[j] >= 0 && seg[j] < kBTP) {  // inside the tile's records: both loads in bounds
+                    cnt[j] = L.rec_count[tile];
+                    rt[j] = L.rec_t[sv[j]];
+                }
+                if (++tx > tx1) {
+                    tx = tx0;
+                    ++ty;
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kBoxSlots; ++j)
+            if (seg[j] >= 0 && seg[j] < cnt[j] && rt[j] == t) idx[j] = sv[j];
+#pragma unroll
+        for (int j = 0; j < kBoxSlots; ++j)
+            if (idx[j] >= 0) add(L.rec_v + (long long)idx[j] * NV);
+        k0 = min(nb, kBoxSlots);
+    }
+    for (int k = k0; k < nb; k += kSearch) {
+        long long rb[kSearch];
+        int lo[kSearch], hi[kSearch], hit[kSearch];
+#pragma unroll
+        for (int j = 0; j < kSearch; ++j) {
+            rb[j] = 0;
+            lo[j] = 0;
+            hi[j] = -1;
+            hit[j] = -1;
+            if (k + j < nb) {
+                const int q = (k + j) / bw;
+                const int tile = tile_base + (ty0 + q) * tiles_x + tx0 + (k + j - q * bw);
+                rb[j] = (long long)tile * kBTP;
+                hi[j] = L.rec_count[tile] - 1;
+            }
+        }
+        for (bool more = true; more;) {
+            more = false;
+#pragma unroll
+            for (int j = 0; j < kSearch; ++j) {
+                if (hit[j] >= 0 || lo[j] > hi[j]) continue;
+                const int mid = (lo[j] + hi[j]) >> 1;
+                const int tm = L.rec_t[rb[j] + mid];
+                if (tm < t) lo[j] = mid + 1;
+                else if (tm > t) hi[j] = mid - 1;
+                else hit[j] = mid;
+                more = more || (hit[j] < 0 && lo[j] <= hi[j]);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kSearch; ++j)
+            if (hit[j] >= 0) add(L.rec_v + (rb[j] + hit[j]) * NV);
+    }
+}
+
+// Records of triangle t over images [v0, v1) in (image, tile) order; done(img)
+// after each image.  Uniform batch: box of (image, t) at image * F + t, tile
+// image * tiles_per_img + ty * tiles_x + tx.  Uneven batch: t's item only, box
+// at t, tiles from the item's first.
+template <int NV, typename Add, typename Done>
+__device__ __forceinline__ void triangle_records(const TriBwdLaunch &L, int t, int v0, int v1, Add &&add,
+                                                 Done &&done) {
     long long box_stride = L.F;
     int tile_stride = L.tiles_per_img, tiles_x = L.tiles_x, tile_base = 0;
     if (L.um.n) {
@@ -302,180 +396,63 @@ __device__ __forceinline__ void gather_records(const TriBwdLaunch &L, int t, int
         tile_base = L.um.bt16[item];
         tiles_x = (L.um.hw[2 * item + 1] + kBT - 1) / kBT;
     }
-    int img = v0 - 1, tx = 0, tx0 = 0, tx1 = -1, ty = 1, ty1 = 0;  // empty range: the first fill reads a box
-    for (;;) {
-        long long rb[kSearch];
-        int lo[kSearch], hi[kSearch], hit[kSearch];
-        int n = 0;
-        bool done = false;
-#pragma unroll
-        for (int k = 0; k < kSearch; ++k) {
-            rb[k] = 0;
-            lo[k] = 0;
-            hi[k] = -1;
-            hit[k] = -1;
-            if (done) continue;
-            if (ty > ty1) {  // next image where t won a pixel
-                done = true;
-                while (++img < v1) {
-                    ItemBox b = L.boxes[(long long)img * box_stride + t];
-                    int c0 = b.cc & 0xffff, c1 = (unsigned)b.cc >> 16, r0 = b.rr & 0xffff, r1 = (unsigned)b.rr >> 16;
-                    if (!b.hit || c0 > c1 || r0 > r1) continue;
-                    tx0 = tx = c0 / kBT;
-                    tx1 = c1 / kBT;
-                    ty = r0 / kBT;
-                    ty1 = r1 / kBT;
-                    done = false;
-                    break;
-                }
-                if (done) continue;
-            }
-            const int tile = tile_base + img * tile_stride + ty * tiles_x + tx;
-            rb[k] = (long long)tile * kBTP;
-            hi[k] = tile;  // replaced by the record count below
-            ++n;
-            if (++tx > tx1) {
-                tx = tx0;
-                ++ty;
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < kSearch; ++k)
-            if (k < n) hi[k] = L.rec_count[hi[k]] - 1;
-        for (bool more = true; more;) {
-            more = false;
-#pragma unroll
-            for (int k = 0; k < kSearch; ++k) {
-                if (hit[k] >= 0 || lo[k] > hi[k]) continue;
-                const int mid = (lo[k] + hi[k]) >> 1;
-                const int tm = L.rec_t[rb[k] + mid];
-                if (tm < t) lo[k] = mid + 1;
-                else if (tm > t) hi[k] = mid - 1;
-                else hit[k] = mid;
-                more = more || (hit[k] < 0 && lo[k] <= hi[k]);
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < kSearch; ++k) {
-            if (hit[k] < 0) continue;
-            const float *q = L.rec_v + (rb[k] + hit[k]) * NV;
-            if (NV % 4 == 0) {  // records are 16-byte aligned: NV * 4 bytes each from an aligned base
-#pragma unroll
-                for (int j = 0; j < NV; j += 4) {
-                    const float4 x = *reinterpret_cast<const float4 *>(q + j);
-                    acc[j] += x.x; acc[j + 1] += x.y; acc[j + 2] += x.z; acc[j + 3] += x.w;
-                }
-            } else {
-#pragma unroll
-                for (int j = 0; j < NV; ++j) acc[j] += q[j];
-            }
-        }
-        if (n < kSearch) return;
+    for (int img = v0; img < v1; ++img) {
+        image_records<NV>(L, t, img * box_stride + t, tile_base + img * tile_stride, tiles_x, add);
+        done(img);
     }
+}
+
+template <int NV>
+__device__ __forceinline__ void add_record(float *acc, const float *q) {
+    if (NV % 4 == 0) {  // records are 16-byte aligned: NV * 4 bytes each from an aligned base
+#pragma unroll
+        for (int j = 0; j < NV; j += 4) {
+            const float4 x = *reinterpret_cast<const float4 *>(q + j);
+            acc[j] += x.x; acc[j + 1] += x.y; acc[j + 2] += x.z; acc[j + 3] += x.w;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < NV; ++j) acc[j] += q[j];
+    }
+}
+
+template <int NV>
+__device__ __forceinline__ void gather_records(const TriBwdLaunch &L, int t, int v0, int v1, float *acc) {
+    triangle_records<NV>(L, t, v0, v1, [&](const float *q) { add_record<NV>(acc, q); }, [](int) {});
 }
 
 // MODE 2: the attribute sums over the group's images (as MODE 1), and per
-// image the position sums written to that image's corner slots (as MODE 0);
-// records come in (image, tile) order, so an image's position sums are complete
-// when the next image's first record (or the end) arrives.
+// image the position sums, written to that image's corner slots (as MODE 0)
 template <int NA>
 __device__ __forceinline__ void gather_records_fused(const TriBwdLaunch &L, int t, int v0, int v1, float *acc) {
-    constexpr int NV = NA + 12;
     const int *slot = L.corner_slot + 3 * (long long)t;
-    auto flush = [&](int img, const float *q12) {
-        float4 *P = reinterpret_cast<float4 *>(L.partial_pos) + (long long)img * 3 * L.F;
-        if (slot[0] >= 0) P[slot[0]] = make_float4(q12[0], q12[1], q12[2], q12[3]);  // -1: vertex out of range
-        if (slot[1] >= 0) P[slot[1]] = make_float4(q12[4], q12[5], q12[6], q12[7]);
-        if (slot[2] >= 0) P[slot[2]] = make_float4(q12[8], q12[9], q12[10], q12[11]);
-    };
-    const float zero12[12] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     float pos[12];
-    int cur = v0 - 1;  // image whose position sums are in `pos`
-    int img = v0 - 1, tx = 0, tx0 = 0, tx1 = -1, ty = 1, ty1 = 0;
-    for (;;) {
-        long long rb[kSearch];
-        int lo[kSearch], hi[kSearch], hit[kSearch], im[kSearch];
-        int n = 0;
-        bool done = false;
 #pragma unroll
-        for (int k = 0; k < kSearch; ++k) {
-            rb[k] = 0;
-            lo[k] = 0;
-            hi[k] = -1;
-            hit[k] = -1;
-            im[k] = -1;
-            if (done) continue;
-            if (ty > ty1) {
-                done = true;
-                while (++img < v1) {
-                    ItemBox b = L.boxes[(long long)img * L.F + t];
-                    int c0 = b.cc & 0xffff, c1 = (unsigned)b.cc >> 16, r0 = b.rr & 0xffff, r1 = (unsigned)b.rr >> 16;
-                    if (!b.hit || c0 > c1 || r0 > r1) continue;
-                    tx0 = tx = c0 / kBT;
-                    tx1 = c1 / kBT;
-                    ty = r0 / kBT;
-                    ty1 = r1 / kBT;
-                    done = false;
-                    break;
-                }
-                if (done) continue;
-            }
-            const int tile = img * L.tiles_per_img + ty * L.tiles_x + tx;
-            rb[k] = (long long)tile * kBTP;
-            hi[k] = tile;
-            im[k] = img;
-            ++n;
-            if (++tx > tx1) {
-                tx = tx0;
-                ++ty;
-            }
-        }
+    for (int j = 0; j < 12; ++j) pos[j] = 0.f;
+    triangle_records<NA + 12>(
+        L, t, v0, v1,
+        [&](const float *q) {
+            add_record<NA>(acc, q);
+            add_record<12>(pos, q + NA);
+        },
+        [&](int img) {
+            float4 *P = reinterpret_cast<float4 *>(L.partial_pos) + (long long)img * 3 * L.F;
+            if (slot[0] >= 0) P[slot[0]] = make_float4(pos[0], pos[1], pos[2], pos[3]);  // -1: vertex out of range
+            if (slot[1] >= 0) P[slot[1]] = make_float4(pos[4], pos[5], pos[6], pos[7]);
+            if (slot[2] >= 0) P[slot[2]] = make_float4(pos[8], pos[9], pos[10], pos[11]);
 #pragma unroll
-        for (int k = 0; k < kSearch; ++k)
-            if (k < n) hi[k] = L.rec_count[hi[k]] - 1;
-        for (bool more = true; more;) {
-            more = false;
-#pragma unroll
-            for (int k = 0; k < kSearch; ++k) {
-                if (hit[k] >= 0 || lo[k] > hi[k]) continue;
-                const int mid = (lo[k] + hi[k]) >> 1;
-                const int tm = L.rec_t[rb[k] + mid];
-                if (tm < t) lo[k] = mid + 1;
-                else if (tm > t) hi[k] = mid - 1;
-                else hit[k] = mid;
-                more = more || (hit[k] < 0 && lo[k] <= hi[k]);
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < kSearch; ++k) {
-            if (k >= n) continue;
-            if (im[k] != cur) {  // a new image: the previous one's sums are complete
-                if (cur >= v0) flush(cur, pos);
-                for (int e = cur + 1; e < im[k]; ++e) flush(e, zero12);
-                cur = im[k];
-#pragma unroll
-                for (int j = 0; j < 12; ++j) pos[j] = 0.f;
-            }
-            if (hit[k] < 0) continue;
-            const float *q = L.rec_v + (rb[k] + hit[k]) * NV;
-#pragma unroll
-            for (int j = 0; j < NV; j += 4) {
-                const float4 x = *reinterpret_cast<const float4 *>(q + j);
-                if (j < NA) {
-                    acc[j] += x.x; acc[j + 1] += x.y; acc[j + 2] += x.z; acc[j + 3] += x.w;
-                } else {
-                    pos[j - NA] += x.x; pos[j - NA + 1] += x.y; pos[j - NA + 2] += x.z; pos[j - NA + 3] += x.w;
-                }
-            }
-        }
-        if (n < kSearch) break;
-    }
-    if (cur >= v0) flush(cur, pos);
-    for (int e = cur + 1; e < v1; ++e) flush(e, zero12);
+            for (int j = 0; j < 12; ++j) pos[j] = 0.f;
+        });
 }
 
+// Waves per SIMD the register budget is cut for: 6 keeps the latency-bound
+// searches overlapped; records wider than 16 floats need the accumulators of
+// 4 waves (128 VGPRs), beyond 40 floats of 3, to stay out of scratch.
 template <int MODE, int C>
-__global__ void __launch_bounds__(256, 6) k_tile_gather(TriBwdLaunch L) {  // 6 waves per SIMD: latency-bound searches
+constexpr int gather_waves() { return tile_nv<MODE>(C) <= 16 ? 6 : (tile_nv<MODE>(C) <= 40 ? 4 : 3); }
+
+template <int MODE, int C>
+__global__ void __launch_bounds__(256, (gather_waves<MODE, C>())) k_tile_gather(TriBwdLaunch L) {
     constexpr int NV = tile_nv<MODE>(C);
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int grp = blockIdx.y;
@@ -573,40 +550,42 @@ int launch_tri_bwd_attr(const TriBwdLaunch &L, int C, hipStream_t st) {
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
-size_t tile_records_bytes_tiles(long long tiles, int nv) {
-    auto a256 = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    return a256((size_t)tiles * 4) + a256((size_t)tiles * kBTP * 4) + a256((size_t)tiles * kBTP * nv * 4);
+// workspace: record counts [tiles], triangles [tiles, 256], box slots
+// [n_boxes, kBoxSlots], sums [tiles, 256, nv]; the box slots hold int record
+// indices, so they are left out (every tile searched) past 2^31 records
+static bool box_slots_fit(long long n_tiles) { return n_tiles * kBTP < (1ll << 31); }
+
+static size_t a256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+size_t tile_records_bytes_tiles(long long n_tiles, int nv, long long n_boxes) {
+    const size_t slots = box_slots_fit(n_tiles) ? a256((size_t)n_boxes * kBoxSlots * 4) : 0;
+    return a256((size_t)n_tiles * 4) + a256((size_t)n_tiles * kBTP * 4) + slots + a256((size_t)n_tiles * kBTP * nv * 4);
 }
 
-size_t tile_records_bytes(int n_img, int H, int W, int nv) {
-    return tile_records_bytes_tiles((long long)n_img * ((W + kBT - 1) / kBT) * ((H + kBT - 1) / kBT), nv);
+size_t tile_records_bytes(int n_img, int H, int W, int nv, long long n_boxes) {
+    return tile_records_bytes_tiles((long long)n_img * ((W + kBT - 1) / kBT) * ((H + kBT - 1) / kBT), nv, n_boxes);
 }
 
-void tile_records_layout_tiles(TriBwdLaunch &T, void *ws, long long n_tiles) {
+void tile_records_layout_tiles(TriBwdLaunch &T, void *ws, long long n_tiles, long long n_boxes) {
     T.n_tiles = (int)n_tiles;
     T.n_tiles_pad = (T.n_tiles + 7) & ~7;
-    auto a256 = [](size_t x) { return (x + 255) & ~(size_t)255; };
     char *b = (char *)ws;
     T.rec_count = (int *)b;
-    b += a256((size_t)T.n_tiles * 4);
+    b += a256((size_t)n_tiles * 4);
     T.rec_t = (int *)b;
-    b += a256((size_t)T.n_tiles * kBTP * 4);
+    b += a256((size_t)n_tiles * kBTP * 4);
+    T.box_slot = nullptr;
+    if (box_slots_fit(n_tiles)) {
+        T.box_slot = (int *)b;
+        b += a256((size_t)n_boxes * kBoxSlots * 4);
+    }
     T.rec_v = (float *)b;
 }
 
-void tile_records_layout(TriBwdLaunch &T, void *ws, int n_img, int H, int W, int nv) {
+void tile_records_layout(TriBwdLaunch &T, void *ws, int n_img, int H, int W, long long n_boxes) {
     T.tiles_x = (W + kBT - 1) / kBT;
     T.tiles_per_img = T.tiles_x * ((H + kBT - 1) / kBT);
-    T.n_tiles = n_img * T.tiles_per_img;
-    T.n_tiles_pad = (T.n_tiles + 7) & ~7;
-    auto a256 = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    char *b = (char *)ws;
-    T.rec_count = (int *)b;
-    b += a256((size_t)T.n_tiles * 4);
-    T.rec_t = (int *)b;
-    b += a256((size_t)T.n_tiles * kBTP * 4);
-    T.rec_v = (float *)b;
-    (void)nv;
+    tile_records_layout_tiles(T, ws, (long long)n_img * T.tiles_per_img, n_boxes);
 }
 
 }  // namespace drp
