@@ -192,44 +192,63 @@ struct AAImg {
     int V, F, H, W;
 };
 
-__device__ __forceinline__ int rast_id(float4 p, int F) {  // triangle + 1, 0 for none
-    int id = p.w > 0.f ? float_to_triidx(p.w) : 0;
+__device__ __forceinline__ int rast_id_w(float w, int F) {  // triangle + 1 from rast's w, 0 for none
+    int id = w > 0.f ? float_to_triidx(w) : 0;
     return id > F ? 0 : id;
 }
+__device__ __forceinline__ int rast_id(float4 p, int F) { return rast_id_w(p.w, F); }
 
 // Pair (a, b) from the two pixels' rast values and their triangles' silhouette
 // bytes (the caller loads them, so the loads of neighbouring pairs overlap).
+// Both candidates' corner indices, then both candidates' vertex projections,
+// are loaded before either is tested, so the pair waits for two rounds of
+// loads, not up to four.
 __device__ __forceinline__ PairRes eval_pair(const AAImg &I, const int *__restrict__ tri, float4 pa, float4 pb,
                                              int sa, int sb, int ra, int ca, int rb, int cb) {
     PairRes res;
     res.recv = -1;
-    int ida = rast_id(pa, I.F), idb = rast_id(pb, I.F);
+    const int ida = rast_id(pa, I.F), idb = rast_id(pb, I.F);
     if (ida == idb) return res;
     int a_first = 1;
     if (ida == 0) a_first = 0;
     else if (idb != 0 && pb.z < pa.z) a_first = 0;
+    int tc[2], mc[2], vi[2][3];
+    bool ok[2];
+#pragma unroll
+    for (int pass = 0; pass < 2; ++pass) {  // candidate 0 first, then 1
+        const int use_a = pass == 0 ? a_first : !a_first;
+        const int id = use_a ? ida : idb;
+        mc[pass] = use_a ? sa : sb;
+        ok[pass] = id != 0 && (mc[pass] & 7);  // a candidate needs a silhouette-capable edge
+        tc[pass] = ok[pass] ? id - 1 : 0;
+        const int *t3 = tri + 3 * (long long)tc[pass];
+        vi[pass][0] = t3[0];
+        vi[pass][1] = t3[1];
+        vi[pass][2] = t3[2];
+    }
+    // pair geometry in double: a float clip coordinate fixes a vertex to
+    // ~W * 2^-25 pixel, and the crossing needs differences of nearby pixel
+    // positions
+    double2 q[2][3];
+#pragma unroll
     for (int pass = 0; pass < 2; ++pass) {
-        int use_a = pass == 0 ? a_first : !a_first;
-        int id = use_a ? ida : idb;
-        if (id == 0) continue;
-        int t = id - 1;
-        const int m = use_a ? sa : sb;
-        if (!(m & 7)) continue;
-        const int *t3 = tri + 3 * (long long)t;
-        int vi[3] = {t3[0], t3[1], t3[2]};
-        if ((unsigned)vi[0] >= (unsigned)I.V || (unsigned)vi[1] >= (unsigned)I.V || (unsigned)vi[2] >= (unsigned)I.V)
-            continue;
-        // pair geometry in double: a float clip coordinate fixes a vertex to
-        // ~W * 2^-25 pixel, and the crossing needs differences of nearby
-        // pixel positions
+        ok[pass] = ok[pass] && (unsigned)vi[pass][0] < (unsigned)I.V && (unsigned)vi[pass][1] < (unsigned)I.V &&
+                   (unsigned)vi[pass][2] < (unsigned)I.V;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) q[pass][k] = ok[pass] ? I.proj[vi[pass][k]] : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int pass = 0; pass < 2; ++pass) {
+        if (!ok[pass]) continue;
+        const int use_a = pass == 0 ? a_first : !a_first;
+        const int t = tc[pass], m = mc[pass];
         double X[3], Y[3];
         bool front = true;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            double2 q = I.proj[vi[k]];
-            front = front && !isnan(q.x);
-            X[k] = q.x;
-            Y[k] = q.y;
+            front = front && !isnan(q[pass][k].x);
+            X[k] = q[pass][k].x;
+            Y[k] = q[pass][k].y;
         }
         if (!front) continue;
         const int o = (m & 8) ? -1 : 1;
@@ -312,6 +331,12 @@ __device__ __forceinline__ PairRes unpack_decision(int d) {
 }
 
 __device__ __forceinline__ int silh_of(const AAImg &I, int id) { return id ? I.silh[id - 1] : 0; }
+// the same without a branch around the load (callers guarantee F > 0), so that
+// several of them issue together
+__device__ __forceinline__ int silh_at(const AAImg &I, int id) {
+    const int s = I.silh[id > 0 ? id - 1 : 0];
+    return id > 0 ? s : 0;
+}
 
 // weights[pix] = (signed weight of pair (pix, right), of pair (pix, above));
 // decisions[pix] (when kept) the two pairs' decisions for the backward.
@@ -341,13 +366,13 @@ __global__ void __launch_bounds__(256) k_aa_pairs(AALaunch L, float2 *__restrict
     const long long q0 = (long long)r * W + c;
     const bool has_r = in && c + 1 < W, has_u = in && r + 1 < H;
     int cand = 0;  // bit 0: pair (pix, right) may blend; bit 1: pair (pix, above)
-    if (in) {
-        const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-        const float4 p0 = I.rast[q0], pr = has_r ? I.rast[q0 + 1] : z4, pu = has_u ? I.rast[q0 + W] : z4;
-        const int i0 = rast_id(p0, I.F), ir = rast_id(pr, I.F), iu = rast_id(pu, I.F);
-        const int s0 = silh_of(I, i0) & 7;
-        if (has_r && ir != i0 && (s0 || (silh_of(I, ir) & 7))) cand |= 1;
-        if (has_u && iu != i0 && (s0 || (silh_of(I, iu) & 7))) cand |= 2;
+    if (in && I.F > 0) {
+        // the neighbours' ids come from rast's w alone; all loads issue before the first wait
+        const float w0 = I.rast[q0].w, wr = I.rast[has_r ? q0 + 1 : q0].w, wu = I.rast[has_u ? q0 + W : q0].w;
+        const int i0 = rast_id_w(w0, I.F), ir = rast_id_w(wr, I.F), iu = rast_id_w(wu, I.F);
+        const int s0 = silh_at(I, i0), sr = silh_at(I, ir), su = silh_at(I, iu);
+        if (has_r && ir != i0 && ((s0 | sr) & 7)) cand |= 1;
+        if (has_u && iu != i0 && ((s0 | su) & 7)) cand |= 2;
     }
     sw[tid] = make_float2(0.f, 0.f);
     sd[tid] = make_int2(0, 0);
@@ -375,7 +400,8 @@ __global__ void __launch_bounds__(256) k_aa_pairs(AALaunch L, float2 *__restrict
         const int rb = dir ? ra + 1 : ra, cb = dir ? ca : ca + 1;
         const float4 pa = I.rast[(long long)ra * W + ca], pb = I.rast[(long long)rb * W + cb];
         const int ia = rast_id(pa, I.F), ib = rast_id(pb, I.F);
-        const PairRes p = eval_pair(I, L.tri, pa, pb, silh_of(I, ia), silh_of(I, ib), ra, ca, rb, cb);
+        const int sa = silh_at(I, ia), sb = silh_at(I, ib);  // queued pairs exist only when F > 0
+        const PairRes p = eval_pair(I, L.tri, pa, pb, sa, sb, ra, ca, rb, cb);
         if (dir) {
             sw[lp].y = signed_w(p);
             sd[lp].y = pack_decision(p);

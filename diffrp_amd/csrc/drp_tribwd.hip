@@ -3,14 +3,15 @@
 //
 // k_tile_reduce, one workgroup per 16 x 16 pixel tile of one image: every
 // pixel computes its contribution once (coalesced loads, no atomics) and the
-// 256 (triangle, pixel) keys are sorted in LDS (bitonic); each run of equal
-// triangles -- a segment -- is summed in pixel order, per component, and
-// written as a record (triangle, sums).  The records of a tile come out
-// sorted by triangle.
+// 256 (triangle, pixel) keys are sorted in LDS (bitonic); the contributions
+// are stored in sorted order, and each run of equal triangles -- a segment --
+// is summed in pixel order, per component, and written as a record
+// (triangle, sums).  The records of a tile come out sorted by triangle.
 //
 // k_tile_gather, one thread per (image [group], triangle) that won a pixel
 // in the forward: the triangle's pixel box names the tiles it touches; the
-// thread binary-searches each tile's records for its triangle and adds them in
+// tile pass stored the record index of the box's first kBoxSlots tiles per
+// box, further tiles' records are binary-searched, and the thread adds them in
 // tile order.  All sums run in a fixed order, so results are reproducible bit
 // for bit.
 //
@@ -93,6 +94,7 @@ __global__ void __launch_bounds__(256) k_tile_reduce(TriBwdLaunch L) {
     constexpr bool QUAD = NV % 4 == 0;
     __shared__ __align__(16) float contrib[QUAD ? NV * kBTP : NV * kRowStride];
     __shared__ int wsum[4];
+    __shared__ unsigned char rank[kBTP];  // sorted place of each pixel's key
     __shared__ double spx[kBT], spy[kBT];  // NDC centres of the tile's columns and rows (position chain)
     const int tid = threadIdx.x;
     const int tile = xcd_remap(blockIdx.x, L.n_tiles_pad);
@@ -215,15 +217,6 @@ __global__ void __launch_bounds__(256) k_tile_reduce(TriBwdLaunch L) {
             reinterpret_cast<float4 *>(L.grad_rast_out)[pix] = gr;
         }
     }
-    if (QUAD) {
-        float4 *c4 = reinterpret_cast<float4 *>(contrib);
-#pragma unroll
-        for (int j = 0; j < NV; j += 4)
-            c4[(j / 4) * kBTP + tid] = make_float4(val[j], val[j + 1], val[j + 2], val[j + 3]);
-    } else {
-#pragma unroll
-        for (int j = 0; j < NV; ++j) contrib[j * kRowStride + tid] = val[j];
-    }
     // sort the 256 keys (triangle << 8 | pixel; uncovered pixels last, keys unique):
     // each wave sorts its 64 across lanes, then every key's place is its lane
     // plus the keys below it in the other three runs
@@ -237,7 +230,20 @@ __global__ void __launch_bounds__(256) k_tile_reduce(TriBwdLaunch L) {
         for (int w = 0; w < 4; ++w)
             if (w != wave) place += count_below64<K>(U.run + 64 * w, k);
         key[place] = k;
+        rank[(int)(k & 255)] = (unsigned char)place;
         __syncthreads();
+    }
+    // this pixel's contributions at its sorted place, so that every segment's
+    // contributions are contiguous (read by the sums after the barriers below)
+    const int my = rank[tid];
+    if (QUAD) {
+        float4 *c4 = reinterpret_cast<float4 *>(contrib);
+#pragma unroll
+        for (int j = 0; j < NV; j += 4)
+            c4[(j / 4) * kBTP + my] = make_float4(val[j], val[j + 1], val[j + 2], val[j + 3]);
+    } else {
+#pragma unroll
+        for (int j = 0; j < NV; ++j) contrib[j * kRowStride + my] = val[j];
     }
     // segments: runs of one triangle in sorted order
     K sk = key[tid];
@@ -264,16 +270,16 @@ __global__ void __launch_bounds__(256) k_tile_reduce(TriBwdLaunch L) {
     }
     if (tid == 0) L.rec_count[tile] = nseg;
     if (QUAD) {
-        // one thread per (segment, four components): each key read serves four sums (pixel order)
+        // one thread per (segment, four components), summed in pixel order over the contiguous segment
         constexpr int NQ = NV / 4;
         const float4 *c4 = reinterpret_cast<const float4 *>(contrib);
         for (int w = tid; w < nseg * NQ; w += kBTP) {
             const int sg = w / NQ, jq = w - sg * NQ;
             const int p0 = U.seg_start[sg], p1 = U.seg_start[sg + 1];
             const float4 *row = c4 + jq * kBTP;
-            float4 sum = row[(int)(key[p0] & 255)];
+            float4 sum = row[p0];
             for (int p = p0 + 1; p < p1; ++p) {
-                const float4 x = row[(int)(key[p] & 255)];
+                const float4 x = row[p];
                 sum.x += x.x; sum.y += x.y; sum.z += x.z; sum.w += x.w;
             }
             reinterpret_cast<float4 *>(L.rec_v + (rbase + sg) * NV)[jq] = sum;
@@ -284,8 +290,8 @@ __global__ void __launch_bounds__(256) k_tile_reduce(TriBwdLaunch L) {
             int sg = w / NV, j = w - sg * NV;
             int p0 = U.seg_start[sg], p1 = U.seg_start[sg + 1];
             const float *row = contrib + j * kRowStride;
-            float s = row[(int)(key[p0] & 255)];
-            for (int p = p0 + 1; p < p1; ++p) s += row[(int)(key[p] & 255)];
+            float s = row[p0];
+            for (int p = p0 + 1; p < p1; ++p) s += row[p];
             L.rec_v[(rbase + sg) * NV + j] = s;
         }
     }
