@@ -21,6 +21,7 @@ Inputs are resident in HBM before the timed region.
 Prints ONE JSON line.  ``--gpus`` other than 1 exits: this build measures one GPU.
 """
 import argparse
+import hashlib
 import faulthandler
 import json
 import os
@@ -250,15 +251,29 @@ OP_KERNELS = {
 TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'profiles', 'traffic_64view_512.txt')
 
 
+def sources_digest():
+    """SHA-256 over the HIP sources and the C-ABI header (file names and contents):
+    the build a counter summary was measured on."""
+    root = os.path.dirname(os.path.abspath(__file__))
+    csrc = os.path.join(root, 'diffrp_amd', 'csrc')
+    h = hashlib.sha256()
+    for path in sorted(os.path.join(csrc, f) for f in os.listdir(csrc)) + [os.path.join(root, 'include', 'drp.h')]:
+        h.update(os.path.basename(path).encode())
+        with open(path, 'rb') as f:
+            h.update(f.read())
+    return h.hexdigest()
+
+
 def measured_traffic(op, S, subdiv):
     """HBM bytes per call of ``op`` from the committed counter summary (FETCH_SIZE and
-    WRITE_SIZE passes, calibrated on a 256 MiB copy), when it covers this workload."""
+    WRITE_SIZE passes, calibrated on a 256 MiB copy), when it covers this workload and
+    was measured on these sources (else None: a summary of other kernels says nothing)."""
     try:
         lines = open(TRAFFIC_FILE).read().splitlines()
     except OSError:
         return None
     want = f'workload views={S.n} res={S.H}x{S.W} subdiv={subdiv} channels={S.C}'
-    if want not in lines:
+    if want not in lines or f'sources sha256={sources_digest()}' not in lines:
         return None
     for ln in lines:
         f = ln.split()
