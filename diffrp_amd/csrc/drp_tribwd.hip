@@ -308,18 +308,29 @@ __global__ void __launch_bounds__(256) k_tile_reduce(TriBwdLaunch L) {
 // dependent loads overlap.
 constexpr int kSearch = 4;
 
+// one image's box of the triangle and its box slots, loaded together
+struct BoxLoad {
+    ItemBox b;
+    int4 s;
+};
+__device__ __forceinline__ BoxLoad load_box(const TriBwdLaunch &L, long long bi) {
+    BoxLoad x;
+    x.b = L.boxes[bi];
+    x.s = L.box_slot ? *reinterpret_cast<const int4 *>(L.box_slot + kBoxSlots * bi) : make_int4(-1, -1, -1, -1);
+    return x;
+}
+
 template <int NV, typename Add>
-__device__ __forceinline__ void image_records(const TriBwdLaunch &L, int t, long long bi, int tile_base, int tiles_x,
-                                              Add &&add) {
-    const ItemBox b = L.boxes[bi];
+__device__ __forceinline__ void image_records(const TriBwdLaunch &L, int t, const BoxLoad &x, int tile_base,
+                                              int tiles_x, Add &&add) {
+    const ItemBox &b = x.b;
     const int c0 = b.cc & 0xffff, c1 = (unsigned)b.cc >> 16, r0 = b.rr & 0xffff, r1 = (unsigned)b.rr >> 16;
     if (!b.hit || c0 > c1 || r0 > r1) return;
     const int tx0 = c0 / kBT, tx1 = c1 / kBT, ty0 = r0 / kBT, bw = tx1 - tx0 + 1;
     const int nb = bw * (r1 / kBT - ty0 + 1);
     int k0 = 0;
     if (L.box_slot) {
-        const int4 s4 = *reinterpret_cast<const int4 *>(L.box_slot + kBoxSlots * bi);
-        const int sv[kBoxSlots] = {s4.x, s4.y, s4.z, s4.w};
+        const int sv[kBoxSlots] = {x.s.x, x.s.y, x.s.z, x.s.w};
         int idx[kBoxSlots], cnt[kBoxSlots], rt[kBoxSlots], seg[kBoxSlots];
         int tx = tx0, ty = ty0;
 #pragma unroll
@@ -387,7 +398,8 @@ __device__ __forceinline__ void image_records(const TriBwdLaunch &L, int t, long
 // Records of triangle t over images [v0, v1) in (image, tile) order; done(img)
 // after each image.  Uniform batch: box of (image, t) at image * F + t, tile
 // image * tiles_per_img + ty * tiles_x + tx.  Uneven batch: t's item only, box
-// at t, tiles from the item's first.
+// at t, tiles from the item's first.  The next image's box and slots are
+// loaded before this image's records are read.
 template <int NV, typename Add, typename Done>
 __device__ __forceinline__ void triangle_records(const TriBwdLaunch &L, int t, int v0, int v1, Add &&add,
                                                  Done &&done) {
@@ -402,9 +414,14 @@ __device__ __forceinline__ void triangle_records(const TriBwdLaunch &L, int t, i
         tile_base = L.um.bt16[item];
         tiles_x = (L.um.hw[2 * item + 1] + kBT - 1) / kBT;
     }
+    if (v0 >= v1) return;
+    BoxLoad cur = load_box(L, v0 * box_stride + t);
     for (int img = v0; img < v1; ++img) {
-        image_records<NV>(L, t, img * box_stride + t, tile_base + img * tile_stride, tiles_x, add);
+        BoxLoad nxt;
+        if (img + 1 < v1) nxt = load_box(L, (img + 1) * box_stride + t);
+        image_records<NV>(L, t, cur, tile_base + img * tile_stride, tiles_x, add);
         done(img);
+        if (img + 1 < v1) cur = nxt;
     }
 }
 
