@@ -384,14 +384,26 @@ __device__ __forceinline__ void fine_entries(FineLds<TW, TH, PEEL> &S, const Fin
             else if (T.full_scan) t = d.f0 + (idx - T.nbin);
             else t = lists[(long long)d.list_off + idx - T.nbin];
             if (t >= d.f0 && t < d.f1) {  // every entry names a triangle of this image
+                // both setup words and the corner indices depend on t alone: one round of loads,
+                // then the corners' positions
                 const long long si = 2 * ((long long)d.list_off + (t - d.f0));
-                const int4 sa = setup[si];
+                const int4 sa = setup[si], sb = setup[si + 1];
+                const int *t3 = tri + 3 * (long long)t;
+                const int i0 = t3[0], i1 = t3[1], i2 = t3[2];
+                const bool in_range = (unsigned)i0 < (unsigned)d.V && (unsigned)i1 < (unsigned)d.V &&
+                                      (unsigned)i2 < (unsigned)d.V;
+                TriVerts v;
+                if (in_range) {
+                    const float4 *P = reinterpret_cast<const float4 *>(pos + d.pos_off);
+                    const float4 a = P[i0], b = P[i1], c = P[i2];
+                    v.x[0] = a.x; v.y[0] = a.y; v.z[0] = a.z; v.w[0] = a.w;
+                    v.x[1] = b.x; v.y[1] = b.y; v.z[1] = b.z; v.w[1] = b.w;
+                    v.x[2] = c.x; v.y[2] = c.y; v.z[2] = c.z; v.w[2] = c.w;
+                }
                 const int cls = sa.w & 3;
                 if (cls != kCull) {
-                    int4 sb = make_int4(0, 0, 0, 0);
                     int c0, c1, r0, r1;
                     if (cls == kDirect) {
-                        sb = setup[si + 1];
                         const int X[3] = {sa.x, sa.y, sa.z}, Y[3] = {sb.x, sb.y, sb.z};
                         direct_box(X, Y, d.H, d.W, c0, c1, r0, r1);  // not empty: the setup pass kept it
                     } else {
@@ -400,8 +412,7 @@ __device__ __forceinline__ void fine_entries(FineLds<TW, TH, PEEL> &S, const Fin
                     }
                     const int x0 = max(c0 - tc0, 0), x1 = min(c1 - tc0, T.tw_img - 1);
                     const int y0 = max(r0 - tr0, 0), y1 = min(r1 - tr0, T.th_img - 1);
-                    TriVerts v;
-                    if (x0 <= x1 && y0 <= y1 && load_tri(pos, tri, t, d, v)) {
+                    if (x0 <= x1 && y0 <= y1 && in_range) {
                         FineRec &R = S.rec[tid];
                         R.t = t;
                         R.v = v;
