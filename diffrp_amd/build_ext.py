@@ -19,7 +19,7 @@ LIB_PATH = os.path.join(OUT_DIR, 'libdrp_hip.so')
 ARCH = os.environ.get('DRP_OFFLOAD_ARCH', 'gfx950')
 
 SOURCES = ['drp_raster.hip', 'drp_tribwd.hip', 'drp_interp.hip', 'drp_aa.hip', 'drp_mesh.hip', 'drp_edge.hip',
-           'drp_gbuffer.hip', 'drp_texture.hip', 'drp_loss.hip', 'drp_clip.hip', 'drp_capi.hip']
+           'drp_gbuffer.hip', 'drp_texture.hip', 'drp_loss.hip', 'drp_clip.hip', 'drp_fork.hip', 'drp_capi.hip']
 
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 CFLAGS = [

@@ -847,11 +847,18 @@ int launch_antialias_bwd(const AALaunch &L0, hipStream_t st) {
     int tiles_x = (L.W + kAAT - 1) / kAAT, tiles_per_img = tiles_x * ((L.H + kAAT - 1) / kAAT);
     long long nb = L.um.n ? (long long)L.total_t32 : (long long)tiles_per_img * L.n_img, nb_pad = (nb + 7) & ~7ll;
     dim3 block(256);
-    hipLaunchKernelGGL(k_aa_pairs_bwd, dim3((unsigned)nb_pad), block, 0, st, L, L.weights,
+    // With the forward's weights the position pass (latency-bound) and the
+    // colour gather (bandwidth-bound) are independent: they run side by side.
+    // Otherwise the position pass evaluates the pairs and stores the weights
+    // the gather reads.
+    Fork f{};
+    const bool par = L.reuse && fork_begin(st, f);
+    hipLaunchKernelGGL(k_aa_pairs_bwd, dim3((unsigned)nb_pad), block, 0, par ? f.side : st, L, L.weights,
                        L.reuse ? (const int2 *)L.decisions : (const int2 *)nullptr, tiles_x, tiles_per_img,
                        (int)nb_pad);
     dim3 grid = gather_grid(L, total);
     DRP_AA_DISPATCH(k_aa_gather_bwd, L, (const float2 *)L.weights)
+    if (par && fork_end(st, f)) return -2;
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

@@ -883,8 +883,11 @@ int drp_interpolate_rasterize_bwd(void *stream, const float *attr, int n_img, in
     T.views_per_block = vpg;
     tile_records_layout(T, records, n_img, H, W, (long long)n_img * F);
     int rc = launch_tri_bwd_fused(T, C, st);
-    if (!rc) rc = launch_gather_attr(partial_attr, groups, F, V, C, inc_off, inc, grad_attr, st);
+    Fork f{};  // the two vertex gathers are independent
+    const bool par = !rc && fork_begin(st, f);
+    if (!rc) rc = launch_gather_attr(partial_attr, groups, F, V, C, inc_off, inc, grad_attr, par ? f.side : st);
     if (!rc) rc = launch_gather_pos(partial_pos, n_img, F, V, pos_batched, inc_off, grad_pos, st);
+    if (par && fork_end(st, f)) rc = -2;
     if (rc) return fail(rc, "fused backward launch failed: %s", hipGetErrorString(hipGetLastError()));
     return 0;
 }

@@ -167,6 +167,15 @@ struct TriBwdLaunch {
     int local_ids;           // uneven batch: rast ids count from each item's first triangle (else global)
 };
 
+// Fork / join onto the device's auxiliary stream (drp_fork.hip).  fork_begin
+// returns false (f.side null) when there is none: run everything on `main`.
+struct Fork {
+    hipStream_t side;
+    hipEvent_t join;
+};
+bool fork_begin(hipStream_t main, Fork &f);
+int fork_end(hipStream_t main, const Fork &f);
+
 int launch_tri_bwd_pos(const TriBwdLaunch &L, hipStream_t st);
 int launch_tri_bwd_attr(const TriBwdLaunch &L, int C, hipStream_t st);
 int launch_tri_bwd_fused(const TriBwdLaunch &L, int C, hipStream_t st);
