@@ -340,16 +340,23 @@ __device__ __forceinline__ int silh_at(const AAImg &I, int id) {
 
 // weights[pix] = (signed weight of pair (pix, right), of pair (pix, above));
 // decisions[pix] (when kept) the two pairs' decisions for the backward.
-// Blocks of 64 x 4 pixels (grid: columns, rows, images).  Only pairs whose two
-// triangles differ and that have a silhouette-capable edge on either side can
-// blend; the block queues those (in LDS) and evaluates the queue densely, so
-// the double-precision crossing runs on full waves, then every pixel writes
-// its two results.
+// Blocks of 64 x 8 pixels, two rows four apart per thread (grid: columns,
+// rows, images), so that a thread's loads for both pixels are in flight
+// together.  Only pairs whose two triangles differ and that have a
+// silhouette-capable edge on either side can blend; the block queues those
+// (in LDS) and evaluates the queue densely, so the double-precision crossing
+// runs on full waves, then every pixel writes its two results.
+constexpr int kAAPairRows = 8;                 // rows of a pair block (kAAPairRows / 4 pixels per thread)
+constexpr int kAAPairPix = 64 * kAAPairRows;   // pixels of a pair block
+constexpr int kAAPairPer = kAAPairPix / 256;   // pixels per thread
+constexpr int kAAPairDir = 9;                  // queue entry: local pixel | direction << kAAPairDir
+static_assert(kAAPairPix <= (1 << kAAPairDir), "the queue's direction bit must lie above every local pixel index");
+
 __global__ void __launch_bounds__(256) k_aa_pairs(AALaunch L, float2 *__restrict__ weights,
                                                   int2 *__restrict__ decisions) {
-    __shared__ int q[512];      // local pixel | direction << 8 (0: right, 1: above)
-    __shared__ float2 sw[256];
-    __shared__ int2 sd[256];
+    __shared__ int q[2 * kAAPairPix];  // local pixel | direction << kAAPairDir (0: right, 1: above)
+    __shared__ float2 sw[kAAPairPix];
+    __shared__ int2 sd[kAAPairPix];
     __shared__ int q_n;
     const int tid = threadIdx.x, lane = tid & 63;
     int bx = blockIdx.x, by = blockIdx.y, img = blockIdx.z;
@@ -361,25 +368,45 @@ __global__ void __launch_bounds__(256) k_aa_pairs(AALaunch L, float2 *__restrict
     }
     const AAImg I = aa_img(L, img);
     const int H = I.H, W = I.W;
-    const int c = bx * 64 + lane, r = by * 4 + (tid >> 6);
-    const bool in = c < W && r < H;
-    const long long q0 = (long long)r * W + c;
-    const bool has_r = in && c + 1 < W, has_u = in && r + 1 < H;
-    int cand = 0;  // bit 0: pair (pix, right) may blend; bit 1: pair (pix, above)
-    if (in && I.F > 0) {
-        // the neighbours' ids come from rast's w alone; all loads issue before the first wait
-        const float w0 = I.rast[q0].w, wr = I.rast[has_r ? q0 + 1 : q0].w, wu = I.rast[has_u ? q0 + W : q0].w;
-        const int i0 = rast_id_w(w0, I.F), ir = rast_id_w(wr, I.F), iu = rast_id_w(wu, I.F);
-        const int s0 = silh_at(I, i0), sr = silh_at(I, ir), su = silh_at(I, iu);
-        if (has_r && ir != i0 && ((s0 | sr) & 7)) cand |= 1;
-        if (has_u && iu != i0 && ((s0 | su) & 7)) cand |= 2;
+    const int c = bx * 64 + lane;
+    bool in[kAAPairPer], has_r[kAAPairPer], has_u[kAAPairPer];
+    long long q0[kAAPairPer];
+    float w0[kAAPairPer], wr[kAAPairPer], wu[kAAPairPer];
+#pragma unroll
+    for (int j = 0; j < kAAPairPer; ++j) {  // pixel j of this thread: local pixel tid + 256 j
+        const int r = by * kAAPairRows + (tid >> 6) + 4 * j;
+        in[j] = c < W && r < H && I.F > 0;
+        q0[j] = (long long)r * W + c;
+        has_r[j] = in[j] && c + 1 < W;
+        has_u[j] = in[j] && r + 1 < H;
+        w0[j] = wr[j] = wu[j] = 0.f;
+        if (in[j]) {  // the neighbours' ids come from rast's w alone; all loads issue before the first wait
+            w0[j] = I.rast[q0[j]].w;
+            wr[j] = I.rast[has_r[j] ? q0[j] + 1 : q0[j]].w;
+            wu[j] = I.rast[has_u[j] ? q0[j] + W : q0[j]].w;
+        }
     }
-    sw[tid] = make_float2(0.f, 0.f);
-    sd[tid] = make_int2(0, 0);
+    int cand[kAAPairPer];  // bit 0: pair (pix, right) may blend; bit 1: pair (pix, above)
+    int nq = 0;
+#pragma unroll
+    for (int j = 0; j < kAAPairPer; ++j) {
+        cand[j] = 0;
+        if (in[j]) {
+            const int i0 = rast_id_w(w0[j], I.F), ir = rast_id_w(wr[j], I.F), iu = rast_id_w(wu[j], I.F);
+            const int s0 = silh_at(I, i0), sr = silh_at(I, ir), su = silh_at(I, iu);
+            if (has_r[j] && ir != i0 && ((s0 | sr) & 7)) cand[j] |= 1;
+            if (has_u[j] && iu != i0 && ((s0 | su) & 7)) cand[j] |= 2;
+        }
+        nq += (cand[j] & 1) + (cand[j] >> 1);
+    }
+#pragma unroll
+    for (int j = 0; j < kAAPairPer; ++j) {
+        sw[tid + 256 * j] = make_float2(0.f, 0.f);
+        sd[tid + 256 * j] = make_int2(0, 0);
+    }
     if (tid == 0) q_n = 0;
     __syncthreads();
     // wave-aggregated append
-    const int nq = (cand & 1) + (cand >> 1);
     int x = nq;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
@@ -390,13 +417,16 @@ __global__ void __launch_bounds__(256) k_aa_pairs(AALaunch L, float2 *__restrict
     int base = 0;
     if (lane == 63 && wtot > 0) base = atomicAdd(&q_n, wtot);
     base = __shfl(base, 63, 64) + x - nq;
-    if (cand & 1) q[base++] = tid;
-    if (cand & 2) q[base] = tid | (1 << 8);
+#pragma unroll
+    for (int j = 0; j < kAAPairPer; ++j) {
+        if (cand[j] & 1) q[base++] = tid + 256 * j;
+        if (cand[j] & 2) q[base++] = (tid + 256 * j) | (1 << kAAPairDir);
+    }
     __syncthreads();
     const int n = q_n;
     for (int e = tid; e < n; e += 256) {
-        const int m = q[e], lp = m & 255, dir = m >> 8;
-        const int ra = by * 4 + (lp >> 6), ca = bx * 64 + (lp & 63);
+        const int m = q[e], lp = m & ((1 << kAAPairDir) - 1), dir = m >> kAAPairDir;
+        const int ra = by * kAAPairRows + (lp >> 6), ca = bx * 64 + (lp & 63);
         const int rb = dir ? ra + 1 : ra, cb = dir ? ca : ca + 1;
         const float4 pa = I.rast[(long long)ra * W + ca], pb = I.rast[(long long)rb * W + cb];
         const int ia = rast_id(pa, I.F), ib = rast_id(pb, I.F);
@@ -411,10 +441,14 @@ __global__ void __launch_bounds__(256) k_aa_pairs(AALaunch L, float2 *__restrict
         }
     }
     __syncthreads();
-    if (!in) return;
-    const long long g = (L.um.n ? L.um.pix[img] : (long long)img * L.H * L.W) + q0;
-    weights[g] = sw[tid];
-    if (decisions) decisions[g] = sd[tid];
+    const long long pix0 = L.um.n ? L.um.pix[img] : (long long)img * L.H * L.W;
+#pragma unroll
+    for (int j = 0; j < kAAPairPer; ++j) {
+        const int r = by * kAAPairRows + (tid >> 6) + 4 * j;
+        if (c >= W || r >= H) continue;
+        weights[pix0 + q0[j]] = sw[tid + 256 * j];
+        if (decisions) decisions[pix0 + q0[j]] = sd[tid + 256 * j];
+    }
 }
 
 // channel chunk handled by one thread: 4 (one float4) when C is a multiple of 4
@@ -830,7 +864,8 @@ int launch_antialias(const AALaunch &L0, hipStream_t st) {
     const long long total = L.total_pix;
     if (total == 0) return 0;
     launch_orient(L, st);
-    dim3 grid((unsigned)((L.W + 63) / 64), (unsigned)((L.H + 3) / 4), (unsigned)L.n_img), block(256);
+    dim3 grid((unsigned)((L.W + 63) / 64), (unsigned)((L.H + kAAPairRows - 1) / kAAPairRows), (unsigned)L.n_img),
+        block(256);
     if (L.um.n) grid = dim3((unsigned)L.total_blk);
     hipLaunchKernelGGL(k_aa_pairs, grid, block, 0, st, L, L.weights, L.decisions);
     grid = gather_grid(L, total);

@@ -231,7 +231,7 @@ static int auto_bin_cap(int F, long long tiles) {
 static size_t plan_desc_off() { return (sizeof(BatchPlanHeader) + 15) & ~(size_t)15; }
 static size_t plan_t2i_off(int n_img) { return plan_desc_off() + (size_t)n_img * sizeof(ImageDesc); }
 // item map (UnevenMap) after the tile -> image table: pixel prefix (int64), the
-// vertex, triangle, 16 x 16 tile, 64 x 4 block and 32 x 32 tile prefixes
+// vertex, triangle, 16 x 16 tile, 64 x 8 block and 32 x 32 tile prefixes
 // (int32, n + 1 each), then (H, W) per item
 static size_t plan_map_off(int n_img, long long n_tiles) {
     return (plan_t2i_off(n_img) + sizeof(int32_t) * (size_t)n_tiles + 15) & ~(size_t)15;
@@ -311,7 +311,7 @@ int drp_batch_plan(const int32_t *items, int n_img, int tile_w, int tile_h, void
             v += V;
             f += F;
             a += (long long)((W + 15) / 16) * ((H + 15) / 16);
-            k += (long long)((W + 63) / 64) * ((H + 3) / 4);
+            k += (long long)((W + 63) / 64) * ((H + 7) / 8);
             c += (long long)((W + 31) / 32) * ((H + 31) / 32);
         }
         if (a > 0x7fffffffll || k > 0x7fffffffll || c > 0x7fffffffll) return fail(-1, "batch too large");

@@ -15,7 +15,7 @@ struct UnevenMap {
     const int *vert;        // [n + 1] vertex offsets
     const int *tri;         // [n + 1] triangle offsets
     const int *bt16;        // [n + 1] 16 x 16 tiles (deterministic backward)
-    const int *blk;         // [n + 1] 64 x 4 blocks (antialias pairs)
+    const int *blk;         // [n + 1] 64 x 8 blocks (antialias pairs)
     const int *t32;         // [n + 1] 32 x 32 tiles (antialias backward)
     const int *hw;          // [2 n] (H, W) per item
     int n;
