@@ -14,12 +14,14 @@ clip values computed from the world vertices) keeps the rounding of the
 inputs out of the comparison: on this mesh the silhouette slivers make the
 gradient move by ~6e-4 under a relative input change of 6e-8.
 
-Central differences use h = 1e-6 and h/2; a coordinate is compared only where
-the two agree (no kink of the piecewise-smooth antialias within h).  At this
-resolution a step of 1e-4 moves edges by ~0.06 pixel and crosses such a kink
-for most vertices; at 1e-6 float64 rounding is ~1e-8 of the derivatives.  The
-loss is evaluated on the window of pixels the coordinate can affect, which
-equals the full-image difference (test_window_fd_equals_full_image_fd).
+Central differences use h = 1e-6 and h/2; where the two disagree (a kink of
+the piecewise-smooth antialias within h) the step shrinks by 4 twice (down to
+6.25e-8) before the coordinate is given up.  At this resolution a step of
+1e-4 moves edges by ~0.06 pixel and crosses such a kink for most vertices; at
+1e-6 float64 rounding is ~1e-8 of the derivatives.  Every coordinate given up
+is printed, and at most one per set may be (none is, on this scene).  The loss is evaluated on the
+window of pixels the coordinate can affect, which equals the full-image
+difference (test_window_fd_equals_full_image_fd).
 """
 import numpy as np
 import pytest
@@ -89,9 +91,15 @@ class _WindowLoss:
 def _compare(grad, coords, deriv, rtol, floor):
     checked, worst = 0, 0.0
     for i in coords:
-        d1, d2 = deriv(int(i), 1e-6), deriv(int(i), 5e-7)
-        if abs(d1 - d2) > 0.25 * rtol * max(abs(d2), floor):
-            continue  # antialias kink within h
+        d2 = None
+        for h in (1e-6, 2.5e-7, 6.25e-8):
+            d1, dh = deriv(int(i), h), deriv(int(i), h / 2)
+            if abs(d1 - dh) <= 0.25 * rtol * max(abs(dh), floor):
+                d2 = dh
+                break
+        if d2 is None:  # an antialias kink within the smallest step
+            print(f'  coordinate {i} given up: FD h / h/2 disagree down to h = 6.25e-8', flush=True)
+            continue
         err = abs(grad[i] - d2) / max(abs(d2), floor)
         worst = max(worst, err)
         checked += 1
@@ -125,7 +133,7 @@ def test_pipeline_gradients_match_fd_1024(cuda):
     n, worst = _compare(gflat, coords, lambda i, h: loss_for(i // 3).d_vertex(i // 3, VP64[:, i % 3], h),
                         rtol=1e-4, floor=1e-3 * gmax)
     print(f'vertex coordinates checked {n}/48, worst rel err {worst:.2e}', flush=True)
-    assert n >= 36
+    assert n >= 47
     aflat = ga.reshape(-1)
     amax = np.abs(ga).max()
     nz = np.flatnonzero(np.abs(aflat) > 1e-3 * amax)
@@ -133,4 +141,4 @@ def test_pipeline_gradients_match_fd_1024(cuda):
     n, worst = _compare(aflat, coords, lambda i, h: loss_for(i // C).d_attr(i // C, i % C, h),
                         rtol=1e-4, floor=1e-3 * amax)
     print(f'attribute coordinates checked {n}/16, worst rel err {worst:.2e}', flush=True)
-    assert n >= 12
+    assert n >= 15
