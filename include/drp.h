@@ -3,9 +3,15 @@
  *
  * Plain pointers and sizes only.  Every pointer except `ranges` host copies is
  * device memory owned by the caller; work is enqueued on `stream` (a
- * hipStream_t) with no implicit synchronisation and no allocation, so calls
- * can be captured into a HIP graph.  Scratch memory is a caller-provided
- * workspace sized by the matching *_workspace_size query.
+ * hipStream_t) with no implicit synchronisation and no device-memory
+ * allocation, so calls can be captured into a HIP graph.  Scratch memory is a
+ * caller-provided workspace sized by the matching *_workspace_size query.
+ * Independent kernels of one call (the antialias backward's two passes, the
+ * fused backward's two vertex gathers) run beside each other on one auxiliary
+ * stream per device, forked from and joined back to `stream` with events
+ * (a fork / join node pair under capture); the stream is created on the
+ * first call on a device that is not under capture and kept until the
+ * process ends, and without it the kernels run on `stream` in order.
  *
  * Return value: 0 on success, < 0 on error (message: drp_last_error(), which
  * is thread-local).
