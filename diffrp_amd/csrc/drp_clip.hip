@@ -28,7 +28,8 @@ __global__ void __launch_bounds__(256) k_clip_bwd(const float4 *__restrict__ gra
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= V) return;
     float gx = 0.f, gy = 0.f, gz = 0.f;
-    for (int b = 0; b < n; ++b) {
+#pragma unroll 8
+    for (int b = 0; b < n; ++b) {  // unrolled: eight views' loads in flight, summed in view order
         const float4 g = grad_pos[(long long)b * V + v];
         const float *m = VP + 16 * b;
         // column k of VP_b: m[k], m[4 + k], m[8 + k], m[12 + k]

@@ -222,6 +222,55 @@ __global__ void __launch_bounds__(256) k_gather_attr(const float *__restrict__ p
     for (int ch = 0; ch < NC; ++ch) o[ch] += s[ch];
 }
 
+// The same with one thread per (vertex, four channels) for 8 and 16 channels,
+// so that a vertex's corner loads spread over NC / 4 lanes (the same sums in
+// the same order per channel).
+template <int NC>
+__global__ void __launch_bounds__(256) k_gather_attr_q(const float *__restrict__ partial, int n_groups, int F,
+                                                       int V, int per_output_groups, const int *__restrict__ inc_off,
+                                                       const int *__restrict__ inc, float *__restrict__ grad_attr) {
+    constexpr int NQ = NC / 4;
+    const long long tq = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int v = (int)(tq / NQ), q = (int)(tq % NQ);
+    const int out = blockIdx.y;
+    if (v >= V) return;
+    float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+    const int i0 = inc_off[v], i1 = inc_off[v + 1];
+    const int g0 = per_output_groups ? out : 0, g1 = per_output_groups ? out + 1 : n_groups;
+    for (int g = g0; g < g1; ++g) {
+        const float *P = partial + (long long)g * F * 3 * NC + 4 * q;
+        for (int i = i0; i < i1; i += 4) {  // four corners' loads in flight, summed in corner order
+            float4 x[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (i + k < i1) x[k] = *reinterpret_cast<const float4 *>(P + (long long)inc[i + k] * NC);
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (i + k < i1) {
+                    s.x += x[k].x; s.y += x[k].y; s.z += x[k].z; s.w += x[k].w;
+                }
+        }
+    }
+    float *o = grad_attr + ((long long)out * V + v) * NC + 4 * q;
+    o[0] += s.x;
+    o[1] += s.y;
+    o[2] += s.z;
+    o[3] += s.w;
+}
+
+template <int NC>
+static void launch_gather_attr_t(const float *partial, int groups, int F, int V, int outs, int per_output_groups,
+                                 const int *inc_off, const int *inc, float *grad_attr, hipStream_t st) {
+    if constexpr (NC == 8 || NC == 16) {
+        const long long n = (long long)V * (NC / 4);
+        hipLaunchKernelGGL(k_gather_attr_q<NC>, dim3((unsigned)((n + 255) / 256), outs), dim3(256), 0, st, partial,
+                           groups, F, V, per_output_groups, inc_off, inc, grad_attr);
+    } else {
+        hipLaunchKernelGGL(k_gather_attr<NC>, dim3((V + 255) / 256, outs), dim3(256), 0, st, partial, groups, F, V,
+                           per_output_groups, inc_off, inc, grad_attr);
+    }
+}
+
 template <int NC>
 static void launch_interp_bwd_tri_t(const InterpLaunch &L, hipStream_t st) {
     int n_img = L.um.n ? 1 : (int)(L.total_pix / L.pix_per_img);  // an uneven batch sums as one group
@@ -251,19 +300,15 @@ static void launch_interp_bwd_tri_t(const InterpLaunch &L, hipStream_t st) {
     }
     launch_tri_bwd_attr(T, NC, st);
     int outs = L.attr_batched ? n_img : 1;
-    hipLaunchKernelGGL(k_gather_attr<NC>, dim3((L.V + 255) / 256, outs), dim3(256), 0, st, L.partial, groups, L.F,
-                       L.V, L.attr_batched, L.inc_off, L.inc, L.grad_attr);
+    launch_gather_attr_t<NC>(L.partial, groups, L.F, L.V, outs, L.attr_batched, L.inc_off, L.inc, L.grad_attr, st);
 }
 
 int launch_gather_attr(const float *partial, int groups, int F, int V, int C, const int *inc_off, const int *inc,
                        float *grad_attr, hipStream_t st) {
     if (V == 0) return 0;
-    dim3 grid((V + 255) / 256, 1);
     switch (C) {
-        case 4: hipLaunchKernelGGL(k_gather_attr<4>, grid, dim3(256), 0, st, partial, groups, F, V, 0, inc_off, inc,
-                                   grad_attr); break;
-        case 8: hipLaunchKernelGGL(k_gather_attr<8>, grid, dim3(256), 0, st, partial, groups, F, V, 0, inc_off, inc,
-                                   grad_attr); break;
+        case 4: launch_gather_attr_t<4>(partial, groups, F, V, 1, 0, inc_off, inc, grad_attr, st); break;
+        case 8: launch_gather_attr_t<8>(partial, groups, F, V, 1, 0, inc_off, inc, grad_attr, st); break;
         default: return -1;
     }
     return hipGetLastError() == hipSuccess ? 0 : -2;
