@@ -503,7 +503,9 @@ __global__ void __launch_bounds__(256) k_aa_gather(AALaunch L, const float2 *__r
     constexpr int CH = AAChunk<CV>::CH;
     const int C = CV > 0 ? CV : L.C;
     const int nch = C / CH;
-    long long gt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    // XCD k takes the k-th eighth of the blocks, so a pixel's lower neighbour
+    // (read a row earlier in the same eighth) is usually still in that XCD's L2
+    long long gt = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     long long g = gt / nch;
     if (g >= L.total_pix) return;
     int ch = (int)(gt - g * nch) * CH;
@@ -781,7 +783,9 @@ __global__ void __launch_bounds__(256) k_aa_gather_bwd(AALaunch L, const float2 
     constexpr int CH = AAChunk<CV>::CH;
     const int C = CV > 0 ? CV : L.C;
     const int nch = C / CH;
-    long long gt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    // XCD k takes the k-th eighth of the blocks, so a pixel's lower neighbour
+    // (read a row earlier in the same eighth) is usually still in that XCD's L2
+    long long gt = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     long long g = gt / nch;
     if (g >= L.total_pix) return;
     int ch = (int)(gt - g * nch) * CH;
@@ -847,9 +851,11 @@ static void launch_orient(const AALaunch &L, hipStream_t st) {
     }
 }
 
+// the gathers' grid, padded to a multiple of 8 blocks (xcd_remap)
 static dim3 gather_grid(const AALaunch &L, long long total) {
     int nch = (L.cv > 0 && L.cv % 4 == 0) ? L.cv / 4 : (L.cv > 0 ? L.cv : L.C);
-    return dim3((unsigned)((total * nch + 255) / 256));
+    const long long blocks = (total * nch + 255) / 256;
+    return dim3((unsigned)((blocks + 7) & ~7ll));
 }
 
 // a uniform launch's totals in the fields the uneven path fills from its plan
