@@ -138,7 +138,8 @@ __global__ void k_aa_project(const float *__restrict__ pos, int V, int n_proj, i
 
 __global__ void k_aa_orient(const float *__restrict__ pos, int V, const int *__restrict__ tri, int F, int n_proj,
                             signed char *__restrict__ orient) {
-    long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    // XCD k takes the k-th eighth of the (image, triangle) items: an image's vertices stay in one L2
+    long long g = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (g >= (long long)n_proj * F) return;
     int img = (int)(g / F), t = (int)(g % F);
     const int *t3 = tri + 3 * (long long)t;
@@ -158,7 +159,7 @@ __global__ void k_aa_orient(const float *__restrict__ pos, int V, const int *__r
 // the pair search reads the edge test and the orientation from this one byte.
 __global__ void k_aa_silhouette(const int *__restrict__ opp, const signed char *__restrict__ orient, int F,
                                 int n_proj, unsigned char *__restrict__ silh) {
-    long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long g = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;  // as k_aa_orient
     if (g >= (long long)n_proj * F) return;
     int img = (int)(g / F), t = (int)(g % F);
     const signed char *O = orient + (long long)img * F;
@@ -359,12 +360,23 @@ __global__ void __launch_bounds__(256) k_aa_pairs(AALaunch L, float2 *__restrict
     __shared__ int2 sd[kAAPairPix];
     __shared__ int q_n;
     const int tid = threadIdx.x, lane = tid & 63;
-    int bx = blockIdx.x, by = blockIdx.y, img = blockIdx.z;
-    if (L.um.n) {  // uneven batch: one grid row of every item's blocks
-        img = um_find(L.um.blk, L.um.n, (int)blockIdx.x);
-        const int lb = blockIdx.x - L.um.blk[img], nbx = (L.um.hw[2 * img + 1] + 63) / 64;
+    // one row of blocks, padded to a multiple of 8; XCD k takes the k-th eighth
+    // of them, whole images in a uniform batch (their vertices and flags stay
+    // in its L2, and a block's upper neighbour is its successor there)
+    const int lin = xcd_remap(blockIdx.x, gridDim.x);
+    int bx, by, img;
+    if (L.um.n) {  // uneven batch: every item's blocks in turn
+        if (lin >= L.total_blk) return;  // uniform per block
+        img = um_find(L.um.blk, L.um.n, lin);
+        const int lb = lin - L.um.blk[img], nbx = (L.um.hw[2 * img + 1] + 63) / 64;
         bx = lb % nbx;
         by = lb / nbx;
+    } else {
+        const int gx = (L.W + 63) / 64, gy = (L.H + kAAPairRows - 1) / kAAPairRows;
+        if (lin >= gx * gy * L.n_img) return;  // uniform per block
+        bx = lin % gx;
+        by = (lin / gx) % gy;
+        img = lin / (gx * gy);
     }
     const AAImg I = aa_img(L, img);
     const int H = I.H, W = I.W;
@@ -844,10 +856,9 @@ static void launch_orient(const AALaunch &L, hipStream_t st) {
     if (nv) hipLaunchKernelGGL(k_aa_project, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, st, L.pos, L.V, n_proj,
                                L.H, L.W, L.um, L.proj);
     if (nf) {
-        hipLaunchKernelGGL(k_aa_orient, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, st, L.pos, L.V, L.tri, L.F,
-                           n_proj, L.orient);
-        hipLaunchKernelGGL(k_aa_silhouette, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, st, L.opp, L.orient,
-                           L.F, n_proj, L.silh);
+        const dim3 grid((unsigned)((((nf + 255) / 256) + 7) & ~7ll));  // a multiple of 8 blocks (xcd_remap)
+        hipLaunchKernelGGL(k_aa_orient, grid, dim3(256), 0, st, L.pos, L.V, L.tri, L.F, n_proj, L.orient);
+        hipLaunchKernelGGL(k_aa_silhouette, grid, dim3(256), 0, st, L.opp, L.orient, L.F, n_proj, L.silh);
     }
 }
 
@@ -870,9 +881,10 @@ int launch_antialias(const AALaunch &L0, hipStream_t st) {
     const long long total = L.total_pix;
     if (total == 0) return 0;
     launch_orient(L, st);
-    dim3 grid((unsigned)((L.W + 63) / 64), (unsigned)((L.H + kAAPairRows - 1) / kAAPairRows), (unsigned)L.n_img),
-        block(256);
-    if (L.um.n) grid = dim3((unsigned)L.total_blk);
+    const long long nblk = L.um.n ? (long long)L.total_blk
+                                  : (long long)((L.W + 63) / 64) * ((L.H + kAAPairRows - 1) / kAAPairRows) * L.n_img;
+    if (nblk >= (1ll << 31)) return -1;
+    dim3 grid((unsigned)((nblk + 7) & ~7ll)), block(256);
     hipLaunchKernelGGL(k_aa_pairs, grid, block, 0, st, L, L.weights, L.decisions);
     grid = gather_grid(L, total);
     DRP_AA_DISPATCH(k_aa_gather, L, (const float2 *)L.weights)
