@@ -477,8 +477,11 @@ constexpr int gather_waves() { return tile_nv<MODE>(C) <= 16 ? 6 : (tile_nv<MODE
 template <int MODE, int C>
 __global__ void __launch_bounds__(256, (gather_waves<MODE, C>())) k_tile_gather(TriBwdLaunch L) {
     constexpr int NV = tile_nv<MODE>(C);
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int grp = blockIdx.y;
+    // XCD k takes the k-th eighth of the (group, block) grid (its width a
+    // multiple of 8): neighbouring triangles share their tiles' record lines
+    const int lin = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+    const int grp = lin / gridDim.x;
+    const int t = (lin - grp * (int)gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= L.F) return;
     float acc[NV];
 #pragma unroll
@@ -508,6 +511,9 @@ __global__ void __launch_bounds__(256, (gather_waves<MODE, C>())) k_tile_gather(
     }
 }
 
+// blocks per group of the tile gather: a multiple of 8 (xcd_remap)
+static unsigned gather_width(int F) { return (unsigned)((((F + 255) / 256) + 7) & ~7); }
+
 static void tile_grid(const TriBwdLaunch &L, dim3 &grid) {
     grid = dim3((unsigned)L.n_tiles_pad);
 }
@@ -523,7 +529,7 @@ int launch_tri_bwd_pos(const TriBwdLaunch &L, hipStream_t st) {
         if (k32) hipLaunchKernelGGL((k_tile_reduce<0, 1, false, unsigned>), grid, dim3(kBTP), 0, st, L);
         else hipLaunchKernelGGL((k_tile_reduce<0, 1, false, unsigned long long>), grid, dim3(kBTP), 0, st, L);
     }
-    hipLaunchKernelGGL((k_tile_gather<0, 1>), dim3((L.F + 255) / 256, L.n_img), dim3(256), 0, st, L);
+    hipLaunchKernelGGL((k_tile_gather<0, 1>), dim3(gather_width(L.F), L.n_img), dim3(256), 0, st, L);
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
@@ -536,7 +542,7 @@ static void launch_attr_c(const TriBwdLaunch &L, hipStream_t st) {
     else
         hipLaunchKernelGGL((k_tile_reduce<1, C, false, unsigned long long>), grid, dim3(kBTP), 0, st, L);
     int groups = (L.n_img + L.views_per_block - 1) / L.views_per_block;
-    hipLaunchKernelGGL((k_tile_gather<1, C>), dim3((L.F + 255) / 256, groups), dim3(256), 0, st, L);
+    hipLaunchKernelGGL((k_tile_gather<1, C>), dim3(gather_width(L.F), groups), dim3(256), 0, st, L);
 }
 
 template <int C>
@@ -548,7 +554,7 @@ static void launch_fused_c(const TriBwdLaunch &L, hipStream_t st) {
     else
         hipLaunchKernelGGL((k_tile_reduce<2, C, false, unsigned long long>), grid, dim3(kBTP), 0, st, L);
     int groups = (L.n_img + L.views_per_block - 1) / L.views_per_block;
-    hipLaunchKernelGGL((k_tile_gather<2, C>), dim3((L.F + 255) / 256, groups), dim3(256), 0, st, L);
+    hipLaunchKernelGGL((k_tile_gather<2, C>), dim3(gather_width(L.F), groups), dim3(256), 0, st, L);
 }
 
 int launch_tri_bwd_fused(const TriBwdLaunch &L, int C, hipStream_t st) {
