@@ -90,7 +90,8 @@ class Scene:
         self.tri = tri.to(dev)
         self.VP = torch.stack(VP).to(dev)
         self.target = target.to(dev)
-        self.ctx = nvdr.RasterizeCudaContext(dev)
+        # the antialias per-vertex / per-triangle state is prepared beside the rasterizer
+        self.ctx = nvdr.RasterizeCudaContext(dev, prepare_antialias=True)
         self.topo = nvdr.antialias_construct_topology_hash(self.tri)
 
     def step_batched(self):

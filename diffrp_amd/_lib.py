@@ -86,6 +86,10 @@ _SIGS = {
     'drp_antialias_workspace_size': (c_size, [c_int, c_int, c_int, c_int, c_int, c_int]),
     'drp_antialias_fwd': (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_int, c_int,
                                   c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_size]),
+    'drp_antialias_prepare': (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_int, c_int, c_void_p, c_int,
+                                      c_void_p, c_void_p, c_size]),
+    'drp_antialias_fwd_prepared': (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_int,
+                                           c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_size]),
     'drp_antialias_bwd': (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_int, c_int,
                                   c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_int, c_void_p,
                                   c_size]),

@@ -880,7 +880,7 @@ int launch_antialias(const AALaunch &L0, hipStream_t st) {
     const AALaunch L = with_totals(L0);
     const long long total = L.total_pix;
     if (total == 0) return 0;
-    launch_orient(L, st);
+    if (!L.prepared) launch_orient(L, st);
     const long long nblk = L.um.n ? (long long)L.total_blk
                                   : (long long)((L.W + 63) / 64) * ((L.H + kAAPairRows - 1) / kAAPairRows) * L.n_img;
     if (nblk >= (1ll << 31)) return -1;
@@ -888,6 +888,15 @@ int launch_antialias(const AALaunch &L0, hipStream_t st) {
     hipLaunchKernelGGL(k_aa_pairs, grid, block, 0, st, L, L.weights, L.decisions);
     grid = gather_grid(L, total);
     DRP_AA_DISPATCH(k_aa_gather, L, (const float2 *)L.weights)
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+// the per-vertex pixel positions and per-triangle orientation and silhouette
+// flags alone, for a forward that follows with L.prepared set
+int launch_antialias_prepare(const AALaunch &L0, hipStream_t st) {
+    const AALaunch L = with_totals(L0);
+    if (L.total_pix == 0) return 0;
+    launch_orient(L, st);
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

@@ -1090,9 +1090,9 @@ static int aa_common(AALaunch &L, void *stream, const float *color, int n_img, i
     return 0;
 }
 
-int drp_antialias_fwd(void *stream, const float *color, int n_img, int H, int W, int C, const float *rast,
-                      const float *pos, int V, int pos_batched, const int32_t *tri, int F, const int32_t *opp,
-                      float *out, void *workspace, size_t workspace_bytes) {
+static int aa_fwd(void *stream, const float *color, int n_img, int H, int W, int C, const float *rast,
+                  const float *pos, int V, int pos_batched, const int32_t *tri, int F, const int32_t *opp, float *out,
+                  void *workspace, size_t workspace_bytes, int prepared) {
     if (n_img == 0) return 0;
     AALaunch L{};
     if (int e = aa_common(L, stream, color, n_img, H, W, C, rast, pos, V, pos_batched, tri, F, opp, workspace,
@@ -1101,9 +1101,56 @@ int drp_antialias_fwd(void *stream, const float *color, int n_img, int H, int W,
     if (!out) return fail(-1, "null output");
     if (C % 4 == 0 && !aligned16(out)) return fail(-1, "out must be 16-byte aligned when C %% 4 == 0");
     L.out = out;
+    L.prepared = prepared;
     int rc = launch_antialias(L, (hipStream_t)stream);
     if (rc) return fail(rc, "antialias launch failed: %s", hipGetErrorString(hipGetLastError()));
     return 0;
+}
+
+int drp_antialias_fwd(void *stream, const float *color, int n_img, int H, int W, int C, const float *rast,
+                      const float *pos, int V, int pos_batched, const int32_t *tri, int F, const int32_t *opp,
+                      float *out, void *workspace, size_t workspace_bytes) {
+    return aa_fwd(stream, color, n_img, H, W, C, rast, pos, V, pos_batched, tri, F, opp, out, workspace,
+                  workspace_bytes, 0);
+}
+
+int drp_antialias_prepare(void *stream, int n_img, int H, int W, const float *pos, int V, int pos_batched,
+                          const int32_t *tri, int F, const int32_t *opp, void *workspace, size_t workspace_bytes) {
+    if (int e = check_dims(n_img, H, W)) return e;
+    if (n_img == 0) return 0;
+    if (n_img > 65535) return fail(-1, "antialias takes at most 65535 images per call (got %d)", n_img);
+    if ((V > 0 && !pos) || (F > 0 && (!tri || !opp))) return fail(-1, "null pointer argument");
+    if (!aligned16(pos)) return fail(-1, "pos must be 16-byte aligned");
+    if (pos_batched && (long long)n_img * V >= (1ll << 31))
+        return fail(-1, "antialias with per-image positions takes fewer than 2^31 vertices in all");
+    const size_t need = drp_antialias_workspace_size(n_img, H, W, V, F, pos_batched);
+    if (!workspace || workspace_bytes < need)
+        return fail(-1, "workspace too small: need %zu bytes, got %zu", need, workspace_bytes);
+    const size_t n_proj = pos_batched ? (size_t)n_img : 1;
+    char *b = (char *)workspace;
+    AALaunch L{};
+    L.pos = pos;
+    L.tri = tri;
+    L.opp = opp;
+    L.n_img = n_img;
+    L.H = H;
+    L.W = W;
+    L.V = V;
+    L.F = F;
+    L.pos_batched = pos_batched;
+    L.proj = (double2 *)b;
+    L.orient = (signed char *)(b + align256(n_proj * (size_t)V * 16));
+    L.silh = (unsigned char *)(b + align256(n_proj * (size_t)V * 16) + align256(n_proj * (size_t)F));
+    int rc = launch_antialias_prepare(L, (hipStream_t)stream);
+    if (rc) return fail(rc, "antialias prepare launch failed: %s", hipGetErrorString(hipGetLastError()));
+    return 0;
+}
+
+int drp_antialias_fwd_prepared(void *stream, const float *color, int n_img, int H, int W, int C, const float *rast,
+                               const float *pos, int V, int pos_batched, const int32_t *tri, int F,
+                               const int32_t *opp, float *out, void *workspace, size_t workspace_bytes) {
+    return aa_fwd(stream, color, n_img, H, W, C, rast, pos, V, pos_batched, tri, F, opp, out, workspace,
+                  workspace_bytes, 1);
 }
 
 int drp_antialias_bwd(void *stream, const float *color, int n_img, int H, int W, int C, const float *rast,

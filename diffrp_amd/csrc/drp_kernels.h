@@ -132,6 +132,7 @@ struct AALaunch {
     float2 *weights;
     int2 *decisions;             // [n, H, W] pair decisions for the backward (nullptr: not kept)
     int reuse;                   // backward: workspace holds the forward's state
+    int prepared;                // forward: drp_antialias_prepare filled the per-vertex / per-triangle state
     float *out;
     const float *grad_out;
     float *grad_color, *grad_pos;
@@ -207,6 +208,7 @@ int launch_interpolate_bwd(const InterpLaunch &L, hipStream_t st);
 int launch_topology(const TopoLaunch &L, hipStream_t st);
 int launch_antialias(const AALaunch &L, hipStream_t st);
 int launch_antialias_bwd(const AALaunch &L, hipStream_t st);
+int launch_antialias_prepare(const AALaunch &L, hipStream_t st);
 size_t incidence_ws_bytes(int V, int F);
 size_t stencil_partition_ws_bytes(long long npx, int n_owner);
 int launch_sample2d(const float *tex, int H, int W, int C, const float *uv, long long n, int wrap, int mode,

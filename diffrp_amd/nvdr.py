@@ -21,6 +21,7 @@ kernels through the C-ABI of ``libdrp_hip.so``; gradients flow through
 ``install_as_nvdiffrast()`` registers this module as ``nvdiffrast.torch`` so
 diffrp itself can import it unchanged.
 """
+import ctypes
 import sys
 import threading
 import types
@@ -54,12 +55,23 @@ class RasterizeCudaContext:
     context may be shared by threads as diffrp's per-thread cache does.
     ``tile`` / ``bin_capacity`` = None lets the heuristic (or an autotuned
     table installed with ``set_tuning``) choose per call shape.
+
+    ``prepare_antialias=True`` (an option of this implementation): every
+    single-layer ``rasterize`` of this context also prepares what a following
+    ``antialias`` of the same ``pos`` and ``tri`` computes per vertex and per
+    triangle (pixel positions, screen orientations, silhouette flags), on an
+    auxiliary stream beside the rasterizer and joined before ``rasterize``
+    returns; ``antialias`` then starts at its pair search.  The results are
+    the same; the preparation is wasted work when no such ``antialias`` follows.
     """
 
-    def __init__(self, device: Optional[Union[int, str, torch.device]] = None, tile=None, bin_capacity=None):
+    def __init__(self, device: Optional[Union[int, str, torch.device]] = None, tile=None, bin_capacity=None,
+                 prepare_antialias=False):
         if device is None:
             device = torch.cuda.current_device()
         self.device = torch.device('cuda', device) if isinstance(device, int) else torch.device(device)
+        self.prepare_antialias = bool(prepare_antialias)
+        self._side = None  # auxiliary stream of the antialias preparation (made on first use)
         if tile is not None and tuple(tile) not in _TILES:
             raise ValueError(f'tile must be one of {sorted(_TILES)}')
         self.tile = None if tile is None else tuple(tile)
@@ -347,13 +359,50 @@ def _rasterize_impl(glctx, pos, tri, resolution, ranges, grad_db, prev):
     elif ranges is not None:
         raise ValueError('ranges are only valid with 2-D pos (range mode)')
     holder = []
+    prep = _aa_prepare_begin(glctx, pos, tri, H, W) if glctx.prepare_antialias and rng is None and prev is None \
+        else None
     rast, rast_db = _RasterizeFunc.apply(glctx, pos, tri, H, W, rng, prev, bool(grad_db), holder)
+    if prep is not None:
+        _aa_prepare_end(prep, rast)
     if holder and holder[0] is not None:
         # pixel boxes of this rasterisation, for the triangle-centric interpolate
         # backward, and the positions it rasterised (when differentiable: the
         # interpolate backward can then run the rasterize backward in the same pass)
         rast._drp_boxes = (holder[0], tri.shape[0], rast._version, pos if rast.requires_grad else None)
     return rast, rast_db
+
+
+def _aa_prepare_begin(glctx, pos, tri, H, W):
+    """Launch the antialias preparation of (pos, tri, H, W) on the context's
+    auxiliary stream, forked from the current one (the rasterizer then runs
+    beside it)."""
+    n = pos.shape[0] if pos.dim() == 3 else 1
+    batched = int(pos.dim() == 3)
+    V, F = pos.shape[-2], tri.shape[0]
+    if F == 0 or V == 0:
+        return None
+    opp = _cached_topology(tri).opp
+    L = _lib.lib()
+    ws_bytes = L.drp_antialias_workspace_size(n, H, W, V, F, batched)
+    ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=pos.device)
+    cur = torch.cuda.current_stream(pos.device)
+    if glctx._side is None:
+        glctx._side = torch.cuda.Stream(pos.device)  # default priority
+    side = glctx._side
+    side.wait_stream(cur)
+    check(L.drp_antialias_prepare(ctypes.c_void_p(side.cuda_stream), n, H, W, ptr(pos), V, batched, ptr(tri), F,
+                                  ptr(opp), ptr(ws), ws_bytes), 'antialias prepare')
+    for t in (ws, pos, tri, opp):
+        t.record_stream(side)  # their memory is not reused before the side stream is done with it
+    return ws, side, pos, tri, opp, (n, H, W, V, F, batched)
+
+
+def _aa_prepare_end(prep, rast):
+    """Join the preparation back into the current stream and hand its workspace
+    to the next antialias of this rast (``_AntialiasFunc``)."""
+    ws, side, pos, tri, opp, shape = prep
+    torch.cuda.current_stream(pos.device).wait_stream(side)
+    rast._drp_aa_prep = (ws, pos, pos._version, tri, opp, shape, rast._version)
 
 
 def rasterize(glctx, pos, tri, resolution, ranges=None, grad_db=True):
@@ -556,7 +605,10 @@ def _build_topology(tri):
 
 def antialias_construct_topology_hash(tri):
     require_device(tri, names=('tri',))
-    return _build_topology(_as_tri(tri))
+    tri = _as_tri(tri)
+    th = _build_topology(tri)
+    _remember_topology(tri, th)  # antialias of this tri without a hash, and its preparation, reuse it
+    return th
 
 
 # per-tensor reuse of the topology table when antialias is called without one
@@ -565,13 +617,17 @@ def antialias_construct_topology_hash(tri):
 _topo_cache = {}
 
 
-def _cached_topology(tri: torch.Tensor):
+def _remember_topology(tri: torch.Tensor, th):
     key = id(tri)
-    ent = _topo_cache.get(key)
+    _topo_cache[key] = (weakref.ref(tri, lambda _r, k=key: _topo_cache.pop(k, None)), tri._version, th)
+
+
+def _cached_topology(tri: torch.Tensor):
+    ent = _topo_cache.get(id(tri))
     if ent is not None and ent[0]() is tri and ent[1] == tri._version:
         return ent[2]
     th = _build_topology(tri)
-    _topo_cache[key] = (weakref.ref(tri, lambda _r, k=key: _topo_cache.pop(k, None)), tri._version, th)
+    _remember_topology(tri, th)
     return th
 
 
@@ -584,9 +640,17 @@ class _AntialiasFunc(torch.autograd.Function):
         out = torch.empty_like(color)
         L = _lib.lib()
         ws_bytes = L.drp_antialias_workspace_size(n, H, W, V, F, batched)
-        ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=color.device)
-        rc = L.drp_antialias_fwd(stream_of(color), ptr(color), n, H, W, C, ptr(rast), ptr(pos), V, batched, ptr(tri),
-                                 F, ptr(opp), ptr(out), ptr(ws), ws_bytes)
+        prep = getattr(rast, '_drp_aa_prep', None)
+        if (prep is not None and prep[1] is pos and prep[2] == pos._version and prep[3] is tri and prep[4] is opp
+                and prep[5] == (n, H, W, V, F, batched) and prep[6] == rast._version):
+            # the rasterize call prepared this workspace (same positions, triangles and size)
+            ws, fwd = prep[0], L.drp_antialias_fwd_prepared
+            rast._drp_aa_prep = None  # used once: the workspace now keeps this call's pair decisions
+        else:
+            ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=color.device)
+            fwd = L.drp_antialias_fwd
+        rc = fwd(stream_of(color), ptr(color), n, H, W, C, ptr(rast), ptr(pos), V, batched, ptr(tri), F, ptr(opp),
+                 ptr(out), ptr(ws), ws_bytes)
         check(rc, 'antialias')
         fctx.save_for_backward(color, rast, pos, tri, opp)
         fctx.boost = boost

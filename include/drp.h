@@ -272,6 +272,20 @@ int drp_antialias_fwd(void *stream, const float *color, int n_img, int H, int W,
                       const float *pos, int V, int pos_batched, const int32_t *tri, int F, const int32_t *opp,
                       float *out, void *workspace, size_t workspace_bytes);
 
+/* The per-vertex and per-triangle part of drp_antialias_fwd (pixel positions
+ * of the vertices, screen orientation and silhouette flags of the triangles;
+ * no counterpart in nvdiffrast, whose antialias computes it in the call), into
+ * an antialias workspace, for a later drp_antialias_fwd_prepared on the same
+ * pos, tri, opp and image size: it depends on those alone, so a caller can run
+ * it beside the rasterizer (nvdr.rasterize does, on an auxiliary stream, for
+ * contexts made with prepare_antialias=True). */
+int drp_antialias_prepare(void *stream, int n_img, int H, int W, const float *pos, int V, int pos_batched,
+                          const int32_t *tri, int F, const int32_t *opp, void *workspace, size_t workspace_bytes);
+/* drp_antialias_fwd with the workspace already prepared by drp_antialias_prepare. */
+int drp_antialias_fwd_prepared(void *stream, const float *color, int n_img, int H, int W, int C, const float *rast,
+                               const float *pos, int V, int pos_batched, const int32_t *tri, int F,
+                               const int32_t *opp, float *out, void *workspace, size_t workspace_bytes);
+
 /* Backward of antialias: grad_color written; grad_pos accumulated (caller
  * zero-fills; may be null to skip), scaled by pos_gradient_boost.
  * reuse_forward != 0: the workspace still holds what drp_antialias_fwd left
