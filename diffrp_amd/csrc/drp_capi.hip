@@ -835,10 +835,9 @@ int drp_interpolate_rasterize_bwd(void *stream, const float *attr, int n_img, in
     if (n_img == 0) return 0;
     if (!(C == 4 || C == 8)) return fail(-1, "the fused backward takes C in {4, 8} (got %d)", C);
     if (F <= 0 || V <= 0) return fail(-1, "the fused backward needs triangles and vertices");
-    if (!attr || !rast || !tri || !pos || !grad_out || !grad_attr || !grad_rast || !grad_pos || !boxes || !inc_off ||
-        !inc)
+    if (!attr || !rast || !tri || !pos || !grad_out || !grad_attr || !grad_pos || !boxes || !inc_off || !inc)
         return fail(-1, "null pointer argument");
-    if (!aligned16(rast) || !aligned16(grad_rast) || !aligned16(pos) || !aligned16(grad_pos) ||
+    if (!aligned16(rast) || (grad_rast && !aligned16(grad_rast)) || !aligned16(pos) || !aligned16(grad_pos) ||
         !aligned16(grad_out) || !aligned16(attr))
         return fail(-1, "rast, grad_rast, pos, grad_pos, grad_out and attr must be 16-byte aligned");
     const size_t need = drp_interpolate_rasterize_bwd_workspace_size(n_img, H, W, F, C, views_per_group);
@@ -1049,6 +1048,23 @@ size_t drp_antialias_workspace_size(int n_img, int H, int W, int V, int F, int p
     return align256(n_proj * (size_t)V * 16) + 2 * align256(n_proj * (size_t)F) + 2 * align256(npx * 8);
 }
 
+// the antialias workspace: per-vertex pixel positions, per-triangle
+// orientation and silhouette flags, per-pixel pair weights and decisions
+static void aa_layout(AALaunch &L, void *workspace, int n_img, int H, int W, int V, int F, int pos_batched) {
+    const size_t n_proj = pos_batched ? (size_t)n_img : 1;
+    const size_t npx = (size_t)n_img * H * W;
+    const size_t o_orient = align256(n_proj * (size_t)V * 16);
+    const size_t o_silh = o_orient + align256(n_proj * (size_t)F), o_w = o_silh + align256(n_proj * (size_t)F);
+    const size_t o_dec = o_w + align256(npx * 8);
+    char *b = (char *)workspace;
+    L.proj = (double2 *)b;
+    L.orient = (signed char *)(b + o_orient);
+    L.silh = (unsigned char *)(b + o_silh);
+    L.weights = (float2 *)(b + o_w);
+    // pair decisions pack the triangle into 27 bits; larger meshes recompute them in the backward
+    L.decisions = F < kMaxDecisionTris ? (int2 *)(b + o_dec) : nullptr;
+}
+
 static int aa_common(AALaunch &L, void *stream, const float *color, int n_img, int H, int W, int C,
                      const float *rast, const float *pos, int V, int pos_batched, const int32_t *tri, int F,
                      const int32_t *opp, void *workspace, size_t workspace_bytes) {
@@ -1061,8 +1077,6 @@ static int aa_common(AALaunch &L, void *stream, const float *color, int n_img, i
     size_t need = drp_antialias_workspace_size(n_img, H, W, V, F, pos_batched);
     if (!workspace || workspace_bytes < need)
         return fail(-1, "workspace too small: need %zu bytes, got %zu", need, workspace_bytes);
-    size_t n_proj = pos_batched ? (size_t)n_img : 1;
-    char *b = (char *)workspace;
     L.color = color;
     L.rast = rast;
     L.pos = pos;
@@ -1076,16 +1090,7 @@ static int aa_common(AALaunch &L, void *stream, const float *color, int n_img, i
     L.F = F;
     L.pos_batched = pos_batched;
     L.cv = (C == 1 || C == 3 || C == 4 || C == 8 || C == 16) ? C : 0;
-    size_t npx = (size_t)n_img * H * W;
-    size_t o_proj = 0, o_orient = align256(n_proj * (size_t)V * 16);
-    size_t o_silh = o_orient + align256(n_proj * (size_t)F), o_w = o_silh + align256(n_proj * (size_t)F);
-    size_t o_dec = o_w + align256(npx * 8);
-    L.proj = (double2 *)(b + o_proj);
-    L.orient = (signed char *)(b + o_orient);
-    L.silh = (unsigned char *)(b + o_silh);
-    L.weights = (float2 *)(b + o_w);
-    // pair decisions pack the triangle into 27 bits; larger meshes recompute them in the backward
-    L.decisions = F < kMaxDecisionTris ? (int2 *)(b + o_dec) : nullptr;
+    aa_layout(L, workspace, n_img, H, W, V, F, pos_batched);
     (void)stream;
     return 0;
 }
@@ -1126,8 +1131,6 @@ int drp_antialias_prepare(void *stream, int n_img, int H, int W, const float *po
     const size_t need = drp_antialias_workspace_size(n_img, H, W, V, F, pos_batched);
     if (!workspace || workspace_bytes < need)
         return fail(-1, "workspace too small: need %zu bytes, got %zu", need, workspace_bytes);
-    const size_t n_proj = pos_batched ? (size_t)n_img : 1;
-    char *b = (char *)workspace;
     AALaunch L{};
     L.pos = pos;
     L.tri = tri;
@@ -1138,9 +1141,7 @@ int drp_antialias_prepare(void *stream, int n_img, int H, int W, const float *po
     L.V = V;
     L.F = F;
     L.pos_batched = pos_batched;
-    L.proj = (double2 *)b;
-    L.orient = (signed char *)(b + align256(n_proj * (size_t)V * 16));
-    L.silh = (unsigned char *)(b + align256(n_proj * (size_t)V * 16) + align256(n_proj * (size_t)F));
+    aa_layout(L, workspace, n_img, H, W, V, F, pos_batched);
     int rc = launch_antialias_prepare(L, (hipStream_t)stream);
     if (rc) return fail(rc, "antialias prepare launch failed: %s", hipGetErrorString(hipGetLastError()));
     return 0;

@@ -152,7 +152,7 @@ struct TriBwdLaunch {
     const float *rast, *grad_rast, *grad_db, *grad_out;
     const float *attr;       // attribute mode: attributes (for grad_rast)
     int V, attr_batched;
-    float *grad_rast_out;    // attribute mode: grad_rast written by the tile pass
+    float *grad_rast_out;    // attribute mode: grad_rast written by the tile pass (fused mode: null skips it)
     const ItemBox *boxes;
     float *partial;
     float *partial_pos;      // fused mode: the position partial [n_img, 3F] float4 (partial: attributes)

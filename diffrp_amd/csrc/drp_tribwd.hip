@@ -214,7 +214,7 @@ __global__ void __launch_bounds__(256) k_tile_reduce(TriBwdLaunch L) {
                     t = -1;
                 }
             }
-            reinterpret_cast<float4 *>(L.grad_rast_out)[pix] = gr;
+            if (MODE == 1 || L.grad_rast_out) reinterpret_cast<float4 *>(L.grad_rast_out)[pix] = gr;
         }
     }
     // sort the 256 keys (triangle << 8 | pixel; uncovered pixels last, keys unique):

@@ -317,11 +317,6 @@ class _RasterizeFunc(torch.autograd.Function):
         gdb_used = fctx.want_db and ddb is not None and ddb.numel() > 0
         if dy is None and not gdb_used:
             return None, None, None, None, None, None, None, None, None
-        # the interpolate backward that produced dy may have computed this pass
-        # already (dy reached here unchanged: interpolate was its only source)
-        pre = getattr(dy, '_drp_pos_grad', None) if dy is not None else None
-        if pre is not None and not gdb_used and pre[1] == rast.data_ptr() and pre[2] == dy._version:
-            return None, pre[0], None, None, None, None, None, None, None
         gpos = torch.zeros_like(pos)
         dy = _as_f32(dy, 'grad_rast') if dy is not None else torch.zeros_like(rast)
         gdb = _as_f32(ddb, 'grad_db') if gdb_used else None
@@ -366,9 +361,11 @@ def _rasterize_impl(glctx, pos, tri, resolution, ranges, grad_db, prev):
         _aa_prepare_end(prep, rast)
     if holder and holder[0] is not None:
         # pixel boxes of this rasterisation, for the triangle-centric interpolate
-        # backward, and the positions it rasterised (when differentiable: the
-        # interpolate backward can then run the rasterize backward in the same pass)
-        rast._drp_boxes = (holder[0], tri.shape[0], rast._version, pos if rast.requires_grad else None)
+        # backward, and the positions and triangles it rasterised (when
+        # differentiable: an interpolate of the same triangles can then run the
+        # rasterize backward in the same pass)
+        rast._drp_boxes = (holder[0], (tri.data_ptr(), tuple(tri.shape), tri._version), rast._version,
+                           pos if rast.requires_grad else None)
     return rast, rast_db
 
 
@@ -397,12 +394,22 @@ def _aa_prepare_begin(glctx, pos, tri, H, W):
     return ws, side, pos, tri, opp, (n, H, W, V, F, batched)
 
 
+class _AAPrep:
+    """An antialias workspace prepared for (pos, tri, opp, shape) and the rast
+    it belongs to."""
+    __slots__ = ('ws', 'pos', 'pos_version', 'tri', 'opp', 'shape', 'rast_version')
+
+    def __init__(self, ws, pos, tri, opp, shape, rast_version):
+        self.ws, self.pos, self.pos_version = ws, pos, pos._version
+        self.tri, self.opp, self.shape, self.rast_version = tri, opp, shape, rast_version
+
+
 def _aa_prepare_end(prep, rast):
     """Join the preparation back into the current stream and hand its workspace
     to the next antialias of this rast (``_AntialiasFunc``)."""
     ws, side, pos, tri, opp, shape = prep
     torch.cuda.current_stream(pos.device).wait_stream(side)
-    rast._drp_aa_prep = (ws, pos, pos._version, tri, opp, shape, rast._version)
+    rast._drp_aa_prep = _AAPrep(ws, pos, tri, opp, shape, rast._version)
 
 
 def rasterize(glctx, pos, tri, resolution, ranges=None, grad_db=True):
@@ -509,24 +516,23 @@ class _InterpolateFunc(torch.autograd.Function):
         nd = fctx.nd
         dy = _as_f32(dy, 'grad_out')
         gattr = torch.zeros_like(attr)
-        grast = torch.empty_like(rast)
         L = _lib.lib()
-        if src_pos is not None and fctx.needs_input_grad[1]:
-            # the rasterize backward of this grad_rast in the same pass; handed to
-            # that backward through the grad_rast tensor (used only when it arrives
-            # there unchanged)
+        if src_pos is not None and fctx.needs_input_grad[6]:
+            # the rasterize backward of this interpolation's grad_rast in the same
+            # pass, returned as the gradient of the rasterised positions; grad_rast
+            # itself is neither written nor returned, so the rasterize backward
+            # (linear in its grad_rast) sees only what rast's other consumers send
             gpos = torch.zeros_like(src_pos)
             off, inc = incidence(tri, V)
             vpg = _views_per_group(n, F)
             ws_bytes = L.drp_interpolate_rasterize_bwd_workspace_size(n, H, W, F, C, vpg)
             ws = torch.empty(ws_bytes, dtype=torch.uint8, device=attr.device)
             rc = L.drp_interpolate_rasterize_bwd(stream_of(attr), ptr(attr), n, V, C, ptr(rast), H, W, ptr(tri), F,
-                                                 ptr(src_pos), int(src_pos.dim() == 3), ptr(dy), ptr(gattr),
-                                                 ptr(grast), ptr(gpos), ptr(boxes), vpg, ptr(off), ptr(inc), ptr(ws),
-                                                 ws_bytes)
+                                                 ptr(src_pos), int(src_pos.dim() == 3), ptr(dy), ptr(gattr), None,
+                                                 ptr(gpos), ptr(boxes), vpg, ptr(off), ptr(inc), ptr(ws), ws_bytes)
             check(rc, 'interpolate + rasterize backward')
-            grast._drp_pos_grad = (gpos, rast.data_ptr(), grast._version)
-            return gattr, grast, None, None, None, None, None
+            return gattr, None, None, None, None, None, gpos
+        grast = torch.empty_like(rast)
         gdb = torch.empty_like(rast) if nd else None
         dda = _as_f32(dda, 'grad_da') if (nd and dda is not None) else None
         if nd and dda is None:
@@ -573,11 +579,14 @@ def interpolate(attr, rast, tri, rast_db=None, diff_attrs=None):
             rast_db = _as_f32(rast_db, 'rast_db')
     boxes = src_pos = None
     meta = getattr(rast, '_drp_boxes', None)
-    # boxes describe where each triangle's pixels lie; valid while rast is unmodified
-    if meta is not None and meta[1] == tri.shape[0] and meta[2] == rast._version and \
+    # boxes describe where each triangle id's pixels lie: valid while rast is
+    # unmodified, for any triangle list of the same length; the positions'
+    # chain only for the triangles that were rasterised
+    if meta is not None and meta[1][1] == tuple(tri.shape) and meta[2] == rast._version and \
             meta[0].shape[0] == rast.shape[0]:
         boxes = meta[0]
-        src_pos = meta[3]
+        if meta[1] == (tri.data_ptr(), tuple(tri.shape), tri._version):
+            src_pos = meta[3]
     return _InterpolateFunc.apply(attr, rast, tri, rast_db if diff_idx is not None else None, diff_idx, boxes,
                                   src_pos)
 
@@ -641,10 +650,11 @@ class _AntialiasFunc(torch.autograd.Function):
         L = _lib.lib()
         ws_bytes = L.drp_antialias_workspace_size(n, H, W, V, F, batched)
         prep = getattr(rast, '_drp_aa_prep', None)
-        if (prep is not None and prep[1] is pos and prep[2] == pos._version and prep[3] is tri and prep[4] is opp
-                and prep[5] == (n, H, W, V, F, batched) and prep[6] == rast._version):
+        if (prep is not None and prep.pos is pos and prep.pos_version == pos._version and prep.tri is tri
+                and prep.opp is opp and prep.shape == (n, H, W, V, F, batched)
+                and prep.rast_version == rast._version):
             # the rasterize call prepared this workspace (same positions, triangles and size)
-            ws, fwd = prep[0], L.drp_antialias_fwd_prepared
+            ws, fwd = prep.ws, L.drp_antialias_fwd_prepared
             rast._drp_aa_prep = None  # used once: the workspace now keeps this call's pair decisions
         else:
             ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=color.device)

@@ -239,15 +239,19 @@ int drp_interpolate_bwd(void *stream, const float *attr, int n_img, int V, int a
                         size_t workspace_bytes);
 
 /* Interpolate backward together with the backward of the rasterize call that
- * produced rast (the chain attr -> interpolate -> rast -> rasterize -> pos of
- * diffrp/rendering/interpolator.py:42-45,83 when interpolate is the only
- * consumer of rast's gradient): one pass over the pixels computes the
- * attribute contributions, grad_rast (written, as drp_interpolate_bwd writes
- * it) and that grad_rast's position contributions; one record search per
- * (image group, triangle) sums both.  grad_attr and grad_pos are accumulated
- * (caller zero-fills) and equal, bit for bit, drp_interpolate_bwd followed by
- * drp_rasterize_bwd on the grad_rast it wrote.  attr [V,C] (not batched), C in
- * {4, 8}; pos [V,4] or [n,V,4] (pos_batched); boxes from the rasterize call. */
+ * produced rast, for interpolate's share of rast's gradient (the chain attr ->
+ * interpolate -> rast -> rasterize -> pos of
+ * diffrp/rendering/interpolator.py:42-45,83): one pass over the pixels
+ * computes the attribute contributions, grad_rast (written, as
+ * drp_interpolate_bwd writes it, unless grad_rast is null) and that
+ * grad_rast's position contributions; one record search per (image group,
+ * triangle) sums both.  grad_attr and grad_pos are accumulated (caller
+ * zero-fills) and equal, bit for bit, drp_interpolate_bwd followed by
+ * drp_rasterize_bwd on the grad_rast it wrote.  The rasterize backward is
+ * linear in grad_rast, so gradients that rast receives from other consumers
+ * can go through drp_rasterize_bwd on their own.  attr [V,C] (not batched),
+ * C in {4, 8}; pos [V,4] or [n,V,4] (pos_batched); tri and boxes those of the
+ * rasterize call. */
 size_t drp_interpolate_rasterize_bwd_workspace_size(int n_img, int H, int W, int F, int C, int views_per_group);
 int drp_interpolate_rasterize_bwd(void *stream, const float *attr, int n_img, int V, int C, const float *rast, int H,
                                   int W, const int32_t *tri, int F, const float *pos, int pos_batched,
