@@ -98,6 +98,8 @@ _SIGS = {
     'drp_l2_loss_workspace_size': (c_size, []),
     'drp_l2_loss_fwd': (c_int, [c_void_p, c_void_p, c_void_p, c_ll, c_void_p, c_void_p, c_size]),
     'drp_l2_loss_bwd': (c_int, [c_void_p, c_void_p, c_void_p, c_ll, c_void_p, c_void_p, c_void_p]),
+    'drp_l2_loss_fwd_grad': (c_int, [c_void_p, c_void_p, c_void_p, c_ll, c_void_p, c_void_p, c_void_p, c_size]),
+    'drp_l2_loss_scale': (c_int, [c_void_p, c_void_p, c_ll, c_void_p]),
 }
 
 EXPORTED = tuple(_SIGS)

@@ -1205,8 +1205,28 @@ int drp_l2_loss_fwd(void *stream, const float *a, const float *b, long long n, f
     if (!loss || (n > 0 && (!a || !b))) return fail(-1, "null pointer argument");
     if (!workspace || workspace_bytes < l2_loss_ws_bytes())
         return fail(-1, "workspace too small: need %zu bytes, got %zu", l2_loss_ws_bytes(), workspace_bytes);
-    int rc = launch_l2_loss(a, b, n, loss, workspace, (hipStream_t)stream);
+    int rc = launch_l2_loss(a, b, n, loss, nullptr, workspace, (hipStream_t)stream);
     if (rc) return fail(rc, "l2 loss launch failed: %s", hipGetErrorString(hipGetLastError()));
+    return 0;
+}
+
+int drp_l2_loss_fwd_grad(void *stream, const float *a, const float *b, long long n, float *loss, float *grad_a,
+                         void *workspace, size_t workspace_bytes) {
+    if (n < 0) return fail(-1, "negative element count");
+    if (!loss || (n > 0 && (!a || !b || !grad_a))) return fail(-1, "null pointer argument");
+    if (!workspace || workspace_bytes < l2_loss_ws_bytes())
+        return fail(-1, "workspace too small: need %zu bytes, got %zu", l2_loss_ws_bytes(), workspace_bytes);
+    int rc = launch_l2_loss(a, b, n, loss, grad_a, workspace, (hipStream_t)stream);
+    if (rc) return fail(rc, "l2 loss launch failed: %s", hipGetErrorString(hipGetLastError()));
+    return 0;
+}
+
+int drp_l2_loss_scale(void *stream, float *grad, long long n, const float *grad_loss) {
+    if (n < 0) return fail(-1, "negative element count");
+    if (n == 0) return 0;
+    if (!grad || !grad_loss) return fail(-1, "null pointer argument");
+    int rc = launch_l2_loss_scale(grad, n, grad_loss, (hipStream_t)stream);
+    if (rc) return fail(rc, "l2 loss scale launch failed: %s", hipGetErrorString(hipGetLastError()));
     return 0;
 }
 

@@ -224,7 +224,9 @@ int launch_stencil_partition(const float *rast, long long npx, const int *stenci
 int launch_count_covered(const float *rast, int n_img, long long npx, int *counts, hipStream_t st);
 int launch_incidence(const int *tri, int F, int V, int *inc_off, int *inc, void *ws, hipStream_t st);
 size_t l2_loss_ws_bytes();
-int launch_l2_loss(const float *a, const float *b, long long n, float *loss, void *ws, hipStream_t st);
+int launch_l2_loss(const float *a, const float *b, long long n, float *loss, float *grad_a, void *ws,
+                   hipStream_t st);
+int launch_l2_loss_scale(float *grad, long long n, const float *grad_loss, hipStream_t st);
 int launch_l2_loss_bwd(const float *a, const float *b, long long n, const float *grad_loss, float *ga, float *gb,
                        hipStream_t st);
 int launch_clip_fwd(const float *xyz, int V, const float *VP, int n, float *pos, hipStream_t st);

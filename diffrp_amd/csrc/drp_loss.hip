@@ -1,6 +1,10 @@
 // Image L2 loss of the training step, sum over all elements of (a - b)^2, and
 // its gradient 2 g (a - b): one read of both images forward, one read of both
-// and one write of the gradient backward.
+// and one write of the gradient backward.  Or, when the forward already knows
+// the gradient will be wanted, the forward also writes 2 (a - b) (the gradient
+// for g = 1) in its pass and the backward scales it by g in place, a pass that
+// reads g alone when g is 1 (the gradient of a loss that is itself the
+// objective).
 //
 // The sum is deterministic: a fixed grid of kLossBlocks blocks, each thread
 // summing its grid-stride elements in double, the block's threads reduced in
@@ -30,7 +34,8 @@ __global__ void __launch_bounds__(kLossThreads) k_l2_partial(const v4f *__restri
                                                              const v4f *__restrict__ b, long long n4,
                                                              const float *__restrict__ a1,
                                                              const float *__restrict__ b1, long long n,
-                                                             double *__restrict__ partial) {
+                                                             double *__restrict__ partial, v4f *__restrict__ ga,
+                                                             float *__restrict__ ga1) {
     __shared__ double red[kLossThreads / 64];
     double s = 0.0;
     const long long stride = (long long)gridDim.x * blockDim.x;
@@ -38,11 +43,16 @@ __global__ void __launch_bounds__(kLossThreads) k_l2_partial(const v4f *__restri
         const v4f x = __builtin_nontemporal_load(a + i), y = __builtin_nontemporal_load(b + i);
         const float d0 = x.x - y.x, d1 = x.y - y.y, d2 = x.z - y.z, d3 = x.w - y.w;
         s += (double)(d0 * d0) + (double)(d1 * d1) + (double)(d2 * d2) + (double)(d3 * d3);
+        if (ga) {  // the gradient for g = 1, as k_l2_grad computes it: (2 g) (a - b)
+            const v4f g = {2.0f * d0, 2.0f * d1, 2.0f * d2, 2.0f * d3};
+            __builtin_nontemporal_store(g, ga + i);
+        }
     }
     // the tail that does not fill a float4
     for (long long i = 4 * n4 + (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         const float d = a1[i] - b1[i];
         s += (double)(d * d);
+        if (ga1) ga1[i] = 2.0f * d;
     }
     s = block_sum(s, red);
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
@@ -90,15 +100,40 @@ __global__ void __launch_bounds__(kLossThreads) k_l2_grad_scalar(const float *__
     if (gb) gb[i] = -g;
 }
 
+// grad *= *grad_loss in place over a bounded grid (grid-stride); every block
+// returns at once when it is 1
+__global__ void __launch_bounds__(kLossThreads) k_l2_scale(v4f *__restrict__ g4, long long n4, float *__restrict__ g1,
+                                                           long long n, const float *__restrict__ grad_loss) {
+    const float k = *grad_loss;
+    if (k == 1.0f) return;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) {
+        const v4f x = __builtin_nontemporal_load(g4 + i);
+        const v4f y = {x.x * k, x.y * k, x.z * k, x.w * k};
+        __builtin_nontemporal_store(y, g4 + i);
+    }
+    for (long long t = 4 * n4 + (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) g1[t] *= k;
+}
+
 size_t l2_loss_ws_bytes() { return sizeof(double) * kLossBlocks; }
 
-int launch_l2_loss(const float *a, const float *b, long long n, float *loss, void *ws, hipStream_t st) {
-    const long long n4 = ((((uintptr_t)a | (uintptr_t)b) & 15) == 0) ? n / 4 : 0;
+int launch_l2_loss(const float *a, const float *b, long long n, float *loss, float *grad_a, void *ws,
+                   hipStream_t st) {
+    const long long n4 = ((((uintptr_t)a | (uintptr_t)b | (uintptr_t)grad_a) & 15) == 0) ? n / 4 : 0;
     long long want = (n4 + kLossThreads - 1) / kLossThreads;
     const int blocks = (int)(want < 1 ? 1 : (want > kLossBlocks ? kLossBlocks : want));
     hipLaunchKernelGGL(k_l2_partial, dim3(blocks), dim3(kLossThreads), 0, st, (const v4f *)a, (const v4f *)b,
-                       n4, a, b, n, (double *)ws);
+                       n4, a, b, n, (double *)ws, (v4f *)grad_a, grad_a);
     hipLaunchKernelGGL(k_l2_finish, dim3(1), dim3(kLossThreads), 0, st, (const double *)ws, blocks, loss);
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int launch_l2_loss_scale(float *grad, long long n, const float *grad_loss, hipStream_t st) {
+    const long long n4 = (((uintptr_t)grad & 15) == 0) ? n / 4 : 0;  // unaligned: all of it in the scalar loop
+    const long long items = n4 > 0 ? n4 : n;
+    const long long want = (items + kLossThreads - 1) / kLossThreads;
+    const int blocks = (int)(want < 1 ? 1 : (want > kLossBlocks ? kLossBlocks : want));
+    hipLaunchKernelGGL(k_l2_scale, dim3(blocks), dim3(kLossThreads), 0, st, (v4f *)grad, n4, grad, n, grad_loss);
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

@@ -331,6 +331,15 @@ int drp_l2_loss_fwd(void *stream, const float *a, const float *b, long long n, f
                     size_t workspace_bytes);
 int drp_l2_loss_bwd(void *stream, const float *a, const float *b, long long n, const float *grad_loss, float *grad_a,
                     float *grad_b);
+/* The forward and, in the same pass over a and b, grad_a = 2 (a - b): the
+ * backward for g = 1, as drp_l2_loss_bwd computes it.  For another g,
+ * drp_l2_loss_scale multiplies it in place (grad *= *grad_loss; a pass that
+ * reads only *grad_loss when it is 1), which equals drp_l2_loss_bwd bit for
+ * bit (scaling by 2 is exact).  diffrp_amd.loss.l2_loss uses the pair when an
+ * input requires grad, keeping grad_a from forward to backward. */
+int drp_l2_loss_fwd_grad(void *stream, const float *a, const float *b, long long n, float *loss, float *grad_a,
+                         void *workspace, size_t workspace_bytes);
+int drp_l2_loss_scale(void *stream, float *grad, long long n, const float *grad_loss);
 
 #ifdef __cplusplus
 }

@@ -64,6 +64,29 @@ def test_l2_loss_unaligned_views_and_target_gradient(cuda):
     assert abs(float(loss) - float((d.double() ** 2).sum())) <= 1e-6 * float((d.double() ** 2).sum())
 
 
+def test_l2_loss_second_backward_recomputes(cuda):
+    # the forward's unit gradient is handed out once; a second backward through
+    # the same graph reads both images again
+    g = torch.Generator().manual_seed(84)
+    a = torch.rand(4 * 1000 + 1, generator=g).cuda().requires_grad_(True)
+    b = torch.rand(4 * 1000 + 1, generator=g).cuda().requires_grad_(True)
+    loss = l2_loss(a, b)
+    loss.backward(torch.tensor(0.5, device='cuda'), retain_graph=True)
+    ga1, gb1 = a.grad.clone(), b.grad.clone()
+    loss.backward(torch.tensor(2.0, device='cuda'))
+    d = a.detach() - b.detach()
+    assert torch.equal(ga1, (2.0 * 0.5) * d) and torch.equal(gb1, -((2.0 * 0.5) * d))
+    assert torch.equal(a.grad, ga1 + (2.0 * 2.0) * d) and torch.equal(b.grad, gb1 - (2.0 * 2.0) * d)
+
+
+def test_l2_loss_without_gradients_writes_none(cuda):
+    a = torch.rand(1000, device='cuda')
+    b = torch.rand(1000, device='cuda')
+    loss = l2_loss(a, b)
+    assert not loss.requires_grad
+    assert abs(float(loss) - float(((a.double() - b.double()) ** 2).sum())) <= 1e-6 * float(loss)
+
+
 def test_l2_loss_is_deterministic_and_capturable(cuda):
     g = torch.Generator().manual_seed(83)
     a = torch.rand(64, 96, 96, 8, generator=g).cuda().requires_grad_(True)
