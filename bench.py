@@ -220,9 +220,10 @@ def per_op_times(S, iters=10):
         'rasterize_bwd': lambda: L.drp_rasterize_bwd(s, ptr(pos), n, V, 1, ptr(S.tri), F, None, H, W, ptr(rast),
                                                      ptr(grast), None, ptr(gpos), ptr(boxes), ptr(off), ptr(inc),
                                                      ptr(pws_t), pws),
-        # what the step runs for the two backward passes above: one pass over the pixels
+        # what the step runs for the two backward passes above: one pass over the pixels (as nvdr calls
+        # it, without writing grad_rast)
         'interpolate_rasterize_bwd': lambda: L.drp_interpolate_rasterize_bwd(
-            s, ptr(attr), n, V, C, ptr(rast), H, W, ptr(S.tri), F, ptr(pos), 1, ptr(gout), ptr(gattr), ptr(grast),
+            s, ptr(attr), n, V, C, ptr(rast), H, W, ptr(S.tri), F, ptr(pos), 1, ptr(gout), ptr(gattr), None,
             ptr(gpos), ptr(boxes), vpg, ptr(off), ptr(inc), ptr(fws_t), fws),
     }
     res = {}
