@@ -235,20 +235,46 @@ __global__ void __launch_bounds__(256) k_gather_attr_q(const float *__restrict__
     const int out = blockIdx.y;
     if (v >= V) return;
     float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
-    const int i0 = inc_off[v], i1 = inc_off[v + 1];
+    const int i0 = inc_off[v], i1 = inc_off[v + 1], deg = i1 - i0;
     const int g0 = per_output_groups ? out : 0, g1 = per_output_groups ? out + 1 : n_groups;
-    for (int g = g0; g < g1; ++g) {
-        const float *P = partial + (long long)g * F * 3 * NC + 4 * q;
-        for (int i = i0; i < i1; i += 4) {  // four corners' loads in flight, summed in corner order
-            float4 x[4];
+    const long long gstride = (long long)F * 3 * NC;
+    if (deg <= 8) {
+        // the vertex's corners once, then two groups' loads of all of them in
+        // flight together; summed group by group in corner order
+        int c[8];
 #pragma unroll
-            for (int k = 0; k < 4; ++k)
-                if (i + k < i1) x[k] = *reinterpret_cast<const float4 *>(P + (long long)inc[i + k] * NC);
+        for (int k = 0; k < 8; ++k) c[k] = k < deg ? inc[i0 + k] : 0;
+        for (int g = g0; g < g1; g += 2) {
+            float4 x[2][8];
 #pragma unroll
-            for (int k = 0; k < 4; ++k)
-                if (i + k < i1) {
-                    s.x += x[k].x; s.y += x[k].y; s.z += x[k].z; s.w += x[k].w;
-                }
+            for (int h = 0; h < 2; ++h) {
+                const float *P = partial + (long long)(g + h) * gstride + 4 * q;
+#pragma unroll
+                for (int k = 0; k < 8; ++k)
+                    if (g + h < g1 && k < deg) x[h][k] = *reinterpret_cast<const float4 *>(P + (long long)c[k] * NC);
+            }
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int k = 0; k < 8; ++k)
+                    if (g + h < g1 && k < deg) {
+                        s.x += x[h][k].x; s.y += x[h][k].y; s.z += x[h][k].z; s.w += x[h][k].w;
+                    }
+        }
+    } else {
+        for (int g = g0; g < g1; ++g) {
+            const float *P = partial + (long long)g * gstride + 4 * q;
+            for (int i = i0; i < i1; i += 4) {  // four corners' loads in flight, summed in corner order
+                float4 x[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    if (i + k < i1) x[k] = *reinterpret_cast<const float4 *>(P + (long long)inc[i + k] * NC);
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    if (i + k < i1) {
+                        s.x += x[k].x; s.y += x[k].y; s.z += x[k].z; s.w += x[k].w;
+                    }
+            }
         }
     }
     float *o = grad_attr + ((long long)out * V + v) * NC + 4 * q;

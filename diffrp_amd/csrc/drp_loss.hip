@@ -6,16 +6,18 @@
 // reads g alone when g is 1 (the gradient of a loss that is itself the
 // objective).
 //
-// The sum is deterministic: a fixed grid of kLossBlocks blocks, each thread
-// summing its grid-stride elements in double, the block's threads reduced in
-// a fixed tree, then one block adding the block sums in block order.
+// The sum is deterministic: a grid fixed by the element count (at most
+// kLossBlocks blocks), each thread summing its elements in double (two float4
+// 256 apart per round, then grid-stride rounds), the block's threads reduced
+// in a fixed tree, then one block adding the block sums in a fixed order.
 #include "drp_common.h"
 #include "drp_kernels.h"
 
 namespace drp {
 
 typedef float v4f __attribute__((ext_vector_type(4)));
-constexpr int kLossBlocks = 2048;
+constexpr int kLossBlocks = 65536;
+constexpr int kLossPer = 2;  // float4 per thread and round
 constexpr int kLossThreads = 256;
 
 __device__ __forceinline__ double block_sum(double v, double *red) {
@@ -38,17 +40,31 @@ __global__ void __launch_bounds__(kLossThreads) k_l2_partial(const v4f *__restri
                                                              float *__restrict__ ga1) {
     __shared__ double red[kLossThreads / 64];
     double s = 0.0;
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) {
-        const v4f x = __builtin_nontemporal_load(a + i), y = __builtin_nontemporal_load(b + i);
-        const float d0 = x.x - y.x, d1 = x.y - y.y, d2 = x.z - y.z, d3 = x.w - y.w;
-        s += (double)(d0 * d0) + (double)(d1 * d1) + (double)(d2 * d2) + (double)(d3 * d3);
-        if (ga) {  // the gradient for g = 1, as k_l2_grad computes it: (2 g) (a - b)
-            const v4f g = {2.0f * d0, 2.0f * d1, 2.0f * d2, 2.0f * d3};
-            __builtin_nontemporal_store(g, ga + i);
+    const long long round = (long long)gridDim.x * blockDim.x * kLossPer;
+    for (long long i0 = (long long)blockIdx.x * blockDim.x * kLossPer + threadIdx.x; i0 < n4; i0 += round) {
+        v4f x[kLossPer], y[kLossPer];
+#pragma unroll
+        for (int k = 0; k < kLossPer; ++k) {
+            const long long i = i0 + (long long)k * blockDim.x;
+            if (i < n4) {
+                x[k] = __builtin_nontemporal_load(a + i);
+                y[k] = __builtin_nontemporal_load(b + i);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kLossPer; ++k) {
+            const long long i = i0 + (long long)k * blockDim.x;
+            if (i >= n4) break;
+            const float d0 = x[k].x - y[k].x, d1 = x[k].y - y[k].y, d2 = x[k].z - y[k].z, d3 = x[k].w - y[k].w;
+            s += (double)(d0 * d0) + (double)(d1 * d1) + (double)(d2 * d2) + (double)(d3 * d3);
+            if (ga) {  // the gradient for g = 1, as k_l2_grad computes it: (2 g) (a - b)
+                const v4f g = {2.0f * d0, 2.0f * d1, 2.0f * d2, 2.0f * d3};
+                __builtin_nontemporal_store(g, ga + i);
+            }
         }
     }
     // the tail that does not fill a float4
+    const long long stride = (long long)gridDim.x * blockDim.x;
     for (long long i = 4 * n4 + (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         const float d = a1[i] - b1[i];
         s += (double)(d * d);
@@ -56,6 +72,18 @@ __global__ void __launch_bounds__(kLossThreads) k_l2_partial(const v4f *__restri
     }
     s = block_sum(s, red);
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// the middle level of the sum: block b adds partials [b per, (b + 1) per) in a fixed order
+constexpr int kLossMid = 256;  // blocks of the middle level
+__global__ void __launch_bounds__(kLossThreads) k_l2_mid(const double *__restrict__ partial, int n_part, int per,
+                                                         double *__restrict__ mid) {
+    __shared__ double red[kLossThreads / 64];
+    double s = 0.0;
+    const int lo = blockIdx.x * per, hi = min(lo + per, n_part);
+    for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) s += partial[i];
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) mid[blockIdx.x] = s;
 }
 
 __global__ void __launch_bounds__(kLossThreads) k_l2_finish(const double *__restrict__ partial, int n_part,
@@ -115,16 +143,25 @@ __global__ void __launch_bounds__(kLossThreads) k_l2_scale(v4f *__restrict__ g4,
     for (long long t = 4 * n4 + (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) g1[t] *= k;
 }
 
-size_t l2_loss_ws_bytes() { return sizeof(double) * kLossBlocks; }
+size_t l2_loss_ws_bytes() { return sizeof(double) * (kLossBlocks + kLossMid); }
 
 int launch_l2_loss(const float *a, const float *b, long long n, float *loss, float *grad_a, void *ws,
                    hipStream_t st) {
     const long long n4 = ((((uintptr_t)a | (uintptr_t)b | (uintptr_t)grad_a) & 15) == 0) ? n / 4 : 0;
-    long long want = (n4 + kLossThreads - 1) / kLossThreads;
+    long long want = (n4 + kLossThreads * kLossPer - 1) / (kLossThreads * kLossPer);
     const int blocks = (int)(want < 1 ? 1 : (want > kLossBlocks ? kLossBlocks : want));
     hipLaunchKernelGGL(k_l2_partial, dim3(blocks), dim3(kLossThreads), 0, st, (const v4f *)a, (const v4f *)b,
                        n4, a, b, n, (double *)ws, (v4f *)grad_a, grad_a);
-    hipLaunchKernelGGL(k_l2_finish, dim3(1), dim3(kLossThreads), 0, st, (const double *)ws, blocks, loss);
+    const double *part = (const double *)ws;
+    int n_part = blocks;
+    if (blocks > kLossThreads * 8) {  // many block sums: a middle level first
+        const int per = (blocks + kLossMid - 1) / kLossMid;
+        double *mid = (double *)ws + kLossBlocks;
+        hipLaunchKernelGGL(k_l2_mid, dim3(kLossMid), dim3(kLossThreads), 0, st, part, blocks, per, mid);
+        part = mid;
+        n_part = kLossMid;
+    }
+    hipLaunchKernelGGL(k_l2_finish, dim3(1), dim3(kLossThreads), 0, st, part, n_part, loss);
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
@@ -132,7 +169,8 @@ int launch_l2_loss_scale(float *grad, long long n, const float *grad_loss, hipSt
     const long long n4 = (((uintptr_t)grad & 15) == 0) ? n / 4 : 0;  // unaligned: all of it in the scalar loop
     const long long items = n4 > 0 ? n4 : n;
     const long long want = (items + kLossThreads - 1) / kLossThreads;
-    const int blocks = (int)(want < 1 ? 1 : (want > kLossBlocks ? kLossBlocks : want));
+    constexpr long long kScaleBlocks = 2048;  // few blocks: when the scalar is 1 they all return at once
+    const int blocks = (int)(want < 1 ? 1 : (want > kScaleBlocks ? kScaleBlocks : want));
     hipLaunchKernelGGL(k_l2_scale, dim3(blocks), dim3(kLossThreads), 0, st, (v4f *)grad, n4, grad, n, grad_loss);
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
