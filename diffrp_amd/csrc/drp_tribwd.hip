@@ -73,6 +73,41 @@ __device__ __forceinline__ int block_excl_scan256(int v, int *wsum, int &total) 
     return pre + x - v;
 }
 
+// Quads [q0, q0 + NQP) of every segment's record: one thread per (segment,
+// quad), the contributions of its quad (rows of c4, one per quad of the phase)
+// summed in pixel order over the contiguous segment.
+template <int NQP>
+__device__ __forceinline__ void segment_sums(const float4 *c4, const int *seg_start, int nseg, float *rec, int nv,
+                                             int q0) {
+    for (int w = threadIdx.x; w < nseg * NQP; w += kBTP) {
+        const int sg = w / NQP, jq = w - sg * NQP;
+        const int p0 = seg_start[sg], p1 = seg_start[sg + 1];
+        const float4 *row = c4 + jq * kBTP;
+        float4 sum = row[p0];
+        for (int p = p0 + 1; p < p1; ++p) {
+            const float4 x = row[p];
+            sum.x += x.x; sum.y += x.y; sum.z += x.z; sum.w += x.w;
+        }
+        reinterpret_cast<float4 *>(rec + (long long)sg * nv)[q0 + jq] = sum;
+    }
+}
+
+// Phase PH of the staged contributions: once the previous phase's sums have
+// read the buffer, this pixel's quads [PH QP, min(NQ, (PH + 1) QP)) go to its
+// sorted place, then their segment sums.
+template <int PH, int QP, int NQ, int NVAL>
+__device__ __forceinline__ void reduce_phase(float4 *c4, const float (&val)[NVAL], int my, const int *seg_start,
+                                             int nseg, float *rec, int nv) {
+    constexpr int j0 = PH * QP, j1 = (PH + 1) * QP < NQ ? (PH + 1) * QP : NQ;
+    __syncthreads();
+#pragma unroll
+    for (int j = j0; j < j1; ++j)
+        c4[(j - j0) * kBTP + my] = make_float4(val[4 * j], val[4 * j + 1], val[4 * j + 2], val[4 * j + 3]);
+    __syncthreads();
+    segment_sums<j1 - j0>(c4, seg_start, nseg, rec, nv, j0);
+}
+
+
 // MODE 0: POS (NV = 12), MODE 1: ATTR (NV = 3 C), MODE 2: both from one pass
 // (NV = 3 C + 12: the attribute sums, then the position sums of the grad_rast
 // this pass writes -- what MODE 0 would read back -- so both come out of one
@@ -80,8 +115,21 @@ __device__ __forceinline__ int block_excl_scan256(int v, int *wsum, int &total) 
 template <int MODE>
 constexpr int tile_nv(int C) { return MODE == 0 ? 12 : (MODE == 1 ? 3 * C : 3 * C + 12); }
 
+// Contribution phases by record width (LDS per block: QP quads x 4 KiB) and
+// the waves per SIMD the register budget is cut for (1: no cut): the
+// 36-float records (C = 8, both sums) in three phases of three quads with 64
+// VGPRs (eight waves; a few registers spill) beat two phases at six waves
+// (0.86 against 0.89 ms for the fused backward at 64 x 512^2); 24-float
+// records are fastest in two.
+template <int MODE, int C>
+constexpr int reduce_phases() {
+    return tile_nv<MODE>(C) % 4 ? 1 : (tile_nv<MODE>(C) > 32 ? 3 : (tile_nv<MODE>(C) > 16 ? 2 : 1));
+}
+template <int MODE, int C>
+constexpr int reduce_waves() { return tile_nv<MODE>(C) > 32 && tile_nv<MODE>(C) <= 36 ? 8 : 1; }
+
 template <int MODE, int C, bool DB, typename K>
-__global__ void __launch_bounds__(256) k_tile_reduce(TriBwdLaunch L) {
+__global__ void __launch_bounds__(256, (reduce_waves<MODE, C>())) k_tile_reduce(TriBwdLaunch L) {
     constexpr int NV = tile_nv<MODE>(C);
     constexpr K kEmptyT = SortKey<K>::kEmptyT;
     __shared__ K key[kBTP];
@@ -89,13 +137,23 @@ __global__ void __launch_bounds__(256) k_tile_reduce(TriBwdLaunch L) {
         K run[kBTP];             // the waves' sorted runs (merge)
         int seg_start[kBTP + 1]; // segment starts (after the merge)
     } U;
-    // contributions: four components per float4, quad-major ([NV / 4][pixel]) when NV % 4 == 0,
-    // else component rows with a padded stride
+    // contributions: four components per float4, quad-major ([quad][pixel]) when NV % 4 == 0,
+    // else component rows with a padded stride.  LDS sets this kernel's
+    // occupancy, so records of more than four quads go through a buffer of
+    // QP quads in NPH phases (quads [0, QP), [QP, 2 QP), ...); the buffer holds
+    // the NDC centres of the tile's columns and rows before the sort.
     constexpr bool QUAD = NV % 4 == 0;
-    __shared__ __align__(16) float contrib[QUAD ? NV * kBTP : NV * kRowStride];
+    constexpr int NQ = NV / 4;
+    constexpr int NPH = reduce_phases<MODE, C>();
+    constexpr int QP = (NQ + NPH - 1) / NPH;  // quads per phase (the last may have fewer)
+    __shared__ __align__(16) union {
+        float contrib[QUAD ? 4 * QP * kBTP : NV * kRowStride];
+        double sp[2 * kBT];  // column centres, then row centres (position chain)
+    } B;
     __shared__ int wsum[4];
     __shared__ unsigned char rank[kBTP];  // sorted place of each pixel's key
-    __shared__ double spx[kBT], spy[kBT];  // NDC centres of the tile's columns and rows (position chain)
+    double *spx = B.sp, *spy = B.sp + kBT;
+    float *contrib = B.contrib;
     const int tid = threadIdx.x;
     const int tile = xcd_remap(blockIdx.x, L.n_tiles_pad);
     if (tile >= L.n_tiles) return;  // uniform per block
@@ -239,8 +297,8 @@ __global__ void __launch_bounds__(256) k_tile_reduce(TriBwdLaunch L) {
     if (QUAD) {
         float4 *c4 = reinterpret_cast<float4 *>(contrib);
 #pragma unroll
-        for (int j = 0; j < NV; j += 4)
-            c4[(j / 4) * kBTP + my] = make_float4(val[j], val[j + 1], val[j + 2], val[j + 3]);
+        for (int j = 0; j < QP; ++j)
+            c4[j * kBTP + my] = make_float4(val[4 * j], val[4 * j + 1], val[4 * j + 2], val[4 * j + 3]);
     } else {
 #pragma unroll
         for (int j = 0; j < NV; ++j) contrib[j * kRowStride + my] = val[j];
@@ -271,19 +329,12 @@ __global__ void __launch_bounds__(256) k_tile_reduce(TriBwdLaunch L) {
     if (tid == 0) L.rec_count[tile] = nseg;
     if (QUAD) {
         // one thread per (segment, four components), summed in pixel order over the contiguous segment
-        constexpr int NQ = NV / 4;
-        const float4 *c4 = reinterpret_cast<const float4 *>(contrib);
-        for (int w = tid; w < nseg * NQ; w += kBTP) {
-            const int sg = w / NQ, jq = w - sg * NQ;
-            const int p0 = U.seg_start[sg], p1 = U.seg_start[sg + 1];
-            const float4 *row = c4 + jq * kBTP;
-            float4 sum = row[p0];
-            for (int p = p0 + 1; p < p1; ++p) {
-                const float4 x = row[p];
-                sum.x += x.x; sum.y += x.y; sum.z += x.z; sum.w += x.w;
-            }
-            reinterpret_cast<float4 *>(L.rec_v + (rbase + sg) * NV)[jq] = sum;
-        }
+        float4 *c4 = reinterpret_cast<float4 *>(contrib);
+        float *rec = L.rec_v + rbase * NV;
+        segment_sums<QP>(c4, U.seg_start, nseg, rec, NV, 0);
+        if constexpr (NPH > 1) reduce_phase<1, QP, NQ>(c4, val, my, U.seg_start, nseg, rec, NV);
+        if constexpr (NPH > 2) reduce_phase<2, QP, NQ>(c4, val, my, U.seg_start, nseg, rec, NV);
+        static_assert(NPH <= 3, "at most three phases");
     } else {
         // one thread per (segment, component): consecutive threads -> consecutive record floats
         for (int w = tid; w < nseg * NV; w += kBTP) {
