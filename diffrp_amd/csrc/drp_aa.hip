@@ -353,7 +353,9 @@ constexpr int kAAPairPer = kAAPairPix / 256;   // pixels per thread
 constexpr int kAAPairDir = 9;                  // queue entry: local pixel | direction << kAAPairDir
 static_assert(kAAPairPix <= (1 << kAAPairDir), "the queue's direction bit must lie above every local pixel index");
 
-__global__ void __launch_bounds__(256) k_aa_pairs(AALaunch L, float2 *__restrict__ weights,
+// eight waves per SIMD (64 VGPRs, no spill; 66 left it at seven): the pair
+// search waits on chains of dependent loads, so more waves hide more of them
+__global__ void __launch_bounds__(256, 8) k_aa_pairs(AALaunch L, float2 *__restrict__ weights,
                                                   int2 *__restrict__ decisions) {
     __shared__ int q[2 * kAAPairPix];  // local pixel | direction << kAAPairDir (0: right, 1: above)
     __shared__ float2 sw[kAAPairPix];
