@@ -503,6 +503,10 @@ class _InterpolateFunc(torch.autograd.Function):
         fctx.save_for_backward(attr, rast, tri, rast_db if nd else None, diff_idx if nd else None, boxes,
                                src_pos if fused else None)
         fctx.nd = nd
+        # an antialias of these positions downstream may leave its position
+        # gradient here (_AntialiasFunc): this node's backward adds its own to it
+        fctx._drp_fused_pos = src_pos if fused else None
+        fctx._drp_aa_pos = None
         if not nd:
             fctx.mark_non_differentiable(out_da)
         return out, out_da
@@ -517,12 +521,13 @@ class _InterpolateFunc(torch.autograd.Function):
         dy = _as_f32(dy, 'grad_out')
         gattr = torch.zeros_like(attr)
         L = _lib.lib()
+        aa_gpos = _take_aa_pos(fctx)
         if src_pos is not None and fctx.needs_input_grad[6]:
             # the rasterize backward of this interpolation's grad_rast in the same
             # pass, returned as the gradient of the rasterised positions; grad_rast
             # itself is neither written nor returned, so the rasterize backward
             # (linear in its grad_rast) sees only what rast's other consumers send
-            gpos = torch.zeros_like(src_pos)
+            gpos = aa_gpos if aa_gpos is not None else torch.zeros_like(src_pos)  # (accumulated into)
             off, inc = incidence(tri, V)
             vpg = _views_per_group(n, F)
             ws_bytes = L.drp_interpolate_rasterize_bwd_workspace_size(n, H, W, F, C, vpg)
@@ -550,7 +555,7 @@ class _InterpolateFunc(torch.autograd.Function):
                                    ptr(rast_db), ptr(diff_idx), nd, ptr(dy), ptr(dda), ptr(gattr), ptr(grast),
                                    ptr(gdb), ptr(boxes), vpg, ptr(off), ptr(inc), ptr(ws), ws_bytes)
         check(rc, 'interpolate backward')
-        return gattr, grast, None, gdb, None, None, None
+        return gattr, grast, None, gdb, None, None, aa_gpos
 
 
 def interpolate(attr, rast, tri, rast_db=None, diff_attrs=None):
@@ -640,6 +645,35 @@ def _cached_topology(tri: torch.Tensor):
     return th
 
 
+_HAND_AA_POS = True  # hand the antialias position gradient to a fused interpolation (tests switch it off)
+
+
+def _fused_host(color, pos, limit=64):
+    """A fused interpolation among the autograd ancestors of ``color`` (at
+    most ``limit`` nodes searched) whose rasterised positions are ``pos``: its
+    backward runs after the antialias backward of ``color`` (the gradient it
+    needs passes through there) and returns ``pos``'s gradient, so it can add
+    the antialias's share to its own instead of autograd adding the two."""
+    stack, seen = [color.grad_fn], set()
+    while stack and len(seen) < limit:
+        f = stack.pop()
+        if f is None or id(f) in seen:
+            continue
+        seen.add(id(f))
+        if getattr(f, '_drp_fused_pos', None) is pos:
+            return f
+        stack.extend(g for g, _ in f.next_functions)
+    return None
+
+
+def _take_aa_pos(fctx):
+    """The antialias position gradient left on an interpolation node in this
+    backward pass (None if none, or if it was left by an earlier pass that did
+    not reach this node)."""
+    dep, fctx._drp_aa_pos = fctx._drp_aa_pos, None
+    return dep[0] if dep is not None and dep[1] == torch._C._current_graph_task_id() else None
+
+
 class _AntialiasFunc(torch.autograd.Function):
     @staticmethod
     def forward(fctx, color, rast, pos, tri, opp, boost):
@@ -667,6 +701,7 @@ class _AntialiasFunc(torch.autograd.Function):
         # the workspace keeps the pair decisions; the backward reads them
         # (saved tensors cannot change in place before backward, torch checks)
         fctx.ws = ws
+        fctx.host = _fused_host(color, pos) if _HAND_AA_POS and fctx.needs_input_grad[2] else None
         return out
 
     @staticmethod
@@ -677,7 +712,14 @@ class _AntialiasFunc(torch.autograd.Function):
         V, F = pos.shape[-2], tri.shape[0]
         dy = _as_f32(dy, 'grad_out')
         gcol = torch.empty_like(color)
-        gpos = torch.zeros_like(pos) if fctx.needs_input_grad[2] else None
+        host, gpos = fctx.host, None
+        if fctx.needs_input_grad[2]:
+            # with a fused interpolation of pos upstream, accumulate into the
+            # gradient already left there in this pass (another antialias), and
+            # leave the sum for that interpolation's backward to add to
+            task = torch._C._current_graph_task_id()
+            dep = host._drp_aa_pos if host is not None else None
+            gpos = dep[0] if dep is not None and dep[1] == task else torch.zeros_like(pos)
         L = _lib.lib()
         ws, reuse = fctx.ws, 1
         if ws is None:
@@ -690,6 +732,9 @@ class _AntialiasFunc(torch.autograd.Function):
                                  ws.numel())
         fctx.ws = None  # released after its one reuse
         check(rc, 'antialias backward')
+        if host is not None and gpos is not None:
+            host._drp_aa_pos = (gpos, task)
+            return gcol, None, None, None, None, None
         return gcol, None, gpos, None, None, None
 
 
