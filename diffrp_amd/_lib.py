@@ -88,6 +88,9 @@ _SIGS = {
                                   c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_size]),
     'drp_antialias_prepare': (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_int, c_int, c_void_p, c_int,
                                       c_void_p, c_void_p, c_size]),
+    'drp_rasterize_fwd_aa_prepare': (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p, c_int, c_int, c_int,
+                                             c_void_p, c_void_p, c_void_p, c_size, c_int, c_int, c_int, c_void_p,
+                                             c_void_p, c_void_p, c_size]),
     'drp_antialias_fwd_prepared': (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_int,
                                            c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_size]),
     'drp_antialias_bwd': (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_int, c_int,

@@ -100,10 +100,12 @@ size_t drp_rasterize_workspace_size(int n_img, int H, int W, long long list_tota
     return raster_ws_layout(nullptr, tx * ty * n_img, n_img, list_total, tx * ty * n_img * (long long)bin_cap).bytes;
 }
 
-int drp_rasterize_fwd(void *stream, const float *pos, int n_img, int V, int pos_batched, const int32_t *tri, int F,
+static void aa_layout(AALaunch &L, void *workspace, int n_img, int H, int W, int V, int F, int pos_batched);
+
+static int raster_fwd(void *stream, const float *pos, int n_img, int V, int pos_batched, const int32_t *tri, int F,
                       const int32_t *ranges_dev, int H, int W, const float *prev, float *rast, float *rast_db,
                       void *workspace, size_t workspace_bytes, int tile_w, int tile_h, int bin_cap,
-                      long long list_total, int max_range, int32_t *boxes) {
+                      long long list_total, int max_range, int32_t *boxes, const AALaunch *aa) {
     if (int e = check_dims(n_img, H, W)) return e;
     if (n_img == 0) return 0;
     if (V < 0 || F < 0) return fail(-1, "negative V (%d) or F (%d)", V, F);
@@ -154,9 +156,47 @@ int drp_rasterize_fwd(void *stream, const float *pos, int n_img, int V, int pos_
     L.prev = prev;
     L.rast = rast;
     L.rast_db = rast_db;
+    L.aa_prepare = aa;
     int rc = launch_rasterize(L, (hipStream_t)stream);
     if (rc) return fail(rc, "rasterize launch failed: %s", hipGetErrorString(hipGetLastError()));
     return 0;
+}
+
+int drp_rasterize_fwd(void *stream, const float *pos, int n_img, int V, int pos_batched, const int32_t *tri, int F,
+                      const int32_t *ranges_dev, int H, int W, const float *prev, float *rast, float *rast_db,
+                      void *workspace, size_t workspace_bytes, int tile_w, int tile_h, int bin_cap,
+                      long long list_total, int max_range, int32_t *boxes) {
+    return raster_fwd(stream, pos, n_img, V, pos_batched, tri, F, ranges_dev, H, W, prev, rast, rast_db, workspace,
+                      workspace_bytes, tile_w, tile_h, bin_cap, list_total, max_range, boxes, nullptr);
+}
+
+int drp_rasterize_fwd_aa_prepare(void *stream, const float *pos, int n_img, int V, int pos_batched,
+                                 const int32_t *tri, int F, int H, int W, float *rast, float *rast_db,
+                                 void *workspace, size_t workspace_bytes, int tile_w, int tile_h, int bin_cap,
+                                 int32_t *boxes, const int32_t *opp, void *aa_workspace, size_t aa_workspace_bytes) {
+    if (int e = check_dims(n_img, H, W)) return e;
+    if (n_img == 0) return 0;
+    if (n_img > 65535) return fail(-1, "antialias takes at most 65535 images per call (got %d)", n_img);
+    if (V <= 0 || F <= 0) return fail(-1, "the antialias preparation needs vertices and triangles");
+    if (!opp) return fail(-1, "null pointer argument");
+    if (pos_batched && (long long)n_img * V >= (1ll << 31))
+        return fail(-1, "antialias with per-image positions takes fewer than 2^31 vertices in all");
+    const size_t need = drp_antialias_workspace_size(n_img, H, W, V, F, pos_batched);
+    if (!aa_workspace || aa_workspace_bytes < need)
+        return fail(-1, "antialias workspace too small: need %zu bytes, got %zu", need, aa_workspace_bytes);
+    AALaunch A{};
+    A.pos = pos;
+    A.tri = tri;
+    A.opp = opp;
+    A.n_img = n_img;
+    A.H = H;
+    A.W = W;
+    A.V = V;
+    A.F = F;
+    A.pos_batched = pos_batched;
+    aa_layout(A, aa_workspace, n_img, H, W, V, F, pos_batched);
+    return raster_fwd(stream, pos, n_img, V, pos_batched, tri, F, nullptr, H, W, nullptr, rast, rast_db, workspace,
+                      workspace_bytes, tile_w, tile_h, bin_cap, (long long)n_img * F, F, boxes, &A);
 }
 
 // ---- allocation-free rasterize ---------------------------------------------

@@ -47,6 +47,8 @@ inline int split_slots(long long n_tiles) { return (int)(n_tiles < kSplitSlotsMa
 // per slot: int4 of bookkeeping, a done counter, 63 queued chunks, 256 depth keys
 inline size_t split_region_bytes(int slots) { return (size_t)slots * (16 + 4 + 4 * 63 + 8 * 256); }
 
+struct AALaunch;
+
 struct RasterLaunch {
     Batch b;
     const float *pos;
@@ -65,6 +67,7 @@ struct RasterLaunch {
     const float *prev;
     float *rast;
     float *rast_db;
+    const AALaunch *aa_prepare;  // optional: the antialias preparation, run beside the fine raster
 };
 
 struct RasterBwdLaunch {

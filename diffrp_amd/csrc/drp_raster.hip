@@ -796,11 +796,19 @@ int launch_rasterize(const RasterLaunch &L, hipStream_t st) {
         hipLaunchKernelGGL(k_setup_bin, grid, dim3(256), 0, st, L.b, L.pos, L.tri, L.tile_w, L.tile_h,
                            L.bin_count, L.bins, L.list_count, L.lists, L.setup, L.boxes);
     }
+    const bool tile_known = (L.tile_w == 8 && L.tile_h == 8) || (L.tile_w == 16 && L.tile_h == 16) ||
+                            (L.tile_w == 32 && L.tile_h == 8) || (L.tile_w == 16 && L.tile_h == 8);
+    if (!tile_known) return -1;
+    // the antialias preparation (latency-bound) beside the fine raster (issue-bound)
+    Fork f{};
+    const bool par = L.aa_prepare && fork_begin(st, f);
+    int rc = L.aa_prepare ? launch_antialias_prepare(*L.aa_prepare, par ? f.side : st) : 0;
     if (L.tile_w == 8 && L.tile_h == 8) launch_fine_t<8, 8>(L, st);
     else if (L.tile_w == 16 && L.tile_h == 16) launch_fine_t<16, 16>(L, st);
     else if (L.tile_w == 32 && L.tile_h == 8) launch_fine_t<32, 8>(L, st);
-    else if (L.tile_w == 16 && L.tile_h == 8) launch_fine_t<16, 8>(L, st);
-    else return -1;
+    else launch_fine_t<16, 8>(L, st);
+    if (par && fork_end(st, f)) rc = -2;
+    if (rc) return rc;
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

@@ -59,9 +59,10 @@ class RasterizeCudaContext:
     ``prepare_antialias=True`` (an option of this implementation): every
     single-layer ``rasterize`` of this context also prepares what a following
     ``antialias`` of the same ``pos`` and ``tri`` computes per vertex and per
-    triangle (pixel positions, screen orientations, silhouette flags), on an
-    auxiliary stream beside the rasterizer and joined before ``rasterize``
-    returns; ``antialias`` then starts at its pair search.  The results are
+    triangle (pixel positions, screen orientations, silhouette flags), on the
+    library's auxiliary stream beside the fine raster pass and joined before
+    the rasterize call's work ends; ``antialias`` then starts at its pair
+    search.  The results are
     the same; the preparation is wasted work when no such ``antialias`` follows.
     """
 
@@ -71,7 +72,6 @@ class RasterizeCudaContext:
             device = torch.cuda.current_device()
         self.device = torch.device('cuda', device) if isinstance(device, int) else torch.device(device)
         self.prepare_antialias = bool(prepare_antialias)
-        self._side = None  # auxiliary stream of the antialias preparation (made on first use)
         if tile is not None and tuple(tile) not in _TILES:
             raise ValueError(f'tile must be one of {sorted(_TILES)}')
         self.tile = None if tile is None else tuple(tile)
@@ -245,7 +245,7 @@ class _Ranges:
 # ----------------------------------------------------------------------------
 # rasterize
 # ----------------------------------------------------------------------------
-def _rasterize_launch(ctx, pos, tri, H, W, rng, prev, want_db):
+def _rasterize_launch(ctx, pos, tri, H, W, rng, prev, want_db, aa=None):
     batched = pos.dim() == 3
     n = pos.shape[0] if batched else (rng.n if rng is not None else 1)
     V = pos.shape[-2]
@@ -264,9 +264,14 @@ def _rasterize_launch(ctx, pos, tri, H, W, rng, prev, want_db):
     L = _lib.lib()
     ws_bytes = L.drp_rasterize_workspace_size(n, H, W, list_total, tw, th, cap)
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
-    rc = L.drp_rasterize_fwd(stream_of(pos), ptr(pos), n, V, int(batched), ptr(tri), F,
-                             ptr(rng.dev) if rng is not None else None, H, W, ptr(prev), ptr(rast), ptr(rast_db),
-                             ptr(ws), ws_bytes, tw, th, cap, list_total, max_range, ptr(boxes))
+    if aa is not None:  # the antialias preparation (aa = (workspace, opp)) beside the fine raster
+        rc = L.drp_rasterize_fwd_aa_prepare(stream_of(pos), ptr(pos), n, V, int(batched), ptr(tri), F, H, W,
+                                            ptr(rast), ptr(rast_db), ptr(ws), ws_bytes, tw, th, cap, ptr(boxes),
+                                            ptr(aa[1]), ptr(aa[0]), aa[0].numel())
+    else:
+        rc = L.drp_rasterize_fwd(stream_of(pos), ptr(pos), n, V, int(batched), ptr(tri), F,
+                                 ptr(rng.dev) if rng is not None else None, H, W, ptr(prev), ptr(rast), ptr(rast_db),
+                                 ptr(ws), ws_bytes, tw, th, cap, list_total, max_range, ptr(boxes))
     check(rc, 'rasterize')
     n_tiles = n * ((W + tw - 1) // tw) * ((H + th - 1) // th)
     ctx._watch(ctx.tuning_key(n, H, W, F), (tw, th), cap, ws[:4 * n_tiles].view(torch.int32), n, F)  # bin counts
@@ -296,8 +301,8 @@ def incidence(tri: torch.Tensor, V: int):
 
 class _RasterizeFunc(torch.autograd.Function):
     @staticmethod
-    def forward(fctx, raster_ctx, pos, tri, H, W, rng, prev, want_db, holder):
-        rast, rast_db, boxes = _rasterize_launch(raster_ctx, pos, tri, H, W, rng, prev, want_db)
+    def forward(fctx, raster_ctx, pos, tri, H, W, rng, prev, want_db, holder, aa):
+        rast, rast_db, boxes = _rasterize_launch(raster_ctx, pos, tri, H, W, rng, prev, want_db, aa)
         fctx.save_for_backward(pos, tri, rast, boxes)
         fctx.rng = rng
         fctx.HW = (H, W)
@@ -316,7 +321,7 @@ class _RasterizeFunc(torch.autograd.Function):
         rng = fctx.rng
         gdb_used = fctx.want_db and ddb is not None and ddb.numel() > 0
         if dy is None and not gdb_used:
-            return None, None, None, None, None, None, None, None, None
+            return None, None, None, None, None, None, None, None, None, None
         gpos = torch.zeros_like(pos)
         dy = _as_f32(dy, 'grad_rast') if dy is not None else torch.zeros_like(rast)
         gdb = _as_f32(ddb, 'grad_db') if gdb_used else None
@@ -332,7 +337,7 @@ class _RasterizeFunc(torch.autograd.Function):
                                  ptr(rng.dev) if rng is not None else None, H, W, ptr(rast), ptr(dy), ptr(gdb),
                                  ptr(gpos), ptr(boxes), ptr(off), ptr(inc), ptr(ws), ws_bytes)
         check(rc, 'rasterize backward')
-        return None, gpos, None, None, None, None, None, None, None
+        return None, gpos, None, None, None, None, None, None, None, None
 
 
 def _rasterize_impl(glctx, pos, tri, resolution, ranges, grad_db, prev):
@@ -354,11 +359,12 @@ def _rasterize_impl(glctx, pos, tri, resolution, ranges, grad_db, prev):
     elif ranges is not None:
         raise ValueError('ranges are only valid with 2-D pos (range mode)')
     holder = []
-    prep = _aa_prepare_begin(glctx, pos, tri, H, W) if glctx.prepare_antialias and rng is None and prev is None \
-        else None
-    rast, rast_db = _RasterizeFunc.apply(glctx, pos, tri, H, W, rng, prev, bool(grad_db), holder)
+    prep = _aa_prepare_args(pos, tri, H, W) if glctx.prepare_antialias and rng is None and prev is None else None
+    rast, rast_db = _RasterizeFunc.apply(glctx, pos, tri, H, W, rng, prev, bool(grad_db), holder,
+                                         None if prep is None else prep[:2])
     if prep is not None:
-        _aa_prepare_end(prep, rast)
+        # the next antialias of this rast, positions and triangles starts at its pair search
+        rast._drp_aa_prep = _AAPrep(prep[0], pos, tri, prep[1], prep[2], rast._version)
     if holder and holder[0] is not None:
         # pixel boxes of this rasterisation, for the triangle-centric interpolate
         # backward, and the positions and triangles it rasterised (when
@@ -369,29 +375,18 @@ def _rasterize_impl(glctx, pos, tri, resolution, ranges, grad_db, prev):
     return rast, rast_db
 
 
-def _aa_prepare_begin(glctx, pos, tri, H, W):
-    """Launch the antialias preparation of (pos, tri, H, W) on the context's
-    auxiliary stream, forked from the current one (the rasterizer then runs
-    beside it)."""
+def _aa_prepare_args(pos, tri, H, W):
+    """The workspace, topology and shape of the antialias preparation that the
+    rasterize call runs beside its fine raster (``drp_rasterize_fwd_aa_prepare``)."""
     n = pos.shape[0] if pos.dim() == 3 else 1
     batched = int(pos.dim() == 3)
     V, F = pos.shape[-2], tri.shape[0]
-    if F == 0 or V == 0:
+    if F == 0 or V == 0 or n == 0:
         return None
     opp = _cached_topology(tri).opp
-    L = _lib.lib()
-    ws_bytes = L.drp_antialias_workspace_size(n, H, W, V, F, batched)
+    ws_bytes = _lib.lib().drp_antialias_workspace_size(n, H, W, V, F, batched)
     ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=pos.device)
-    cur = torch.cuda.current_stream(pos.device)
-    if glctx._side is None:
-        glctx._side = torch.cuda.Stream(pos.device)  # default priority
-    side = glctx._side
-    side.wait_stream(cur)
-    check(L.drp_antialias_prepare(ctypes.c_void_p(side.cuda_stream), n, H, W, ptr(pos), V, batched, ptr(tri), F,
-                                  ptr(opp), ptr(ws), ws_bytes), 'antialias prepare')
-    for t in (ws, pos, tri, opp):
-        t.record_stream(side)  # their memory is not reused before the side stream is done with it
-    return ws, side, pos, tri, opp, (n, H, W, V, F, batched)
+    return ws, opp, (n, H, W, V, F, batched)
 
 
 class _AAPrep:
@@ -402,14 +397,6 @@ class _AAPrep:
     def __init__(self, ws, pos, tri, opp, shape, rast_version):
         self.ws, self.pos, self.pos_version = ws, pos, pos._version
         self.tri, self.opp, self.shape, self.rast_version = tri, opp, shape, rast_version
-
-
-def _aa_prepare_end(prep, rast):
-    """Join the preparation back into the current stream and hand its workspace
-    to the next antialias of this rast (``_AntialiasFunc``)."""
-    ws, side, pos, tri, opp, shape = prep
-    torch.cuda.current_stream(pos.device).wait_stream(side)
-    rast._drp_aa_prep = _AAPrep(ws, pos, tri, opp, shape, rast._version)
 
 
 def rasterize(glctx, pos, tri, resolution, ranges=None, grad_db=True):

@@ -285,6 +285,17 @@ int drp_antialias_fwd(void *stream, const float *color, int n_img, int H, int W,
  * contexts made with prepare_antialias=True). */
 int drp_antialias_prepare(void *stream, int n_img, int H, int W, const float *pos, int V, int pos_batched,
                           const int32_t *tri, int F, const int32_t *opp, void *workspace, size_t workspace_bytes);
+/* drp_rasterize_fwd of one layer with every triangle (no ranges, no previous
+ * layer) that also runs drp_antialias_prepare for the same pos, tri and opp
+ * into aa_workspace (no counterpart in nvdiffrast): its per-vertex and
+ * per-triangle passes, which wait on memory, run on the auxiliary stream
+ * beside the fine raster, which waits on instruction issue, and are joined
+ * before the call's work ends.  nvdr.rasterize uses it for contexts made with
+ * prepare_antialias=True. */
+int drp_rasterize_fwd_aa_prepare(void *stream, const float *pos, int n_img, int V, int pos_batched,
+                                 const int32_t *tri, int F, int H, int W, float *rast, float *rast_db,
+                                 void *workspace, size_t workspace_bytes, int tile_w, int tile_h, int bin_cap,
+                                 int32_t *boxes, const int32_t *opp, void *aa_workspace, size_t aa_workspace_bytes);
 /* drp_antialias_fwd with the workspace already prepared by drp_antialias_prepare. */
 int drp_antialias_fwd_prepared(void *stream, const float *color, int n_img, int H, int W, int C, const float *rast,
                                const float *pos, int V, int pos_batched, const int32_t *tri, int F,
