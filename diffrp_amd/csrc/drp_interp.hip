@@ -222,47 +222,51 @@ __global__ void __launch_bounds__(256) k_gather_attr(const float *__restrict__ p
     for (int ch = 0; ch < NC; ++ch) o[ch] += s[ch];
 }
 
-// The same with one thread per (vertex, four channels) for 8 and 16 channels,
-// so that a vertex's corner loads spread over NC / 4 lanes (the same sums in
-// the same order per channel).
+// The same for 8 and 16 channels with kAttrLanes lanes per (vertex, four
+// channels): lane s takes groups s, s + kAttrLanes, ... (two groups' corner
+// rows in flight together, each group's corners summed in order), and the
+// lanes' sums are added in a fixed tree.  (One lane per (vertex, quad) left
+// a grid of barely one wave per SIMD waiting on its loads.)
+constexpr int kAttrLanes = 4;
 template <int NC>
 __global__ void __launch_bounds__(256) k_gather_attr_q(const float *__restrict__ partial, int n_groups, int F,
                                                        int V, int per_output_groups, const int *__restrict__ inc_off,
                                                        const int *__restrict__ inc, float *__restrict__ grad_attr) {
     constexpr int NQ = NC / 4;
-    const long long tq = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long tl = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = (int)(tl % kAttrLanes);
+    const long long tq = tl / kAttrLanes;
     const int v = (int)(tq / NQ), q = (int)(tq % NQ);
     const int out = blockIdx.y;
-    if (v >= V) return;
-    float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (v >= V) return;  // all lanes of (v, q) together
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
     const int i0 = inc_off[v], i1 = inc_off[v + 1], deg = i1 - i0;
     const int g0 = per_output_groups ? out : 0, g1 = per_output_groups ? out + 1 : n_groups;
     const long long gstride = (long long)F * 3 * NC;
     if (deg <= 8) {
-        // the vertex's corners once, then two groups' loads of all of them in
-        // flight together; summed group by group in corner order
         int c[8];
 #pragma unroll
         for (int k = 0; k < 8; ++k) c[k] = k < deg ? inc[i0 + k] : 0;
-        for (int g = g0; g < g1; g += 2) {
+        for (int g = g0 + s; g < g1; g += 2 * kAttrLanes) {
             float4 x[2][8];
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
-                const float *P = partial + (long long)(g + h) * gstride + 4 * q;
+                const int gh = g + h * kAttrLanes;
+                const float *P = partial + (long long)gh * gstride + 4 * q;
 #pragma unroll
                 for (int k = 0; k < 8; ++k)
-                    if (g + h < g1 && k < deg) x[h][k] = *reinterpret_cast<const float4 *>(P + (long long)c[k] * NC);
+                    if (gh < g1 && k < deg) x[h][k] = *reinterpret_cast<const float4 *>(P + (long long)c[k] * NC);
             }
 #pragma unroll
             for (int h = 0; h < 2; ++h)
 #pragma unroll
                 for (int k = 0; k < 8; ++k)
-                    if (g + h < g1 && k < deg) {
-                        s.x += x[h][k].x; s.y += x[h][k].y; s.z += x[h][k].z; s.w += x[h][k].w;
+                    if (g + h * kAttrLanes < g1 && k < deg) {
+                        acc.x += x[h][k].x; acc.y += x[h][k].y; acc.z += x[h][k].z; acc.w += x[h][k].w;
                     }
         }
     } else {
-        for (int g = g0; g < g1; ++g) {
+        for (int g = g0 + s; g < g1; g += kAttrLanes) {
             const float *P = partial + (long long)g * gstride + 4 * q;
             for (int i = i0; i < i1; i += 4) {  // four corners' loads in flight, summed in corner order
                 float4 x[4];
@@ -272,23 +276,30 @@ __global__ void __launch_bounds__(256) k_gather_attr_q(const float *__restrict__
 #pragma unroll
                 for (int k = 0; k < 4; ++k)
                     if (i + k < i1) {
-                        s.x += x[k].x; s.y += x[k].y; s.z += x[k].z; s.w += x[k].w;
+                        acc.x += x[k].x; acc.y += x[k].y; acc.z += x[k].z; acc.w += x[k].w;
                     }
             }
         }
     }
+    for (int o = 1; o < kAttrLanes; o <<= 1) {  // (a + b on one lane, b + a on the other: the same sum)
+        acc.x += __shfl_xor(acc.x, o, 64);
+        acc.y += __shfl_xor(acc.y, o, 64);
+        acc.z += __shfl_xor(acc.z, o, 64);
+        acc.w += __shfl_xor(acc.w, o, 64);
+    }
+    if (s != 0) return;
     float *o = grad_attr + ((long long)out * V + v) * NC + 4 * q;
-    o[0] += s.x;
-    o[1] += s.y;
-    o[2] += s.z;
-    o[3] += s.w;
+    o[0] += acc.x;
+    o[1] += acc.y;
+    o[2] += acc.z;
+    o[3] += acc.w;
 }
 
 template <int NC>
 static void launch_gather_attr_t(const float *partial, int groups, int F, int V, int outs, int per_output_groups,
                                  const int *inc_off, const int *inc, float *grad_attr, hipStream_t st) {
     if constexpr (NC == 8 || NC == 16) {
-        const long long n = (long long)V * (NC / 4);
+        const long long n = (long long)V * (NC / 4) * kAttrLanes;
         hipLaunchKernelGGL(k_gather_attr_q<NC>, dim3((unsigned)((n + 255) / 256), outs), dim3(256), 0, st, partial,
                            groups, F, V, per_output_groups, inc_off, inc, grad_attr);
     } else {

@@ -681,18 +681,35 @@ __global__ void __launch_bounds__(256) k_raster_bwd(Batch b, const float *__rest
 }
 
 // vertex gather: grad_pos[img][v] += sum over incident corners of partial[pimg][t][corner]
-// (pos_batched: one output per image; else all images sum into one buffer)
+// (pos_batched: one output per image; else all images sum into one buffer).
+// The corners of a block's 256 vertices are one contiguous range of each
+// image's partials (CSR order): it is staged in LDS with coalesced loads
+// when it fits, and each vertex then sums its slice from there, in corner
+// order as the direct loads do.
+constexpr int kGatherPosLds = 2048;  // float4 per block (32 KiB)
 __global__ void __launch_bounds__(256) k_gather_pos(const float4 *__restrict__ partial, int n_img, int F, int V,
                                                     int pos_batched, const int *__restrict__ inc_off,
                                                     float4 *__restrict__ grad_pos) {
-    int v = blockIdx.x * blockDim.x + threadIdx.x;
-    int oimg = blockIdx.y;
-    if (v >= V) return;
-    int i0 = inc_off[v], i1 = inc_off[v + 1];
+    __shared__ float4 buf[kGatherPosLds];
+    const int vb = blockIdx.x * blockDim.x, v = vb + threadIdx.x;
+    const int oimg = blockIdx.y;
+    const int b0 = inc_off[vb], b1 = inc_off[min(vb + (int)blockDim.x, V)];  // (vb < V: the grid covers V)
+    const bool staged = b1 - b0 <= kGatherPosLds;  // uniform per block
+    const int i0 = v < V ? inc_off[v] : b0, i1 = v < V ? inc_off[v + 1] : b0;
     float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
-    int ia = pos_batched ? oimg : 0, ib = pos_batched ? oimg + 1 : n_img;
+    const int ia = pos_batched ? oimg : 0, ib = pos_batched ? oimg + 1 : n_img;
     for (int img = ia; img < ib; ++img) {
         const float4 *P = partial + (long long)img * F * 3;  // this vertex's corners are P[i0 .. i1)
+        if (staged) {
+            if (img > ia) __syncthreads();  // the previous image's sums have read the buffer
+            for (int i = b0 + (int)threadIdx.x; i < b1; i += blockDim.x) buf[i - b0] = P[i];
+            __syncthreads();
+            for (int i = i0; i < i1; ++i) {
+                const float4 q = buf[i - b0];
+                s.x += q.x; s.y += q.y; s.z += q.z; s.w += q.w;
+            }
+            continue;
+        }
         // eight corners' loads in flight at once, summed in corner order
         for (int i = i0; i < i1; i += 8) {
             float4 q[8];
@@ -704,6 +721,7 @@ __global__ void __launch_bounds__(256) k_gather_pos(const float4 *__restrict__ p
                 if (i + k < i1) { s.x += q[k].x; s.y += q[k].y; s.z += q[k].z; s.w += q[k].w; }
         }
     }
+    if (v >= V) return;
     float4 *o = grad_pos + (long long)oimg * V + v;
     float4 cur = *o;
     *o = make_float4(cur.x + s.x, cur.y + s.y, cur.z + s.z, cur.w + s.w);
