@@ -222,20 +222,22 @@ __global__ void __launch_bounds__(256) k_gather_attr(const float *__restrict__ p
     for (int ch = 0; ch < NC; ++ch) o[ch] += s[ch];
 }
 
-// The same for 8 and 16 channels with kAttrLanes lanes per (vertex, four
-// channels): lane s takes groups s, s + kAttrLanes, ... (two groups' corner
-// rows in flight together, each group's corners summed in order), and the
-// lanes' sums are added in a fixed tree.  (One lane per (vertex, quad) left
-// a grid of barely one wave per SIMD waiting on its loads.)
+// The same for 8 and 16 channels with `lanes` lanes per (vertex, four
+// channels) (1, 2 or 4: up to the groups each output sums): lane s takes
+// groups s, s + lanes, ... (two groups' corner rows in flight together, each
+// group's corners summed in order), and the lanes' sums are added in a fixed
+// tree.  (One lane per (vertex, quad) left the 64-view grid at barely one wave
+// per SIMD waiting on its loads.)
 constexpr int kAttrLanes = 4;
 template <int NC>
 __global__ void __launch_bounds__(256) k_gather_attr_q(const float *__restrict__ partial, int n_groups, int F,
                                                        int V, int per_output_groups, const int *__restrict__ inc_off,
-                                                       const int *__restrict__ inc, float *__restrict__ grad_attr) {
+                                                       const int *__restrict__ inc, float *__restrict__ grad_attr,
+                                                       int lanes) {
     constexpr int NQ = NC / 4;
     const long long tl = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int s = (int)(tl % kAttrLanes);
-    const long long tq = tl / kAttrLanes;
+    const int s = (int)(tl & (lanes - 1));
+    const long long tq = tl / lanes;
     const int v = (int)(tq / NQ), q = (int)(tq % NQ);
     const int out = blockIdx.y;
     if (v >= V) return;  // all lanes of (v, q) together
@@ -247,11 +249,11 @@ __global__ void __launch_bounds__(256) k_gather_attr_q(const float *__restrict__
         int c[8];
 #pragma unroll
         for (int k = 0; k < 8; ++k) c[k] = k < deg ? inc[i0 + k] : 0;
-        for (int g = g0 + s; g < g1; g += 2 * kAttrLanes) {
+        for (int g = g0 + s; g < g1; g += 2 * lanes) {
             float4 x[2][8];
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
-                const int gh = g + h * kAttrLanes;
+                const int gh = g + h * lanes;
                 const float *P = partial + (long long)gh * gstride + 4 * q;
 #pragma unroll
                 for (int k = 0; k < 8; ++k)
@@ -261,12 +263,12 @@ __global__ void __launch_bounds__(256) k_gather_attr_q(const float *__restrict__
             for (int h = 0; h < 2; ++h)
 #pragma unroll
                 for (int k = 0; k < 8; ++k)
-                    if (g + h * kAttrLanes < g1 && k < deg) {
+                    if (g + h * lanes < g1 && k < deg) {
                         acc.x += x[h][k].x; acc.y += x[h][k].y; acc.z += x[h][k].z; acc.w += x[h][k].w;
                     }
         }
     } else {
-        for (int g = g0 + s; g < g1; g += kAttrLanes) {
+        for (int g = g0 + s; g < g1; g += lanes) {
             const float *P = partial + (long long)g * gstride + 4 * q;
             for (int i = i0; i < i1; i += 4) {  // four corners' loads in flight, summed in corner order
                 float4 x[4];
@@ -281,7 +283,7 @@ __global__ void __launch_bounds__(256) k_gather_attr_q(const float *__restrict__
             }
         }
     }
-    for (int o = 1; o < kAttrLanes; o <<= 1) {  // (a + b on one lane, b + a on the other: the same sum)
+    for (int o = 1; o < lanes; o <<= 1) {  // (a + b on one lane, b + a on the other: the same sum)
         acc.x += __shfl_xor(acc.x, o, 64);
         acc.y += __shfl_xor(acc.y, o, 64);
         acc.z += __shfl_xor(acc.z, o, 64);
@@ -299,9 +301,11 @@ template <int NC>
 static void launch_gather_attr_t(const float *partial, int groups, int F, int V, int outs, int per_output_groups,
                                  const int *inc_off, const int *inc, float *grad_attr, hipStream_t st) {
     if constexpr (NC == 8 || NC == 16) {
-        const long long n = (long long)V * (NC / 4) * kAttrLanes;
+        const int per_out = per_output_groups ? 1 : groups;
+        const int lanes = per_out >= kAttrLanes ? kAttrLanes : (per_out >= 2 ? 2 : 1);
+        const long long n = (long long)V * (NC / 4) * lanes;
         hipLaunchKernelGGL(k_gather_attr_q<NC>, dim3((unsigned)((n + 255) / 256), outs), dim3(256), 0, st, partial,
-                           groups, F, V, per_output_groups, inc_off, inc, grad_attr);
+                           groups, F, V, per_output_groups, inc_off, inc, grad_attr, lanes);
     } else {
         hipLaunchKernelGGL(k_gather_attr<NC>, dim3((V + 255) / 256, outs), dim3(256), 0, st, partial, groups, F, V,
                            per_output_groups, inc_off, inc, grad_attr);
