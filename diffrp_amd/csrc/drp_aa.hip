@@ -470,12 +470,22 @@ template <int CV>
 struct AAChunk {
     static constexpr int CH = (CV > 0 && CV % 4 == 0) ? 4 : 1;
 };
+// the backward gather takes a whole pixel for 8 channels (two float4 per
+// thread: 0.438 -> 0.424 ms for the antialias backward at 64 x 512^2; the
+// forward gather is faster with four channels per thread)
+template <int CV>
+struct AAChunkBwd {
+    static constexpr int CH = CV == 8 ? 8 : AAChunk<CV>::CH;
+};
 
 template <int CH>
 __device__ __forceinline__ void ld(const float *p, float (&x)[CH]) {
-    if (CH == 4) {
-        float4 v = *reinterpret_cast<const float4 *>(p);
-        x[0] = v.x; x[1] = v.y; x[2] = v.z; x[3] = v.w;
+    if (CH % 4 == 0) {
+#pragma unroll
+        for (int k = 0; k < CH; k += 4) {
+            const float4 v = *reinterpret_cast<const float4 *>(p + k);
+            x[k] = v.x; x[k + 1] = v.y; x[k + 2] = v.z; x[k + 3] = v.w;
+        }
     } else {
 #pragma unroll
         for (int i = 0; i < CH; ++i) x[i] = p[i];
@@ -484,10 +494,13 @@ __device__ __forceinline__ void ld(const float *p, float (&x)[CH]) {
 // streaming (non-temporal) stores: the output is not read again by this pass
 template <int CH>
 __device__ __forceinline__ void st(float *p, const float (&x)[CH]) {
-    if (CH == 4) {
+    if (CH % 4 == 0) {
         typedef float v4 __attribute__((ext_vector_type(4)));
-        v4 v = {x[0], x[1], x[2], x[3]};
-        __builtin_nontemporal_store(v, reinterpret_cast<v4 *>(p));
+#pragma unroll
+        for (int k = 0; k < CH; k += 4) {
+            const v4 v = {x[k], x[k + 1], x[k + 2], x[k + 3]};
+            __builtin_nontemporal_store(v, reinterpret_cast<v4 *>(p + k));
+        }
     } else
 #pragma unroll
         for (int i = 0; i < CH; ++i) __builtin_nontemporal_store(x[i], p + i);
@@ -794,7 +807,7 @@ __global__ void __launch_bounds__(256) k_aa_pairs_bwd(AALaunch L, float2 *__rest
 
 template <int CV>
 __global__ void __launch_bounds__(256) k_aa_gather_bwd(AALaunch L, const float2 *__restrict__ weights) {
-    constexpr int CH = AAChunk<CV>::CH;
+    constexpr int CH = AAChunkBwd<CV>::CH;
     const int C = CV > 0 ? CV : L.C;
     const int nch = C / CH;
     // XCD k takes the k-th eighth of the blocks, so a pixel's lower neighbour
@@ -865,8 +878,9 @@ static void launch_orient(const AALaunch &L, hipStream_t st) {
 }
 
 // the gathers' grid, padded to a multiple of 8 blocks (xcd_remap)
-static dim3 gather_grid(const AALaunch &L, long long total) {
+static dim3 gather_grid(const AALaunch &L, long long total, bool bwd = false) {
     int nch = (L.cv > 0 && L.cv % 4 == 0) ? L.cv / 4 : (L.cv > 0 ? L.cv : L.C);
+    if (bwd && L.cv == 8) nch = 1;  // AAChunkBwd: a pixel per thread
     const long long blocks = (total * nch + 255) / 256;
     return dim3((unsigned)((blocks + 7) & ~7ll));
 }
@@ -920,7 +934,7 @@ int launch_antialias_bwd(const AALaunch &L0, hipStream_t st) {
     hipLaunchKernelGGL(k_aa_pairs_bwd, dim3((unsigned)nb_pad), block, 0, par ? f.side : st, L, L.weights,
                        L.reuse ? (const int2 *)L.decisions : (const int2 *)nullptr, tiles_x, tiles_per_img,
                        (int)nb_pad);
-    dim3 grid = gather_grid(L, total);
+    dim3 grid = gather_grid(L, total, true);
     DRP_AA_DISPATCH(k_aa_gather_bwd, L, (const float2 *)L.weights)
     if (par && fork_end(st, f)) return -2;
     return hipGetLastError() == hipSuccess ? 0 : -2;
