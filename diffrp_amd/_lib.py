@@ -47,6 +47,10 @@ _SIGS = {
     'drp_batch_interpolate_bwd': (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
                                           c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                           c_size]),
+    'drp_batch_interpolate_rasterize_bwd_workspace_size': (c_size, [c_void_p, c_int]),
+    'drp_batch_interpolate_rasterize_bwd': (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p,
+                                                    c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                                    c_void_p, c_void_p, c_void_p, c_size]),
     'drp_batch_rasterize_bwd_workspace_size': (c_size, [c_void_p]),
     'drp_batch_rasterize_bwd': (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
                                         c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,

@@ -144,6 +144,7 @@ class _RasterizeBatchFunc(torch.autograd.Function):
         fctx.save_for_backward(pos, tri, rast, boxes)
         fctx.plan = plan
         fctx.want_db = want_db
+        fctx.set_materialize_grads(False)  # a gradient nothing sends arrives as None, not as zeros
         if db is None:
             db = torch.empty((plan.n_pix, 0), dtype=torch.float32, device=pos.device)
         return rast, db
@@ -152,10 +153,10 @@ class _RasterizeBatchFunc(torch.autograd.Function):
     def backward(fctx, grast, gdb):
         pos, tri, rast, boxes = fctx.saved_tensors
         plan = fctx.plan
-        if grast is None:
-            return torch.zeros_like(pos), None, None, None, None
-        grast = nvdr._as_f32(grast, 'grad_rast')
         gdb = nvdr._as_f32(gdb, 'grad_db') if (fctx.want_db and gdb is not None and gdb.numel()) else None
+        if grast is None and gdb is None:
+            return None, None, None, None, None
+        grast = torch.zeros_like(rast) if grast is None else nvdr._as_f32(grast, 'grad_rast')
         tri_g = _global_tri(plan, tri)
         inc = nvdr.incidence(tri_g, plan.n_vert)
         return _rasterize_bwd(plan, pos, tri_g, rast, True, grast, gdb, boxes, inc), None, None, None, None
@@ -332,7 +333,12 @@ class UnevenBatch:
         pos = nvdr._as_f32(pos, 'pos')
         if pos.shape != (self.plan.n_vert, 4):
             raise ValueError(f'pos must be [{self.plan.n_vert}, 4] (got {tuple(pos.shape)})')
-        return _URasterize.apply(pos, self)
+        rast, boxes = _URasterize.apply(pos, self)
+        if rast.requires_grad:
+            # the positions it rasterised: an interpolate of this rast can run the
+            # rasterize backward in its own pass over the pixels
+            rast._drp_upos = (self, pos, rast._version)
+        return rast, boxes
 
     def interpolate(self, attr: torch.Tensor, rast: torch.Tensor, boxes: torch.Tensor) -> torch.Tensor:
         """[n_pix, C] attributes; attr [sum V_i, C]."""
@@ -340,7 +346,10 @@ class UnevenBatch:
         attr = nvdr._as_f32(attr, 'attr')
         if attr.dim() != 2 or attr.shape[0] != self.plan.n_vert:
             raise ValueError(f'attr must be [{self.plan.n_vert}, C] (got {tuple(attr.shape)})')
-        return _UInterpolate.apply(attr, nvdr._as_f32(rast, 'rast'), boxes, self)
+        meta = getattr(rast, '_drp_upos', None)
+        src_pos = meta[1] if (meta is not None and meta[0] is self and meta[2] == rast._version
+                              and attr.shape[1] in _FUSED_C) else None
+        return _UInterpolate.apply(attr, nvdr._as_f32(rast, 'rast'), boxes, self, src_pos)
 
     def antialias(self, color: torch.Tensor, rast: torch.Tensor, pos: torch.Tensor,
                   pos_gradient_boost: float = 1.0) -> torch.Tensor:
@@ -360,6 +369,9 @@ class _URasterize(torch.autograd.Function):
         fctx.save_for_backward(pos, rast, boxes)
         fctx.ub = ub
         fctx.mark_non_differentiable(boxes)
+        # no gradient reaching rast (its consumers fused or non-differentiable)
+        # arrives as None, not as zeros to run the backward over
+        fctx.set_materialize_grads(False)
         return rast, boxes
 
     @staticmethod
@@ -367,38 +379,54 @@ class _URasterize(torch.autograd.Function):
         pos, rast, boxes = fctx.saved_tensors
         ub = fctx.ub
         if grast is None:
-            return torch.zeros_like(pos), None
+            return None, None
         grast = nvdr._as_f32(grast, 'grad_rast')
         return _rasterize_bwd(ub.plan, pos, ub.tri, rast, False, grast, None, boxes, ub.inc), None
 
 
+_FUSED_C = (4, 8)  # channel counts of the fused attribute + position backward (tests switch it off)
+
+
 class _UInterpolate(torch.autograd.Function):
     @staticmethod
-    def forward(fctx, attr, rast, boxes, ub):
+    def forward(fctx, attr, rast, boxes, ub, src_pos):
         C = attr.shape[1]
         out = torch.empty((ub.plan.n_pix, C), dtype=torch.float32, device=attr.device)
         check(_lib.lib().drp_batch_interpolate_fwd(stream_of(attr), ub.plan.host_ptr(), ptr(attr), C, ptr(rast),
                                                    ptr(ub.tri), ptr(out)), 'batch interpolate')
-        fctx.save_for_backward(attr, rast, boxes)
+        fctx.save_for_backward(attr, rast, boxes, src_pos)
         fctx.ub = ub
         return out
 
     @staticmethod
     def backward(fctx, gout):
-        attr, rast, boxes = fctx.saved_tensors
+        attr, rast, boxes, src_pos = fctx.saved_tensors
         ub = fctx.ub
         C = attr.shape[1]
         gout = nvdr._as_f32(gout, 'grad_out')
         gattr = torch.zeros_like(attr)
-        grast = torch.empty_like(rast)
         L = _lib.lib()
+        if src_pos is not None and fctx.needs_input_grad[4]:
+            # the rasterize backward of this interpolation's grad_rast in the same
+            # pass, returned as the gradient of the rasterised positions (the
+            # rasterize backward, linear in grad_rast, sees only rast's other consumers)
+            gpos = torch.zeros_like(src_pos)
+            ws_bytes = L.drp_batch_interpolate_rasterize_bwd_workspace_size(ub.plan.host_ptr(), C)
+            ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=attr.device)
+            check(L.drp_batch_interpolate_rasterize_bwd(stream_of(attr), ub.plan.host_ptr(), ptr(ub.plan.dev),
+                                                        ptr(attr), C, ptr(src_pos), ptr(rast), ptr(ub.tri),
+                                                        ptr(gout), ptr(gattr), ptr(gpos), ptr(boxes),
+                                                        ptr(ub.inc[0]), ptr(ub.inc[1]), ptr(ws), ws_bytes),
+                  'batch interpolate + rasterize backward')
+            return gattr, None, None, None, gpos
+        grast = torch.empty_like(rast)
         ws_bytes = L.drp_batch_interpolate_bwd_workspace_size(ub.plan.host_ptr(), C)
         ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=attr.device)
         check(L.drp_batch_interpolate_bwd(stream_of(attr), ub.plan.host_ptr(), ptr(ub.plan.dev), ptr(attr), C,
                                           ptr(rast), ptr(ub.tri), ptr(gout), ptr(gattr), ptr(grast), ptr(boxes),
                                           ptr(ub.inc[0]), ptr(ub.inc[1]), ptr(ws), ws_bytes),
               'batch interpolate backward')
-        return gattr, grast, None, None
+        return gattr, grast, None, None, None
 
 
 class _UAntialias(torch.autograd.Function):

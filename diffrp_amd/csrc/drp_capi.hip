@@ -561,6 +561,83 @@ int drp_batch_interpolate_bwd(void *stream, const void *plan, const void *plan_d
     return 0;
 }
 
+// the attribute and position backward of an uneven batch from one pass over
+// its pixels (the batched form of drp_interpolate_rasterize_bwd)
+size_t drp_batch_interpolate_rasterize_bwd_workspace_size(const void *plan, int C) {
+    if (!plan || C <= 0) return 0;
+    BatchPlanHeader h;
+    memcpy(&h, plan, sizeof(h));
+    if (h.n_tri <= 0) return 0;
+    long long tiles = plan_map(plan, h.n_img, h.n_tiles).bt16[h.n_img];
+    return align256((size_t)h.n_tri * 3 * (size_t)C * 4) + align256((size_t)h.n_tri * 3 * 16) +
+           tile_records_bytes_tiles(tiles, 3 * C + 12, h.n_tri) + align256((size_t)h.n_tri * 3 * 4);
+}
+
+int drp_batch_interpolate_rasterize_bwd(void *stream, const void *plan, const void *plan_dev, const float *attr,
+                                        int C, const float *pos, const float *rast, const int32_t *tri,
+                                        const float *grad_out, float *grad_attr, float *grad_pos,
+                                        const int32_t *boxes, const int32_t *inc_off, const int32_t *inc,
+                                        void *workspace, size_t workspace_bytes) {
+    PlanView v;
+    if (int e = plan_view(plan, plan_dev, v)) return e;
+    if (!(C == 4 || C == 8)) return fail(-1, "the fused backward takes C in {4, 8} (got %d)", C);
+    if (v.h.n_pix == 0 || v.h.n_tri == 0 || v.h.n_vert == 0) return 0;
+    if (!attr || !pos || !rast || !tri || !grad_out || !grad_attr || !grad_pos || !boxes || !inc_off || !inc)
+        return fail(-1, "null pointer argument");
+    if (!aligned16(attr) || !aligned16(pos) || !aligned16(rast) || !aligned16(grad_out) || !aligned16(grad_pos))
+        return fail(-1, "attr, pos, rast, grad_out and grad_pos must be 16-byte aligned");
+    const size_t need = drp_batch_interpolate_rasterize_bwd_workspace_size(plan, C);
+    if (!workspace || workspace_bytes < need)
+        return fail(-1, "workspace too small: need %zu bytes, got %zu", need, workspace_bytes);
+    const int V = (int)v.h.n_vert, F = (int)v.h.n_tri;
+    char *b = (char *)workspace;
+    float *partial_attr = (float *)b;
+    b += align256((size_t)F * 3 * (size_t)C * 4);
+    float *partial_pos = (float *)b;
+    b += align256((size_t)F * 3 * 16);
+    void *records = b;
+    b += tile_records_bytes_tiles(v.bt16, 3 * C + 12, F);
+    int *slot = (int *)b;
+    hipStream_t st = (hipStream_t)stream;
+    if (int rc = launch_corner_slots(inc, inc_off, V, F, slot, st))
+        return fail(rc, "corner slots launch failed: %s", hipGetErrorString(hipGetLastError()));
+    TriBwdLaunch T{};
+    T.b.n_img = 1;  // one image of packed items: the item map names each tile's item
+    T.b.V = V;
+    T.b.F = F;
+    T.b.H = 1;
+    T.b.W = 1;
+    T.b.pos_batched = 0;
+    T.pos = pos;
+    T.tri = tri;
+    T.rast = rast;
+    T.grad_out = grad_out;
+    T.attr = attr;
+    T.V = V;
+    T.attr_batched = 0;
+    T.grad_rast_out = nullptr;
+    T.boxes = (const ItemBox *)boxes;
+    T.partial = partial_attr;
+    T.partial_pos = partial_pos;
+    T.corner_slot = slot;
+    T.n_img = 1;
+    T.F = F;
+    T.H = 1;
+    T.W = 1;
+    T.views_per_block = 1;
+    T.um = v.um;
+    T.local_ids = 0;
+    tile_records_layout_tiles(T, records, v.bt16, F);
+    int rc = launch_tri_bwd_fused(T, C, st);
+    Fork f{};  // the two vertex gathers are independent
+    const bool par = !rc && fork_begin(st, f);
+    if (!rc) rc = launch_gather_attr(partial_attr, 1, F, V, C, inc_off, inc, grad_attr, par ? f.side : st);
+    if (!rc) rc = launch_gather_pos(partial_pos, 1, F, V, 0, inc_off, grad_pos, st);
+    if (par && fork_end(st, f)) rc = -2;
+    if (rc) return fail(rc, "batch fused backward launch failed: %s", hipGetErrorString(hipGetLastError()));
+    return 0;
+}
+
 size_t drp_batch_rasterize_bwd_workspace_size(const void *plan) {
     if (!plan) return 0;
     BatchPlanHeader h;

@@ -144,6 +144,18 @@ int drp_batch_rasterize_bwd(void *stream, const void *plan, const void *plan_dev
                             const float *rast, int local_ids, const float *grad_rast, const float *grad_db,
                             float *grad_pos, const int32_t *boxes, const int32_t *inc_off, const int32_t *inc,
                             void *workspace, size_t workspace_bytes);
+/* The attribute and position backward of an uneven batch from one pass over
+ * its pixels: drp_batch_interpolate_bwd and drp_batch_rasterize_bwd (global
+ * ids, no grad_db) of the grad_rast the first would write, without writing
+ * it (the batched form of drp_interpolate_rasterize_bwd); grad_attr and
+ * grad_pos accumulated (caller zero-fills).  C in {4, 8}; pos [n_vert, 4] as
+ * rasterized, tri the batch triangles (global vertex indices). */
+size_t drp_batch_interpolate_rasterize_bwd_workspace_size(const void *plan, int C);
+int drp_batch_interpolate_rasterize_bwd(void *stream, const void *plan, const void *plan_dev, const float *attr,
+                                        int C, const float *pos, const float *rast, const int32_t *tri,
+                                        const float *grad_out, float *grad_attr, float *grad_pos,
+                                        const int32_t *boxes, const int32_t *inc_off, const int32_t *inc,
+                                        void *workspace, size_t workspace_bytes);
 size_t drp_batch_antialias_workspace_size(const void *plan);
 int drp_batch_antialias_fwd(void *stream, const void *plan, const void *plan_dev, const float *color, int C,
                             const float *rast, const float *pos, const int32_t *tri, const int32_t *opp, float *out,

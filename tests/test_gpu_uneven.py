@@ -157,3 +157,48 @@ def test_uneven_batch_full_size_equals_per_item_calls(cuda):
             scale = float(w.abs().max())
             err = float((got[vb:vb + V] - w).abs().max())
             assert err <= 1e-5 * max(scale, 1e-30), f'item {i} {what}: {err:.3e} of {scale:.3e}'
+
+
+@pytest.mark.parametrize('C', [4, 8])
+def test_uneven_fused_backward_equals_separate_passes(cuda, C):
+    """The batch's attribute and position backward from one pass over the
+    pixels (drp_batch_interpolate_rasterize_bwd) equals the two batch passes,
+    bit for bit, with antialias and its position gradient on top."""
+    print(f'case: 6 meshes s=2..5, C={C}', flush=True)
+    pos, tri = _scene(SMALL_S, SMALL_RES, 0.02, 17)
+    grads = []
+    for fused in (True, False):
+        saved = batch._FUSED_C
+        batch._FUSED_C = saved if fused else ()
+        try:
+            ub, P, A, T = _build(pos, tri, SMALL_RES, C, 18)
+            _step(ub, P, A, T)
+            torch.cuda.synchronize()
+            grads.append((P.grad.clone(), A.grad.clone()))
+        finally:
+            batch._FUSED_C = saved
+    (gp_f, ga_f), (gp_s, ga_s) = grads
+    assert torch.equal(ga_f, ga_s)
+    # the antialias position gradient is summed with float atomics; the rest is bit-exact
+    assert float((gp_f - gp_s).abs().max()) <= 1e-5 * float(gp_s.abs().max())
+    assert float(gp_f.abs().max()) > 0
+
+
+def test_uneven_fused_backward_without_antialias_is_bit_exact(cuda):
+    pos, tri = _scene(SMALL_S, SMALL_RES, 0.02, 19)
+    grads = []
+    for fused in (True, False):
+        saved = batch._FUSED_C
+        batch._FUSED_C = saved if fused else ()
+        try:
+            ub, P, A, T = _build(pos, tri, SMALL_RES, 8, 20)
+            rast, boxes = ub.rasterize(P)
+            col = ub.interpolate(A, rast, boxes)
+            l2_loss(col, T).backward()
+            torch.cuda.synchronize()
+            grads.append((P.grad.clone(), A.grad.clone()))
+        finally:
+            batch._FUSED_C = saved
+    (gp_f, ga_f), (gp_s, ga_s) = grads
+    assert torch.equal(ga_f, ga_s) and torch.equal(gp_f, gp_s)
+    assert float(gp_f.abs().max()) > 0
