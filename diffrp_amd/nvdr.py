@@ -632,7 +632,10 @@ def _cached_topology(tri: torch.Tensor):
     return th
 
 
-_HAND_AA_POS = True  # hand the antialias position gradient to a fused interpolation (tests switch it off)
+# hand the antialias position gradient to a fused interpolation (tests switch it
+# off); it needs autograd's id of the running backward pass to tell passes apart
+_graph_task_id = getattr(torch._C, '_current_graph_task_id', None)
+_HAND_AA_POS = _graph_task_id is not None
 
 
 def _fused_host(color, pos, limit=64):
@@ -658,7 +661,7 @@ def _take_aa_pos(fctx):
     backward pass (None if none, or if it was left by an earlier pass that did
     not reach this node)."""
     dep, fctx._drp_aa_pos = fctx._drp_aa_pos, None
-    return dep[0] if dep is not None and dep[1] == torch._C._current_graph_task_id() else None
+    return dep[0] if dep is not None and dep[1] == _graph_task_id() else None
 
 
 class _AntialiasFunc(torch.autograd.Function):
@@ -704,7 +707,7 @@ class _AntialiasFunc(torch.autograd.Function):
             # with a fused interpolation of pos upstream, accumulate into the
             # gradient already left there in this pass (another antialias), and
             # leave the sum for that interpolation's backward to add to
-            task = torch._C._current_graph_task_id()
+            task = _graph_task_id() if host is not None else None
             dep = host._drp_aa_pos if host is not None else None
             gpos = dep[0] if dep is not None and dep[1] == task else torch.zeros_like(pos)
         L = _lib.lib()
