@@ -17,6 +17,19 @@ namespace drp {
 // ----------------------------------------------------------------------------
 __device__ __forceinline__ double dfma(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
+// 1/x in double to within about an ulp: the hardware reciprocal estimate
+// (~26 bits) refined by two Newton steps, the refinement that the compiler's
+// correctly rounded division also uses, without its scaling and fix-up for
+// exponents near the ends of the range (s and wn of a covered pixel are far
+// from them; x = 0 gives a non-finite result, as the division does).
+__device__ __forceinline__ double drcp(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = dfma(-x, r, 1.0);
+    r = dfma(r, e, r);
+    e = dfma(-x, r, 1.0);
+    return dfma(r, e, r);
+}
+
 // NDC centre of pixel column c (row r) in double: (2c + 1 - W) / W
 __device__ __forceinline__ double pix_ndc_d(int c, int W) { return (double)(2 * c + 1 - W) / (double)W; }
 
@@ -38,7 +51,7 @@ __device__ __forceinline__ void raster_pixel_grad_at(const TriVerts &v, double p
     const double s = (e[0] + e[1]) + e[2];
     const double zn = dfma(e[0], zk[0], dfma(e[1], zk[1], e[2] * zk[2]));
     const double wn = dfma(e[0], wk[0], dfma(e[1], wk[1], e[2] * wk[2]));
-    const double inv_s = 1.0 / s, inv_wn = 1.0 / wn;
+    const double inv_s = drcp(s), inv_wn = drcp(wn);
     const double u = e[0] * inv_s, vv = e[1] * inv_s, zw = zn * inv_wn;
     double gu = g.x, gv = g.y, gz = g.z;
     double gX[3] = {0.0, 0.0, 0.0}, gY[3] = {0.0, 0.0, 0.0}, gw[3] = {0.0, 0.0, 0.0}, gs = 0.0;
