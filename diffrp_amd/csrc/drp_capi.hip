@@ -157,6 +157,7 @@ static int raster_fwd(void *stream, const float *pos, int n_img, int V, int pos_
     L.rast = rast;
     L.rast_db = rast_db;
     L.aa_prepare = aa;
+    L.depth_cull = depth_cull_wanted(bin_cap, tile_w, tile_h);
     int rc = launch_rasterize(L, (hipStream_t)stream);
     if (rc) return fail(rc, "rasterize launch failed: %s", hipGetErrorString(hipGetLastError()));
     return 0;
@@ -446,6 +447,12 @@ int drp_rasterize_batch_fwd(void *stream, const float *pos, const int32_t *tri, 
     L.prev = prev;
     L.rast = rast;
     L.rast_db = rast_db;
+    {
+        const ImageDesc *D = (const ImageDesc *)((const char *)plan + plan_desc_off());
+        int cap = 0;
+        for (int i = 0; i < h.n_img; ++i) cap = D[i].bin_cap > cap ? D[i].bin_cap : cap;
+        L.depth_cull = depth_cull_wanted(cap, h.tile_w, h.tile_h);
+    }
     int rc = launch_rasterize(L, (hipStream_t)stream);
     if (rc) return fail(rc, "batch rasterize launch failed: %s", hipGetErrorString(hipGetLastError()));
     return 0;

@@ -68,7 +68,16 @@ struct RasterLaunch {
     float *rast;
     float *rast_db;
     const AALaunch *aa_prepare;  // optional: the antialias preparation, run beside the fine raster
+    int depth_cull;   // fine raster with the depth cull (depth_cull_wanted)
 };
+
+// The fine raster culls triangles by depth where bins hold more than
+// kCullRounds rounds of a tile's lanes (a bin capacity learnt or tuned from
+// the exact counts is the largest count + 15 %, rounded up to a power of two).
+constexpr int kCullRounds = 4;
+inline int depth_cull_wanted(int bin_cap, int tile_w, int tile_h) {
+    return bin_cap > kCullRounds * tile_w * tile_h ? 1 : 0;
+}
 
 struct RasterBwdLaunch {
     Batch b;

@@ -277,6 +277,69 @@ __device__ __forceinline__ float inv_ord_f32(uint32_t k) {
     return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
 }
 
+// Depth cull of a direct-class triangle against a tile whose every pixel
+// already holds a fragment: true when no pixel centre of the box
+// [px0, px1] x [py0, py1] can receive from this triangle a depth z/w at or
+// below zcut, the farthest of the tile's current nearest depths, so that the
+// triangle cannot change any depth key (the test is a strict >, whatever the
+// triangle index).
+//
+// With X_k = x_k - w_k px, Y_k = y_k - w_k py and e_k = X_i Y_j - X_j Y_i
+// ((i, j) = (k+1, k+2) mod 3), wn* = sum e_k w_k is the determinant of the
+// rows (x_k, y_k, w_k), the same at every pixel, and zn* = sum e_k z_k is
+// affine in (px, py), so the exact depth zn* / wn* is affine over the box:
+// its minimum is at a corner.  eval_pixel's float zn and wn (X, Y one
+// rounding each; e_k two products and an fma: 4.02 u T_k; the nested fmas:
+// 3.01 u sum |e_k z_k|) are within Ez = 7.04 u sum T_k |z_k| and
+// Ew = 7.04 u sum T_k w_k of the exact ones at any pixel centre of the box,
+// with u = 2^-24 and T_k = |X_i Y_j| + |X_j Y_i| bounded through the box's
+// end columns and rows (X_k is affine in px, Y_k in py); 8 u in place of
+// 7.04 u covers the roundings of the bound itself.  The corners are evaluated
+// with eval_pixel too, and their depths taken as zn * rcp(wn) (within 4 u of
+// zn / wn).  With W0 = |wn(corner 0)| - Ew <= |wn*| > 3 Ew (slivers are never
+// culled), every |wn| >= W0 - Ew > 0; the exact depth over the box is at most
+// ZB = 2 (QA (1 + 5 u) + Ez / (W0 - Ew)) in magnitude (QA: the corners'
+// largest |depth|), a pixel's computed depth is within E + u (ZB + E) of the
+// exact one, E = (Ez + ZB Ew) / (W0 - Ew), and the exact minimum is within
+// E + 4 u QA (1 + 5 u) of the corners' smallest depth: the margin below
+// (2 E + 8 u QA + 2 u (ZB + E)) exceeds the sum.
+__device__ __forceinline__ bool depth_culled(const TriVerts &v, float px0, float px1, float py0, float py1,
+                                             float zcut) {
+    constexpr float u = 0x1p-24f;
+    float ez = 0.f, ew = 0.f;
+    {
+        float MX[3], MY[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            MX[k] = fmaxf(fabsf(fmaf(-v.w[k], px0, v.x[k])), fabsf(fmaf(-v.w[k], px1, v.x[k])));
+            MY[k] = fmaxf(fabsf(fmaf(-v.w[k], py0, v.y[k])), fabsf(fmaf(-v.w[k], py1, v.y[k])));
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const int i = k == 2 ? 0 : k + 1, j = k == 0 ? 2 : k - 1;
+            const float T = fmaf(MX[i], MY[j], MX[j] * MY[i]);
+            ez = fmaf(T, fabsf(v.z[k]), ez);
+            ew = fmaf(T, v.w[k], ew);
+        }
+        ez *= 8.f * u;
+        ew *= 8.f * u;
+    }
+    float qmin = INFINITY, qa = 0.f, w0 = 0.f;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        PixEval p;
+        eval_pixel(v, (c & 1) ? px1 : px0, (c & 2) ? py1 : py0, p);
+        if (c == 0) w0 = fabsf(p.wn) - ew;
+        const float q = p.zn * __builtin_amdgcn_rcpf(p.wn);
+        qmin = fminf(qmin, q);
+        qa = fmaxf(qa, fabsf(q));
+    }
+    if (!(w0 > 3.f * ew) || !(qa < INFINITY)) return false;
+    const float zb = 2.f * (qa * (1.f + 5.f * u) + ez / (w0 - ew));
+    const float E = (ez + zb * ew) / (w0 - ew);
+    return qmin - (2.f * E + 8.f * u * qa + 2.f * u * (zb + E)) > zcut;
+}
+
 // LDS of one fine-raster block
 template <int TW, int TH, bool PEEL>
 struct FineLds {
@@ -289,6 +352,7 @@ struct FineLds {
     unsigned long long pkey[PEEL ? NT : 1];
     float pxs[TW], pys[TH];             // NDC centres of the tile's columns and rows (one division each per tile)
     int wsum[3 * (NT / 64)];            // scan partials: records; row spans (two, alternating)
+    float zred[NT / 64];                // per-wave maxima of the current nearest depths
     int bcast;                          // block-wide broadcast slot
 };
 
@@ -363,8 +427,9 @@ __device__ __forceinline__ FineTile fine_begin(FineLds<TW, TH, PEEL> &S, const B
     return T;
 }
 
-// depth test of the tile's entries [e0, e1) into S.zbuf
-template <int TW, int TH, bool PEEL>
+// depth test of the tile's entries [e0, e1) into S.zbuf; with CULL, after the
+// first round, direct-class triangles are culled against the tile's depths
+template <int TW, int TH, bool PEEL, bool CULL>
 __device__ __forceinline__ void fine_entries(FineLds<TW, TH, PEEL> &S, const FineTile &T, int e0, int e1,
                                              const float *__restrict__ pos, const int *__restrict__ tri,
                                              const int *__restrict__ bins, const int *__restrict__ lists,
@@ -376,6 +441,24 @@ __device__ __forceinline__ void fine_entries(FineLds<TW, TH, PEEL> &S, const Fin
     const int tc0 = T.tc0, tr0 = T.tr0;
     int par = 0;  // scan partials of the row spans in use; alternates every sub-round
     for (int base = e0; base < e1; base += NT) {
+        // After the first round: the farthest of the tile's current nearest
+        // depths (infinite while a pixel of the image has no fragment yet),
+        // against which direct-class triangles are culled (depth_culled).
+        float zcut = INFINITY;
+        if (CULL && base > e0) {
+            float zt = -INFINITY;
+            if ((tid % TW) < T.tw_img && (tid / TW) < T.th_img) {
+                const unsigned long long key = S.zbuf[tid];
+                zt = key == ~0ull ? INFINITY : inv_ord_f32((uint32_t)(key >> 32));
+            }
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) zt = fmaxf(zt, __shfl_xor(zt, o, 64));
+            if ((tid & 63) == 0) S.zred[tid >> 6] = zt;
+            __syncthreads();
+#pragma unroll
+            for (int w = 0; w < NT / 64; ++w) zt = fmaxf(zt, S.zred[w]);
+            zcut = zt;
+        }
         int idx = base + tid;
         int count = 0;
         if (idx < e1) {
@@ -431,6 +514,13 @@ __device__ __forceinline__ void fine_entries(FineLds<TW, TH, PEEL> &S, const Fin
                     }
                 }
             }
+        }
+        if (CULL && count > 0 && zcut < INFINITY) {  // the lane's own record, read back from LDS
+            const FineRec &R = S.rec[tid];
+            const int box = R.box, x0 = rec_x0(box), y0 = rec_y0(box);
+            if (rec_cls(box) == kDirect &&
+                depth_culled(R.v, S.pxs[x0], S.pxs[rec_x1(box)], S.pys[y0], S.pys[y0 + count - 1], zcut))
+                count = 0;
         }
         int nrows;
         const int ex = block_excl_scan<NT>(count, S.wsum, nrows);  // this lane's record owns rows [ex, ex + count)
@@ -550,7 +640,7 @@ __device__ __forceinline__ SplitWs split_ws(int *list_count, int n_img, char *ba
 }
 static_assert(kMaxFineNT == 256, "split_region_bytes assumes 256 keys per slot");
 
-template <int TW, int TH, bool PEEL, bool DB>
+template <int TW, int TH, bool PEEL, bool DB, bool CULL>
 __global__ void __launch_bounds__(TW *TH, kFineWavesPerEU) k_fine(Batch b, const float *__restrict__ pos,
                                                  const int *__restrict__ tri,
                                                  const int *__restrict__ bin_count, const int *__restrict__ bins,
@@ -587,7 +677,7 @@ __global__ void __launch_bounds__(TW *TH, kFineWavesPerEU) k_fine(Batch b, const
         slot = __builtin_amdgcn_readfirstlane(S.bcast);
         if (slot >= 0) e1 = per;
     }
-    fine_entries<TW, TH, PEEL>(S, T, 0, e1, pos, tri, bins, lists, setup);
+    fine_entries<TW, TH, PEEL, CULL>(S, T, 0, e1, pos, tri, bins, lists, setup);
     if (slot >= 0) {  // chunk 0 of a split tile: its keys seed the merge
         const SplitWs sp = split_ws(list_count, b.n_img, split_base, split_slots);
         sp.gkey[(long long)slot * kMaxFineNT + threadIdx.x] = S.zbuf[threadIdx.x];
@@ -598,7 +688,7 @@ __global__ void __launch_bounds__(TW *TH, kFineWavesPerEU) k_fine(Batch b, const
 }
 
 // chunks 1.. of split tiles: a fixed grid drains the queue k_fine filled
-template <int TW, int TH, bool PEEL, bool DB>
+template <int TW, int TH, bool PEEL, bool DB, bool CULL>
 __global__ void __launch_bounds__(TW *TH, kFineWavesPerEU) k_fine_split(Batch b, const float *__restrict__ pos,
                                                  const int *__restrict__ tri,
                                                  const int *__restrict__ bin_count, const int *__restrict__ bins,
@@ -625,7 +715,7 @@ __global__ void __launch_bounds__(TW *TH, kFineWavesPerEU) k_fine_split(Batch b,
                   nch = __builtin_amdgcn_readfirstlane(inf.z);
         const FineTile T = fine_begin<TW, TH, PEEL>(S, b, itile, bin_count, list_count, prev);
         const int e0 = chunk * per, e1 = min(e0 + per, T.total);
-        fine_entries<TW, TH, PEEL>(S, T, e0, e1, pos, tri, bins, lists, setup);
+        fine_entries<TW, TH, PEEL, CULL>(S, T, e0, e1, pos, tri, bins, lists, setup);
         unsigned long long *g = sp.gkey + (long long)slot * kMaxFineNT + tid;
         const unsigned long long mine = S.zbuf[tid];
         if (mine != ~0ull) atomicMin(g, mine);
@@ -771,9 +861,8 @@ int launch_count_covered(const float *rast, int n_img, long long npx, int *count
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
-template <int TW, int TH, bool PEEL, bool DB>
-static void launch_fine_v(const RasterLaunch &L, hipStream_t st) {
-    if (L.n_tiles <= 0) return;
+template <int TW, int TH, bool PEEL, bool DB, bool CULL>
+static void launch_fine_c(const RasterLaunch &L, hipStream_t st) {
     // Eight or more images of one size: each XCD takes one contiguous eighth of
     // the tiles (whole images, so its L2 holds one image's triangles and setup
     // records at a time).  Otherwise groups of kFineXcdGroup tiles are dealt
@@ -782,16 +871,26 @@ static void launch_fine_v(const RasterLaunch &L, hipStream_t st) {
     const int group = eighths ? max(1, (L.n_tiles + 7) / 8) : kFineXcdGroup;
     const long long unit = 8ll * group;
     const int pad = (int)((L.n_tiles + unit - 1) / unit * unit);
-    hipLaunchKernelGGL((k_fine<TW, TH, PEEL, DB>), dim3(pad), dim3(TW * TH), 0, st, L.b, L.pos, L.tri,
+    hipLaunchKernelGGL((k_fine<TW, TH, PEEL, DB, CULL>), dim3(pad), dim3(TW * TH), 0, st, L.b, L.pos, L.tri,
                        L.bin_count, L.bins, L.list_count, L.lists, L.setup, L.n_tiles, group, L.split_base,
                        L.split_slots, (const float4 *)L.prev, (float4 *)L.rast, (float4 *)L.rast_db, L.boxes);
     if (L.split_slots > 0) {
         // six blocks of 256 lanes per CU, whatever the tile
         const int grid = 6 * 256 * (256 / (TW * TH));
-        hipLaunchKernelGGL((k_fine_split<TW, TH, PEEL, DB>), dim3(grid), dim3(TW * TH), 0, st, L.b, L.pos, L.tri,
+        hipLaunchKernelGGL((k_fine_split<TW, TH, PEEL, DB, CULL>), dim3(grid), dim3(TW * TH), 0, st, L.b, L.pos, L.tri,
                            L.bin_count, L.bins, L.list_count, L.lists, L.setup, L.split_base, L.split_slots,
                            (const float4 *)L.prev, (float4 *)L.rast, (float4 *)L.rast_db, L.boxes);
     }
+}
+
+// The culling kernels only where tiles hold many rounds of entries: in
+// one-round tiles the cull never runs, yet its registers and code slow the
+// fine raster slightly.
+template <int TW, int TH, bool PEEL, bool DB>
+static void launch_fine_v(const RasterLaunch &L, hipStream_t st) {
+    if (L.n_tiles <= 0) return;
+    if (L.depth_cull) launch_fine_c<TW, TH, PEEL, DB, true>(L, st);
+    else launch_fine_c<TW, TH, PEEL, DB, false>(L, st);
 }
 
 template <int TW, int TH>
