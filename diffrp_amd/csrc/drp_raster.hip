@@ -277,12 +277,12 @@ __device__ __forceinline__ float inv_ord_f32(uint32_t k) {
     return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
 }
 
-// Depth cull of a direct-class triangle against a tile whose every pixel
+// Depth cull of a direct-class triangle against tile rows whose every pixel
 // already holds a fragment: true when no pixel centre of the box
 // [px0, px1] x [py0, py1] can receive from this triangle a depth z/w at or
-// below zcut, the farthest of the tile's current nearest depths, so that the
-// triangle cannot change any depth key (the test is a strict >, whatever the
-// triangle index).
+// below zcut, the farthest current nearest depth over the box's rows, so that
+// the triangle cannot change any depth key (the test is a strict >, whatever
+// the triangle index).
 //
 // With X_k = x_k - w_k px, Y_k = y_k - w_k py and e_k = X_i Y_j - X_j Y_i
 // ((i, j) = (k+1, k+2) mod 3), wn* = sum e_k w_k is the determinant of the
@@ -352,7 +352,7 @@ struct FineLds {
     unsigned long long pkey[PEEL ? NT : 1];
     float pxs[TW], pys[TH];             // NDC centres of the tile's columns and rows (one division each per tile)
     int wsum[3 * (NT / 64)];            // scan partials: records; row spans (two, alternating)
-    float zred[NT / 64];                // per-wave maxima of the current nearest depths
+    float zrow[TH];                     // per tile row: the farthest current nearest depth (depth cull)
     int bcast;                          // block-wide broadcast slot
 };
 
@@ -428,12 +428,13 @@ __device__ __forceinline__ FineTile fine_begin(FineLds<TW, TH, PEEL> &S, const B
 }
 
 // depth test of the tile's entries [e0, e1) into S.zbuf; with CULL, after the
-// first round, direct-class triangles are culled against the tile's depths
+// first round (from the first when S.zbuf was seeded with real keys),
+// direct-class triangles are culled against the tile's depths
 template <int TW, int TH, bool PEEL, bool CULL>
 __device__ __forceinline__ void fine_entries(FineLds<TW, TH, PEEL> &S, const FineTile &T, int e0, int e1,
                                              const float *__restrict__ pos, const int *__restrict__ tri,
                                              const int *__restrict__ bins, const int *__restrict__ lists,
-                                             const int4 *__restrict__ setup) {
+                                             const int4 *__restrict__ setup, bool seeded) {
     constexpr int NT = TW * TH;
     constexpr int kQueue = NT * kQueuePerLane;
     const ImageDesc &d = T.d;
@@ -441,23 +442,21 @@ __device__ __forceinline__ void fine_entries(FineLds<TW, TH, PEEL> &S, const Fin
     const int tc0 = T.tc0, tr0 = T.tr0;
     int par = 0;  // scan partials of the row spans in use; alternates every sub-round
     for (int base = e0; base < e1; base += NT) {
-        // After the first round: the farthest of the tile's current nearest
-        // depths (infinite while a pixel of the image has no fragment yet),
-        // against which direct-class triangles are culled (depth_culled).
-        float zcut = INFINITY;
-        if (CULL && base > e0) {
+        // After the first round: per tile row, the farthest of its current
+        // nearest depths (infinite while a pixel of the row has no fragment
+        // yet), against which direct-class triangles are culled over the rows
+        // of their box (depth_culled).
+        const bool cull_round = CULL && (base > e0 || seeded);
+        if (cull_round) {
             float zt = -INFINITY;
             if ((tid % TW) < T.tw_img && (tid / TW) < T.th_img) {
                 const unsigned long long key = S.zbuf[tid];
                 zt = key == ~0ull ? INFINITY : inv_ord_f32((uint32_t)(key >> 32));
             }
 #pragma unroll
-            for (int o = 32; o >= 1; o >>= 1) zt = fmaxf(zt, __shfl_xor(zt, o, 64));
-            if ((tid & 63) == 0) S.zred[tid >> 6] = zt;
+            for (int o = TW / 2; o >= 1; o >>= 1) zt = fmaxf(zt, __shfl_xor(zt, o, 64));  // a row: TW lanes
+            if (tid % TW == 0) S.zrow[tid / TW] = zt;
             __syncthreads();
-#pragma unroll
-            for (int w = 0; w < NT / 64; ++w) zt = fmaxf(zt, S.zred[w]);
-            zcut = zt;
         }
         int idx = base + tid;
         int count = 0;
@@ -515,10 +514,12 @@ __device__ __forceinline__ void fine_entries(FineLds<TW, TH, PEEL> &S, const Fin
                 }
             }
         }
-        if (CULL && count > 0 && zcut < INFINITY) {  // the lane's own record, read back from LDS
+        if (cull_round && count > 0) {  // the lane's own record, read back from LDS
             const FineRec &R = S.rec[tid];
             const int box = R.box, x0 = rec_x0(box), y0 = rec_y0(box);
-            if (rec_cls(box) == kDirect &&
+            float zcut = -INFINITY;
+            for (int y = y0; y < y0 + count; ++y) zcut = fmaxf(zcut, S.zrow[y]);
+            if (rec_cls(box) == kDirect && zcut < INFINITY &&
                 depth_culled(R.v, S.pxs[x0], S.pxs[rec_x1(box)], S.pys[y0], S.pys[y0 + count - 1], zcut))
                 count = 0;
         }
@@ -677,7 +678,7 @@ __global__ void __launch_bounds__(TW *TH, kFineWavesPerEU) k_fine(Batch b, const
         slot = __builtin_amdgcn_readfirstlane(S.bcast);
         if (slot >= 0) e1 = per;
     }
-    fine_entries<TW, TH, PEEL, CULL>(S, T, 0, e1, pos, tri, bins, lists, setup);
+    fine_entries<TW, TH, PEEL, CULL>(S, T, 0, e1, pos, tri, bins, lists, setup, false);
     if (slot >= 0) {  // chunk 0 of a split tile: its keys seed the merge
         const SplitWs sp = split_ws(list_count, b.n_img, split_base, split_slots);
         sp.gkey[(long long)slot * kMaxFineNT + threadIdx.x] = S.zbuf[threadIdx.x];
@@ -715,8 +716,14 @@ __global__ void __launch_bounds__(TW *TH, kFineWavesPerEU) k_fine_split(Batch b,
                   nch = __builtin_amdgcn_readfirstlane(inf.z);
         const FineTile T = fine_begin<TW, TH, PEEL>(S, b, itile, bin_count, list_count, prev);
         const int e0 = chunk * per, e1 = min(e0 + per, T.total);
-        fine_entries<TW, TH, PEEL, CULL>(S, T, e0, e1, pos, tri, bins, lists, setup);
         unsigned long long *g = sp.gkey + (long long)slot * kMaxFineNT + tid;
+        // With the cull, the chunk starts from the tile's keys merged so far
+        // (chunk 0's, stored before this kernel began, and any chunk merged
+        // since; an atomic load, so never half of one key and half of
+        // another): each is a real fragment's key, so the depth test and the
+        // cull see bounds of the final minimum from the first round.
+        if (CULL) S.zbuf[tid] = __hip_atomic_load(g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        fine_entries<TW, TH, PEEL, CULL>(S, T, e0, e1, pos, tri, bins, lists, setup, CULL);
         const unsigned long long mine = S.zbuf[tid];
         if (mine != ~0ull) atomicMin(g, mine);
         __threadfence();
