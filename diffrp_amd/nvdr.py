@@ -54,7 +54,10 @@ class RasterizeCudaContext:
     Scratch memory is taken from PyTorch's caching allocator per call, so one
     context may be shared by threads as diffrp's per-thread cache does.
     ``tile`` / ``bin_capacity`` = None lets the heuristic (or an autotuned
-    table installed with ``set_tuning``) choose per call shape.
+    table installed with ``set_tuning``) choose per call shape.  Both affect
+    speed only; a capacity above four rounds of a tile's pixels (learnt for
+    meshes whose tiles hold that many triangles) also selects fine-raster
+    kernels that cull triangles behind a tile's current depths.
 
     ``prepare_antialias=True`` (an option of this implementation): every
     single-layer ``rasterize`` of this context also prepares what a following

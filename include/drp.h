@@ -56,7 +56,9 @@ int drp_abi_version(void);
  *   prev: previous depth layer (depth peeling) or nullptr.
  *   rast_db: nullptr unless derivatives are wanted.
  *   tile_w x tile_h in {8x8, 16x8, 16x16, 32x8}; bin_cap >= 1 (performance
- *   only: full bins spill to a per-image list, a full list to a full scan).
+ *   only: full bins spill to a per-image list, a full list to a full scan;
+ *   above 4 x tile_w x tile_h the fine raster also culls triangles that lie
+ *   behind a tile's current depths, with the same results).
  *   boxes (nullable, int32 [n,F,3], 4-byte aligned): per (image, triangle) the
  *   packed pixel box (c0 | c1 << 16, r0 | r1 << 16), c0 > c1 when culled, and a
  *   hit flag, nonzero when the triangle won at least one pixel of the image;
